@@ -1,0 +1,18 @@
+#!/bin/bash
+# Build liborf_rcwa.so (gfx950) in-tree.  Used by __graft_entry__.build().
+set -euo pipefail
+HERE="$(cd "$(dirname "$0")" && pwd)"
+OUT="${HERE}/liborf_rcwa.so"
+HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
+FLAGS=(--offload-arch=gfx950 -O3 -fPIC -std=c++17 -Xclang -target-feature -Xclang -packed-fp32-ops
+       -munsafe-fp-atomics -Wno-unused-result)
+mkdir -p "${HERE}/_build"
+objs=()
+for src in la_kernels rcwa_kernels orf_plan; do
+  "${HIPCC}" "${FLAGS[@]}" -c "${HERE}/csrc/${src}.hip" -o "${HERE}/_build/${src}.o" &
+  objs+=("${HERE}/_build/${src}.o")
+done
+wait
+"${HIPCC}" --offload-arch=gfx950 -shared -o "${OUT}.tmp" "${objs[@]}"
+mv "${OUT}.tmp" "${OUT}"
+echo "built ${OUT}"

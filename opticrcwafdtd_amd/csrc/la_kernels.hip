@@ -1,0 +1,747 @@
+// Batched FP64 complex linear algebra for the RCWA solve on gfx950.
+//
+// These kernels replace the page-batched MATLAB builtins the reference's
+// solver is written with: pagemtimes (smatrixStar.m:5-10, smatrixLayer.m:13-14),
+// pagemldivide / pagemrdivide (smatrixLayer.m:6-14, smatrixStar.m:5-6) and
+// pageeig (modesLayer.m:2-5).  Every matrix is column-major interleaved
+// complex128 (double2), one "page" per batch entry at a fixed stride; a stride
+// of 0 broadcasts one matrix over the batch.
+//
+//  * zgemm_mfma:  64x64 output tile per 256-thread workgroup, 4 waves of 32x32,
+//                 v_mfma_f64_16x16x4f64 with separate real/imag accumulators.
+//  * zgetrf:      one workgroup per page, partial pivoting (max |re|+|im|, first
+//                 index on ties, as LAPACK izamax), matrix staged in LDS when it fits.
+//  * zgetrs:      left (A X = B) and right (X A = B) solves from the LU factors.
+//  * eig:         Householder Hessenberg reduction with the unitary factor
+//                 accumulated, complex single-shift QR (Wilkinson shift,
+//                 exceptional shifts, Ahues-Tisseur deflation) to Schur form,
+//                 then eigenvectors of the triangular factor by back
+//                 substitution (LAPACK ztrevc's small-pivot rule), back-
+//                 transformed by a ZGEMM and normalised to unit 2-norm.
+#include "orf_common.h"
+#include "orf_internal.h"
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// ============================================================== ZGEMM (MFMA)
+namespace {
+constexpr int GT_M = 64, GT_N = 64, GT_K = 16, GT_PAD = 2;
+
+template <int OP>
+ORF_DEV cplx load_op(const cplx* __restrict__ X, int ld, int r, int c) {
+  // element (r, c) of op(X)
+  if (OP == 0) return X[(size_t)r + (size_t)c * ld];
+  cplx v = X[(size_t)c + (size_t)r * ld];
+  if (OP == 2) v.y = -v.y;
+  return v;
+}
+
+template <int OPA, int OPB>
+__global__ __launch_bounds__(256) void zgemm_mfma_kernel(
+    int M, int N, int K, cplx alpha, const cplx* __restrict__ A, int lda, long long sA,
+    const cplx* __restrict__ B, int ldb, long long sB, cplx beta, cplx* __restrict__ C, int ldc,
+    long long sC) {
+  __shared__ double Asr[GT_K][GT_M + GT_PAD], Asi[GT_K][GT_M + GT_PAD];
+  __shared__ double Bsr[GT_K][GT_N + GT_PAD], Bsi[GT_K][GT_N + GT_PAD];
+  const int page = blockIdx.z;
+  A += (size_t)page * sA;
+  B += (size_t)page * sB;
+  C += (size_t)page * sC;
+  const int m0 = blockIdx.y * GT_M, n0 = blockIdx.x * GT_N;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+  v4d accR[2][2], accI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accR[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      accI[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+    }
+  for (int k0 = 0; k0 < K; k0 += GT_K) {
+    // ---- stage op(A)[m0:m0+64, k0:k0+16]
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int r, k;
+      if (OPA == 0) { r = t & 63; k = (t >> 6) + 4 * q; }
+      else { k = t & 15; r = (t >> 4) + 16 * q; }
+      int gr = m0 + r, gk = k0 + k;
+      cplx v = (gr < M && gk < K) ? load_op<OPA>(A, lda, gr, gk) : cmk(0.0, 0.0);
+      Asr[k][r] = v.x;
+      Asi[k][r] = v.y;
+    }
+    // ---- stage op(B)[k0:k0+16, n0:n0+64]
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int k, c;
+      if (OPB == 0) { k = t & 15; c = (t >> 4) + 16 * q; }
+      else { c = t & 63; k = (t >> 6) + 4 * q; }
+      int gk = k0 + k, gc = n0 + c;
+      cplx v = (gk < K && gc < N) ? load_op<OPB>(B, ldb, gk, gc) : cmk(0.0, 0.0);
+      Bsr[k][c] = v.x;
+      Bsi[k][c] = v.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GT_K; kk += 4) {
+      const int kr = kk + (lane >> 4), rr = lane & 15;
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = Asr[kr][wm + 16 * i + rr];
+        ai[i] = Asi[kr][wm + 16 * i + rr];
+        br[i] = Bsr[kr][wn + 16 * i + rr];
+        bi[i] = Bsi[kr][wn + 16 * i + rr];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accR[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  // ---- epilogue: lane holds rows 4*(lane/16)+v, column lane%16 of each 16x16 block
+  const bool beta0 = (beta.x == 0.0 && beta.y == 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int gr = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
+        int gc = n0 + wn + 16 * j + (lane & 15);
+        if (gr < M && gc < N) {
+          cplx acc = cmk(accR[i][j][v], accI[i][j][v]);
+          cplx out = cmul(alpha, acc);
+          size_t idx = (size_t)gr + (size_t)gc * ldc;
+          if (!beta0) out = cadd(out, cmul(beta, C[idx]));
+          C[idx] = out;
+        }
+      }
+}
+}  // namespace
+
+void orf_zgemm(hipStream_t st, char opA, char opB, int M, int N, int K, cplx alpha, const cplx* A,
+               int lda, long long sA, const cplx* B, int ldb, long long sB, cplx beta, cplx* C,
+               int ldc, long long sC, int batch) {
+  if (M <= 0 || N <= 0 || batch <= 0) return;
+  while (batch > 65535) {  // grid z limit
+    orf_zgemm(st, opA, opB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, 65535);
+    A += (size_t)sA * 65535;
+    B += (size_t)sB * 65535;
+    C += (size_t)sC * 65535;
+    batch -= 65535;
+  }
+  dim3 grid((N + GT_N - 1) / GT_N, (M + GT_M - 1) / GT_M, batch);
+  int oa = opA == 'N' ? 0 : (opA == 'T' ? 1 : 2);
+  int ob = opB == 'N' ? 0 : (opB == 'T' ? 1 : 2);
+#define ORF_GEMM_CASE(a, b)                                                                       \
+  if (oa == a && ob == b) {                                                                       \
+    zgemm_mfma_kernel<a, b><<<grid, 256, 0, st>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, \
+                                                  ldc, sC);                                       \
+    return;                                                                                       \
+  }
+  ORF_GEMM_CASE(0, 0) ORF_GEMM_CASE(0, 1) ORF_GEMM_CASE(0, 2)
+  ORF_GEMM_CASE(1, 0) ORF_GEMM_CASE(1, 1) ORF_GEMM_CASE(1, 2)
+  ORF_GEMM_CASE(2, 0) ORF_GEMM_CASE(2, 1) ORF_GEMM_CASE(2, 2)
+#undef ORF_GEMM_CASE
+}
+
+// ============================================================== LU (getrf)
+namespace {
+// Block argmax of (value, index); ties resolve to the smaller index.
+ORF_DEV void block_argmax(double& v, int& idx, double* rv, int* ri) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) {
+    double ov = __shfl_xor(v, o, 64);
+    int oi = __shfl_xor(idx, o, 64);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+  __syncthreads();
+  if (lane == 0) { rv[w] = v; ri[w] = idx; }
+  __syncthreads();
+  v = rv[0];
+  idx = ri[0];
+  for (int i = 1; i < nw; ++i)
+    if (rv[i] > v || (rv[i] == v && ri[i] < idx)) { v = rv[i]; idx = ri[i]; }
+}
+
+__global__ __launch_bounds__(256) void zgetrf_kernel(int n, cplx* __restrict__ Ag, int lda,
+                                                     long long sA, int* __restrict__ ipiv,
+                                                     int* __restrict__ info, int use_lds) {
+  extern __shared__ cplx lds_mat[];
+  __shared__ double rv[4];
+  __shared__ int ri[4];
+  const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
+  Ag += (size_t)page * sA;
+  ipiv += (size_t)page * n;
+  cplx* A = Ag;
+  int ld = lda;
+  if (use_lds) {
+    A = lds_mat;
+    ld = n;
+    for (int e = t; e < n * n; e += nt) {
+      int r = e % n, c = e / n;
+      A[e] = Ag[(size_t)r + (size_t)c * lda];
+    }
+    __syncthreads();
+  }
+  int sing = 0;
+  for (int k = 0; k < n; ++k) {
+    double best = -1.0;
+    int bi = n;
+    for (int i = k + t; i < n; i += nt) {
+      double v = cabs1(A[i + (size_t)k * ld]);
+      if (v > best) { best = v; bi = i; }
+    }
+    block_argmax(best, bi, rv, ri);
+    const int p = bi;
+    if (t == 0) ipiv[k] = p;
+    if (best == 0.0) {
+      if (!sing) sing = k + 1;
+      __syncthreads();
+      continue;
+    }
+    if (p != k) {
+      for (int j = t; j < n; j += nt) {
+        cplx a = A[k + (size_t)j * ld], b = A[p + (size_t)j * ld];
+        A[k + (size_t)j * ld] = b;
+        A[p + (size_t)j * ld] = a;
+      }
+    }
+    __syncthreads();
+    const cplx rp = crecip(A[k + (size_t)k * ld]);
+    for (int i = k + 1 + t; i < n; i += nt) A[i + (size_t)k * ld] = cmul(A[i + (size_t)k * ld], rp);
+    __syncthreads();
+    // trailing rank-1 update, 64 rows x 4 columns per pass
+    const int m = n - k - 1;
+    const int ti = t & 63, tj = t >> 6;
+    for (int j = tj; j < m; j += 4) {
+      const cplx u = A[k + (size_t)(k + 1 + j) * ld];
+      cplx* col = A + (size_t)(k + 1 + j) * ld;
+      const cplx* lcol = A + (size_t)k * ld;
+      for (int i = ti; i < m; i += 64) {
+        cplx l = lcol[k + 1 + i];
+        cplx a = col[k + 1 + i];
+        col[k + 1 + i] = csub(a, cmul(l, u));
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0 && info) info[page] = sing;
+  if (use_lds) {
+    for (int e = t; e < n * n; e += nt) {
+      int r = e % n, c = e / n;
+      Ag[(size_t)r + (size_t)c * lda] = A[e];
+    }
+  }
+}
+
+// Left solve A X = B with A = P L U (ipiv 0-based, LAPACK order).
+// One workgroup per (column block of CB columns, page); the block lives in LDS.
+__global__ __launch_bounds__(256) void zgetrs_left_kernel(int n, int nrhs, int CB,
+                                                          const cplx* __restrict__ LU, int lda,
+                                                          long long sA, const int* __restrict__ ipiv,
+                                                          cplx* __restrict__ Bg, int ldb,
+                                                          long long sB) {
+  extern __shared__ cplx X[];  // n x CB, column-major with ld n
+  const int page = blockIdx.y, c0 = blockIdx.x * CB, t = threadIdx.x, nt = blockDim.x;
+  LU += (size_t)page * sA;
+  ipiv += (size_t)page * n;
+  Bg += (size_t)page * sB;
+  const int nc = min(CB, nrhs - c0);
+  for (int e = t; e < n * nc; e += nt) {
+    int r = e % n, c = e / n;
+    X[r + c * n] = Bg[(size_t)r + (size_t)(c0 + c) * ldb];
+  }
+  __syncthreads();
+  // row interchanges in order (one wave, CB <= 64 lanes)
+  if (t < 64) {
+    for (int k = 0; k < n; ++k) {
+      int p = ipiv[k];
+      if (p != k && t < nc) {
+        cplx a = X[k + t * n];
+        X[k + t * n] = X[p + t * n];
+        X[p + t * n] = a;
+      }
+    }
+  }
+  __syncthreads();
+  // forward: unit lower triangular
+  for (int k = 0; k < n - 1; ++k) {
+    const int m = n - k - 1;
+    for (int e = t; e < m * nc; e += nt) {
+      int i = k + 1 + e % m, c = e / m;
+      X[i + c * n] = csub(X[i + c * n], cmul(LU[i + (size_t)k * lda], X[k + c * n]));
+    }
+    __syncthreads();
+  }
+  // backward: upper triangular
+  for (int k = n - 1; k >= 0; --k) {
+    if (t < nc) X[k + t * n] = cdiv(X[k + t * n], LU[k + (size_t)k * lda]);
+    __syncthreads();
+    for (int e = t; e < k * nc; e += nt) {
+      int i = e % k, c = e / k;
+      X[i + c * n] = csub(X[i + c * n], cmul(LU[i + (size_t)k * lda], X[k + c * n]));
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < n * nc; e += nt) {
+    int r = e % n, c = e / n;
+    Bg[(size_t)r + (size_t)(c0 + c) * ldb] = X[r + c * n];
+  }
+}
+
+// Right solve X A = B (B is nrows x n) with A = P L U.  One workgroup per
+// (block of RB rows, page); the row block lives in LDS as RB x n column-major.
+__global__ __launch_bounds__(256) void zgetrs_right_kernel(int n, int nrows, int RB,
+                                                           const cplx* __restrict__ LU, int lda,
+                                                           long long sA,
+                                                           const int* __restrict__ ipiv,
+                                                           cplx* __restrict__ Bg, int ldb,
+                                                           long long sB) {
+  extern __shared__ cplx X[];  // RB x n, ld RB
+  const int page = blockIdx.y, r0 = blockIdx.x * RB, t = threadIdx.x, nt = blockDim.x;
+  LU += (size_t)page * sA;
+  ipiv += (size_t)page * n;
+  Bg += (size_t)page * sB;
+  const int nr = min(RB, nrows - r0);
+  for (int e = t; e < nr * n; e += nt) {
+    int r = e % nr, c = e / nr;
+    X[r + c * RB] = Bg[(size_t)(r0 + r) + (size_t)c * ldb];
+  }
+  __syncthreads();
+  // Z U = B: column j final after dividing by U[j,j]; then B[:, j'>j] -= Z[:, j] U[j, j']
+  for (int j = 0; j < n; ++j) {
+    if (t < nr) X[t + j * RB] = cdiv(X[t + j * RB], LU[j + (size_t)j * lda]);
+    __syncthreads();
+    const int m = n - j - 1;
+    for (int e = t; e < nr * m; e += nt) {
+      int r = e % nr, c = j + 1 + e / nr;
+      X[r + c * RB] = csub(X[r + c * RB], cmul(X[r + j * RB], LU[j + (size_t)c * lda]));
+    }
+    __syncthreads();
+  }
+  // Y L = Z (L unit lower): for j = n-1..0: Y[:, j] final; Z[:, j'<j] -= Y[:, j] L[j, j']
+  for (int j = n - 1; j > 0; --j) {
+    for (int e = t; e < nr * j; e += nt) {
+      int r = e % nr, c = e / nr;
+      X[r + c * RB] = csub(X[r + c * RB], cmul(X[r + j * RB], LU[j + (size_t)c * lda]));
+    }
+    __syncthreads();
+  }
+  // X = Y P^-1: column swaps in reverse order
+  if (t < 64) {
+    for (int k = n - 1; k >= 0; --k) {
+      int p = ipiv[k];
+      if (p != k) {
+        for (int r = t; r < nr; r += 64) {
+          cplx a = X[r + k * RB];
+          X[r + k * RB] = X[r + p * RB];
+          X[r + p * RB] = a;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < nr * n; e += nt) {
+    int r = e % nr, c = e / nr;
+    Bg[(size_t)(r0 + r) + (size_t)c * ldb] = X[r + c * RB];
+  }
+}
+}  // namespace
+
+static int g_lds_limit = 65536;
+
+void orf_set_lds_limit(int bytes) { g_lds_limit = bytes; }
+int orf_lds_limit() { return g_lds_limit; }
+
+void orf_zgetrf(hipStream_t st, int n, cplx* A, int lda, long long sA, int* ipiv, int* info,
+                int batch) {
+  if (n <= 0 || batch <= 0) return;
+  size_t bytes = (size_t)n * n * sizeof(cplx);
+  int use_lds = bytes <= (size_t)g_lds_limit;
+  zgetrf_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, ipiv, info, use_lds);
+}
+
+static int rhs_block(int n) {
+  int cb = g_lds_limit / (n * (int)sizeof(cplx));
+  if (cb > 64) cb = 64;
+  int p = 1;
+  while (p * 2 <= cb) p *= 2;
+  return p < 1 ? 1 : p;
+}
+
+void orf_zgetrs(hipStream_t st, char side, int n, int nrhs, const cplx* LU, int lda, long long sA,
+                const int* ipiv, cplx* B, int ldb, long long sB, int batch) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return;
+  while (batch > 65535) {  // grid y limit
+    orf_zgetrs(st, side, n, nrhs, LU, lda, sA, ipiv, B, ldb, sB, 65535);
+    LU += (size_t)sA * 65535;
+    ipiv += (size_t)n * 65535;
+    B += (size_t)sB * 65535;
+    batch -= 65535;
+  }
+  int cb = rhs_block(n);
+  if (side == 'L') {
+    dim3 grid((nrhs + cb - 1) / cb, batch);
+    zgetrs_left_kernel<<<grid, 256, (size_t)n * cb * sizeof(cplx), st>>>(n, nrhs, cb, LU, lda, sA,
+                                                                          ipiv, B, ldb, sB);
+  } else {
+    dim3 grid((nrhs + cb - 1) / cb, batch);
+    zgetrs_right_kernel<<<grid, 256, (size_t)n * cb * sizeof(cplx), st>>>(n, nrhs, cb, LU, lda, sA,
+                                                                           ipiv, B, ldb, sB);
+  }
+}
+
+// ============================================================== eigen-decomposition
+namespace {
+constexpr double ULP = 2.220446049250313e-16;   // LAPACK dlamch('P') = eps * base
+constexpr double SAFMIN = 2.2250738585072014e-308;
+
+// Householder reduction to upper Hessenberg form, Z = H(0) H(1) ... accumulated.
+// A (n x n, ld) is reduced in place; entries below the subdiagonal are zeroed.
+__global__ __launch_bounds__(256) void hessenberg_kernel(int n, cplx* __restrict__ Ag, int lda,
+                                                         long long sA, cplx* __restrict__ Zg,
+                                                         long long sZ, int use_lds) {
+  extern __shared__ cplx lds_mat[];
+  __shared__ cplx v[1024];
+  __shared__ double red[4];
+  const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
+  const int lane = t & 63, w = t >> 6;
+  Ag += (size_t)page * sA;
+  cplx* Z = Zg + (size_t)page * sZ;
+  cplx* A = Ag;
+  int ld = lda;
+  if (use_lds) {
+    A = lds_mat;
+    ld = n;
+    for (int e = t; e < n * n; e += nt) A[e] = Ag[(size_t)(e % n) + (size_t)(e / n) * lda];
+  }
+  for (int e = t; e < n * n; e += nt) Z[e] = cmk((e % n) == (e / n) ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  for (int k = 0; k < n - 2; ++k) {
+    const int m = n - k - 1;
+    cplx* colk = A + (size_t)k * ld;
+    double s2 = 0.0;
+    for (int i = k + 2 + t; i < n; i += nt) s2 += cnorm2(colk[i]);
+    s2 = block_sum(s2, red);
+    const cplx alpha = colk[k + 1];
+    if (s2 == 0.0 && alpha.y == 0.0) {
+      __syncthreads();
+      continue;  // H(k) = I
+    }
+    const double beta = -copysign(sqrt(cnorm2(alpha) + s2), alpha.x);
+    const cplx tau = cmk((beta - alpha.x) / beta, -alpha.y / beta);
+    const cplx scal = crecip(csub(alpha, cmk(beta, 0.0)));
+    for (int i = t; i < m; i += nt) v[i] = (i == 0) ? cmk(1.0, 0.0) : cmul(colk[k + 1 + i], scal);
+    __syncthreads();
+    for (int i = k + 1 + t; i < n; i += nt) colk[i] = (i == k + 1) ? cmk(beta, 0.0) : cmk(0.0, 0.0);
+    // right: rows 0..n-1 of A and Z, columns k+1..n-1:  X -= tau (X v) v^H
+    for (int r = t; r < n; r += nt) {
+      cplx wa = cmk(0.0, 0.0), wz = cmk(0.0, 0.0);
+      for (int j = 0; j < m; ++j) {
+        const cplx vj = v[j];
+        wa = cadd(wa, cmul(A[r + (size_t)(k + 1 + j) * ld], vj));
+        wz = cadd(wz, cmul(Z[r + (size_t)(k + 1 + j) * n], vj));
+      }
+      wa = cmul(tau, wa);
+      wz = cmul(tau, wz);
+      for (int j = 0; j < m; ++j) {
+        const cplx vjc = cconj(v[j]);
+        cplx* pa = A + r + (size_t)(k + 1 + j) * ld;
+        *pa = csub(*pa, cmul(wa, vjc));
+        cplx* pz = Z + r + (size_t)(k + 1 + j) * n;
+        *pz = csub(*pz, cmul(wz, vjc));
+      }
+    }
+    __syncthreads();
+    // left: rows k+1..n-1, columns k+1..n-1:  A -= conj(tau) v (v^H A); one wave per column
+    const cplx taub = cconj(tau);
+    for (int j = k + 1 + w; j < n; j += nt >> 6) {
+      cplx* col = A + (size_t)j * ld;
+      double zr = 0.0, zi = 0.0;
+      for (int i = lane; i < m; i += 64) {
+        cplx p = cmulc(v[i], col[k + 1 + i]);
+        zr += p.x;
+        zi += p.y;
+      }
+      zr = wave_sum(zr);
+      zi = wave_sum(zi);
+      const cplx f = cmul(taub, cmk(zr, zi));
+      for (int i = lane; i < m; i += 64) col[k + 1 + i] = csub(col[k + 1 + i], cmul(v[i], f));
+    }
+    __syncthreads();
+  }
+  if (use_lds) {
+    for (int e = t; e < n * n; e += nt) Ag[(size_t)(e % n) + (size_t)(e / n) * lda] = A[e];
+  }
+}
+
+ORF_DEV void givens(cplx a, cplx b, double& c, cplx& s, cplx& r) {
+  const double aa = cabs(a), bb = cabs(b);
+  if (bb == 0.0) { c = 1.0; s = cmk(0.0, 0.0); r = a; return; }
+  if (aa == 0.0) { c = 0.0; s = cconj(cscale(b, 1.0 / bb)); r = cmk(bb, 0.0); return; }
+  const double rho = hypot(aa, bb);
+  c = aa / rho;
+  const cplx ph = cscale(a, 1.0 / aa);
+  s = cscale(cmul(ph, cconj(b)), 1.0 / rho);
+  r = cscale(ph, rho);
+}
+
+// Complex Schur form by single-shift QR on a Hessenberg matrix (in place);
+// Schur vectors accumulated into Z; eigenvalues to wout.  info[page] = 0 on
+// success, i+1 if eigenvalue i did not converge within the iteration limit.
+__global__ __launch_bounds__(256) void schur_qr_kernel(int n, cplx* __restrict__ Hg, int ldh,
+                                                       long long sH, cplx* __restrict__ Zg,
+                                                       long long sZ, cplx* __restrict__ wout,
+                                                       int* __restrict__ info, int use_lds) {
+  extern __shared__ cplx lds_mat[];
+  __shared__ int redk[4];
+  const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
+  const int lane = t & 63, wv = t >> 6, nw = nt >> 6;
+  Hg += (size_t)page * sH;
+  cplx* Z = Zg + (size_t)page * sZ;
+  wout += (size_t)page * n;
+  cplx* H = Hg;
+  int ld = ldh;
+  if (use_lds) {
+    H = lds_mat;
+    ld = n;
+    for (int e = t; e < n * n; e += nt) H[e] = Hg[(size_t)(e % n) + (size_t)(e / n) * ldh];
+    __syncthreads();
+  }
+#define HH(r, c) H[(size_t)(r) + (size_t)(c) * ld]
+  const double smlnum = SAFMIN * ((double)n / ULP);
+  const int itmax = 100;  // sweeps per eigenvalue before giving up (reported via info)
+  int failed = 0;
+  int i = n - 1;
+  while (i >= 0) {
+    int l = 0;
+    int its;
+    for (its = 0; its <= itmax; ++its) {
+      // ---- deflation search: largest k in (l, i] with negligible H[k,k-1]
+      int kfound = l;
+      for (int k = i - t; k > l; k -= nt) {
+        const cplx hkk1 = HH(k, k - 1);
+        bool small = false;
+        if (cabs1(hkk1) <= smlnum) small = true;
+        else {
+          double tst = cabs1(HH(k - 1, k - 1)) + cabs1(HH(k, k));
+          if (tst == 0.0) {
+            if (k - 2 >= 0) tst += cabs1(HH(k - 1, k - 2));
+            if (k + 1 <= n - 1) tst += cabs1(HH(k + 1, k));
+          }
+          if (cabs1(hkk1) <= ULP * tst) {
+            const double ab = fmax(cabs1(hkk1), cabs1(HH(k - 1, k)));
+            const double ba = fmin(cabs1(hkk1), cabs1(HH(k - 1, k)));
+            const cplx dd = csub(HH(k - 1, k - 1), HH(k, k));
+            const double aa = fmax(cabs1(HH(k, k)), cabs1(dd));
+            const double bb = fmin(cabs1(HH(k, k)), cabs1(dd));
+            const double s = aa + ab;
+            if (ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)))) small = true;
+          }
+        }
+        if (small) { kfound = k; break; }
+      }
+      // block max of kfound
+      int kv = kfound;
+      for (int o = 32; o > 0; o >>= 1) kv = max(kv, __shfl_xor(kv, o, 64));
+      __syncthreads();
+      if (lane == 0) redk[wv] = kv;
+      __syncthreads();
+      int kmax = redk[0];
+      for (int q = 1; q < nw; ++q) kmax = max(kmax, redk[q]);
+      l = kmax;
+      if (l > 0) {
+        __syncthreads();
+        if (t == 0) HH(l, l - 1) = cmk(0.0, 0.0);
+        __syncthreads();
+      }
+      if (l >= i) break;
+      // ---- shift
+      cplx sig;
+      if (its % 20 == 10) {
+        sig = cadd(cmk(0.75 * cabs1(HH(l + 1, l)), 0.0), HH(l, l));
+      } else if (its > 0 && its % 20 == 0) {
+        sig = cadd(cmk(0.75 * cabs1(HH(i, i - 1)), 0.0), HH(i, i));
+      } else {
+        cplx tt = HH(i, i);
+        const cplx u = cmul(csqrt(HH(i - 1, i)), csqrt(HH(i, i - 1)));
+        double s = cabs1(u);
+        if (s != 0.0) {
+          const cplx x = cscale(csub(HH(i - 1, i - 1), tt), 0.5);
+          const double sx = cabs1(x);
+          s = fmax(s, cabs1(x));
+          const cplx xs = cscale(x, 1.0 / s), us = cscale(u, 1.0 / s);
+          cplx y = cscale(csqrt(cadd(cmul(xs, xs), cmul(us, us))), s);
+          if (sx > 0.0) {
+            const cplx xx = cscale(x, 1.0 / sx);
+            if (xx.x * y.x + xx.y * y.y < 0.0) y = cneg(y);
+          }
+          tt = csub(tt, cmul(u, cdiv(u, cadd(x, y))));
+        }
+        sig = tt;
+      }
+      if (!(isfinite(sig.x) && isfinite(sig.y))) {  // non-finite input: stop at once
+        its = itmax + 1;
+        break;
+      }
+      // ---- single-shift QR sweep on rows/cols l..i
+      for (int k = l; k < i; ++k) {
+        cplx a, b;
+        if (k == l) { a = csub(HH(l, l), sig); b = HH(l + 1, l); }
+        else { a = HH(k, k - 1); b = HH(k + 1, k - 1); }
+        double c;
+        cplx s, r;
+        givens(a, b, c, s, r);
+        // the first rotation reads entries the row pass rewrites
+        if (k == l) __syncthreads();
+        const cplx sc = cconj(s);
+        // rows k, k+1 for columns k..n-1 (column k-1 is written below, after the barrier)
+        for (int j = k + t; j < n; j += nt) {
+          const cplx h1 = HH(k, j), h2 = HH(k + 1, j);
+          HH(k, j) = cadd(cscale(h1, c), cmul(s, h2));
+          HH(k + 1, j) = csub(cscale(h2, c), cmul(sc, h1));
+        }
+        __syncthreads();
+        if (k > l && t == 0) {
+          HH(k, k - 1) = r;
+          HH(k + 1, k - 1) = cmk(0.0, 0.0);
+        }
+        // columns k, k+1 for rows 0..min(k+2, i); Z rows 0..n-1
+        const int rmax = min(k + 2, i);
+        for (int q = t; q <= rmax; q += nt) {
+          const cplx h1 = HH(q, k), h2 = HH(q, k + 1);
+          HH(q, k) = cadd(cscale(h1, c), cmul(sc, h2));
+          HH(q, k + 1) = csub(cscale(h2, c), cmul(s, h1));
+        }
+        for (int q = t; q < n; q += nt) {
+          cplx* z1 = Z + q + (size_t)k * n;
+          cplx* z2 = Z + q + (size_t)(k + 1) * n;
+          const cplx h1 = *z1, h2 = *z2;
+          *z1 = cadd(cscale(h1, c), cmul(sc, h2));
+          *z2 = csub(cscale(h2, c), cmul(s, h1));
+        }
+        __syncthreads();
+      }
+    }
+    if (its > itmax) {
+      if (!failed) failed = i + 1;
+      l = i;  // give up on this eigenvalue; keep going so every wave exits
+    }
+    if (t == 0) wout[i] = HH(i, i);
+    // deflated block of size 1 at i
+    i = l - 1;
+    __syncthreads();
+  }
+  if (t == 0 && info) info[page] = failed;
+  if (use_lds) {
+    for (int e = t; e < n * n; e += nt) Hg[(size_t)(e % n) + (size_t)(e / n) * ldh] = H[e];
+  }
+#undef HH
+}
+
+// Eigenvectors of upper-triangular T (ztrevc, side R, no back-transform):
+// Y[:, k] = x with (T - T[k,k]) x = 0, x[k] = 1, x[j>k] = 0, scaled to max |.|_1 = 1.
+// One wave per eigenvector; x distributed over lanes, 64 rows per slot.
+constexpr int TREVC_SLOTS = 12;  // n <= 768
+__global__ __launch_bounds__(256) void trevc_kernel(int n, const cplx* __restrict__ T, int ldt,
+                                                    long long sT, cplx* __restrict__ Y, int ldy,
+                                                    long long sY) {
+  const int page = blockIdx.y, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int k = blockIdx.x * (blockDim.x >> 6) + wv;
+  if (k >= n) return;
+  T += (size_t)page * sT;
+  Y += (size_t)page * sY;
+  const double smlnum = SAFMIN * ((double)n / ULP);
+  const cplx tkk = T[k + (size_t)k * ldt];
+  const double smin = fmax(ULP * cabs1(tkk), smlnum);
+  cplx x[TREVC_SLOTS];
+#pragma unroll
+  for (int s = 0; s < TREVC_SLOTS; ++s) {
+    const int r = lane + 64 * s;
+    x[s] = (r < k) ? cneg(T[r + (size_t)k * ldt]) : cmk(r == k ? 1.0 : 0.0, 0.0);
+  }
+  for (int j = k - 1; j >= 0; --j) {
+    const int sj = j >> 6, lj = j & 63;
+    cplx d = csub(T[j + (size_t)j * ldt], tkk);
+    if (cabs1(d) < smin) d = cmk(smin, 0.0);
+    cplx xj = cmk(0.0, 0.0);
+#pragma unroll
+    for (int s = 0; s < TREVC_SLOTS; ++s)
+      if (s == sj) xj = x[s];
+    xj.x = __shfl(xj.x, lj, 64);
+    xj.y = __shfl(xj.y, lj, 64);
+    xj = cdiv(xj, d);
+    const cplx* colj = T + (size_t)j * ldt;
+#pragma unroll
+    for (int s = 0; s < TREVC_SLOTS; ++s) {
+      const int r = lane + 64 * s;
+      if (s == sj && lane == lj) x[s] = xj;
+      else if (r < j) x[s] = csub(x[s], cmul(colj[r], xj));
+    }
+  }
+  double mx = 0.0;
+#pragma unroll
+  for (int s = 0; s < TREVC_SLOTS; ++s)
+    if (lane + 64 * s <= k) mx = fmax(mx, cabs1(x[s]));
+  mx = wave_max(mx);
+  const double sc = mx > 0.0 ? 1.0 / mx : 1.0;
+#pragma unroll
+  for (int s = 0; s < TREVC_SLOTS; ++s) {
+    const int r = lane + 64 * s;
+    if (r < n) Y[r + (size_t)k * ldy] = (r <= k) ? cscale(x[s], sc) : cmk(0.0, 0.0);
+  }
+}
+
+// Scale every column of W to unit 2-norm.
+__global__ __launch_bounds__(256) void colnorm_kernel(int n, cplx* __restrict__ W, int ld,
+                                                      long long sW) {
+  const int page = blockIdx.y, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int k = blockIdx.x * (blockDim.x >> 6) + wv;
+  if (k >= n) return;
+  cplx* col = W + (size_t)page * sW + (size_t)k * ld;
+  double s = 0.0;
+  for (int r = lane; r < n; r += 64) s += cnorm2(col[r]);
+  s = wave_sum(s);
+  const double f = s > 0.0 ? 1.0 / sqrt(s) : 1.0;
+  for (int r = lane; r < n; r += 64) col[r] = cscale(col[r], f);
+}
+}  // namespace
+
+// A is overwritten by the Schur factor T; Zwork and Ywork hold n x n per page.
+void orf_zgeev(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V, int ldv,
+               long long sV, cplx* Zwork, cplx* Ywork, int* info, int batch) {
+  orf_zgeev_timed(st, n, A, lda, sA, w, V, ldv, sV, Zwork, Ywork, info, batch, OrfTimeHook());
+}
+
+void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V,
+                     int ldv, long long sV, cplx* Zwork, cplx* Ywork, int* info, int batch,
+                     const OrfTimeHook& hook) {
+  if (n <= 0 || batch <= 0) return;
+  auto tb = [&](const char* nm) { if (hook) hook(nm, 0); };
+  auto te = [&](const char* nm) { if (hook) hook(nm, 1); };
+  const long long sZ = (long long)n * n;
+  const size_t bytes = (size_t)n * n * sizeof(cplx);
+  const int use_lds = bytes <= (size_t)g_lds_limit;
+  tb("eig_hessenberg");
+  hessenberg_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, Zwork, sZ, use_lds);
+  te("eig_hessenberg");
+  tb("eig_schur_qr");
+  schur_qr_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, Zwork, sZ, w, info,
+                                                           use_lds);
+  te("eig_schur_qr");
+  dim3 g2((n + 3) / 4, batch);
+  tb("eig_vectors");
+  trevc_kernel<<<g2, 256, 0, st>>>(n, A, lda, sA, Ywork, n, sZ);
+  orf_zgemm(st, 'N', 'N', n, n, n, cmk(1.0, 0.0), Zwork, n, sZ, Ywork, n, sZ, cmk(0.0, 0.0), V,
+            ldv, sV, batch);
+  colnorm_kernel<<<g2, 256, 0, st>>>(n, V, ldv, sV);
+  te("eig_vectors");
+}

@@ -1,0 +1,28 @@
+// Internal launch wrappers shared by the kernel files and the plan layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "orf_common.h"
+#include <functional>
+
+typedef std::function<void(const char*, int)> OrfTimeHook;  // (name, 0 = begin / 1 = end)
+
+void orf_set_lds_limit(int bytes);
+int orf_lds_limit();
+
+// C = alpha op(A) op(B) + beta C, batched; op in {'N','T','C'}; stride 0 broadcasts.
+void orf_zgemm(hipStream_t st, char opA, char opB, int M, int N, int K, cplx alpha, const cplx* A,
+               int lda, long long sA, const cplx* B, int ldb, long long sB, cplx beta, cplx* C,
+               int ldc, long long sC, int batch);
+// LU with partial pivoting, in place; ipiv [n x batch] 0-based; info [batch] (0 ok, k+1 zero pivot).
+void orf_zgetrf(hipStream_t st, int n, cplx* A, int lda, long long sA, int* ipiv, int* info,
+                int batch);
+// side 'L': A X = B (B n x nrhs); side 'R': X A = B (B nrhs x n); B overwritten with X.
+void orf_zgetrs(hipStream_t st, char side, int n, int nrhs, const cplx* LU, int lda, long long sA,
+                const int* ipiv, cplx* B, int ldb, long long sB, int batch);
+// Eigen-decomposition; A (overwritten), w [n x batch], V (unit-norm eigenvectors),
+// Zwork/Ywork n*n per page; info [batch].
+void orf_zgeev(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V, int ldv,
+               long long sV, cplx* Zwork, cplx* Ywork, int* info, int batch);
+void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V,
+                     int ldv, long long sV, cplx* Zwork, cplx* Ywork, int* info, int batch,
+                     const OrfTimeHook& hook);

@@ -1,0 +1,1341 @@
+// Host runtime of the gfx950 RCWA solver: validation, device-resident plans,
+// the per-layer stage chain and the C ABI declared in include/orf_rcwa.h.
+//
+// The stage chain restates the reference's rcwa2D pipeline (rcwa2D.m:1-35):
+// harmonics and k-vectors (harmonics2D.m, A.2 sign convention), geometry
+// sampling (cellGrid.m, layerEps.m), Laurent / Li convolution matrices
+// (fourierWeights.m, convMatrix2D.m, convMatricesLi2D.m), Omega^2 = P Q
+// (pqMatrices.m), eigenmodes (modesLayer.m, modesHomogeneous.m), layer and
+// half-space S-matrices (smatrixLayer.m, smatrixRegion.m), Redheffer star
+// products (smatrixStar.m) and the efficiency reduction (rcwa2D.m:31-34).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/orf_pageops.h"
+#include "../../include/orf_rcwa.h"
+#include "orf_internal.h"
+#include "orf_rcwa_dev.h"
+
+typedef std::complex<double> zc;
+
+// kernels from rcwa_kernels.hip
+__global__ void raster_kernel(int, int, double, double, const DevGrid*, const DevShape*, const double*, uint8_t*);
+__global__ void grid_symmetry_kernel(int, int, const uint8_t*, int*);
+__global__ void chi_x_kernel(int, int, int, int, const uint8_t*, const cplx*, cplx*);
+__global__ void chi_y_kernel(int, int, int, int, const uint8_t*, const cplx*, cplx*);
+__global__ void chi_2d_kernel(int, int, int, int, const cplx*, const cplx*, cplx*);
+__global__ void laurent_kernel(int, int, int, const DevPage*, const cplx*, const cplx*, int, cplx*, long long);
+__global__ void li_toeplitz_kernel(int, int, int, int, const DevPage*, const cplx*, long long, const cplx*, int, cplx*);
+__global__ void li_gather_kernel(int, int, int, const cplx*, const cplx*, cplx*, cplx*, long long);
+__global__ void omega_rhs_kernel(int, const DevPage*, const double*, const cplx*, const cplx*, long long, cplx*, long long);
+__global__ void omega_assemble_kernel(int, const DevPage*, const double*, const cplx*, long long, const cplx*,
+                                      const cplx*, long long, cplx*, long long);
+__global__ void sector_project_kernel(int, int, const DevSolve*, const DevSector*, int, int, const cplx*, long long,
+                                      int, cplx*, long long);
+__global__ void sector_expand_kernel(int, int, const DevSolve*, const DevSector*, const cplx*, long long, cplx*,
+                                     long long);
+__global__ void sector_rows_kernel(int, int, int, const DevSolve*, const DevSector*, const cplx*, long long, cplx*,
+                                   long long);
+__global__ void lambda_kernel(int, int, const cplx*, cplx*);
+__global__ void qw_assemble_kernel(int, int, const DevSolve*, const DevPage*, const double*, const cplx*, long long,
+                                   const cplx*, long long, cplx*, long long);
+__global__ void colscale_inv_kernel(int, int, const cplx*, cplx*, long long);
+__global__ void homog_modes_kernel(int, int, int, const DevSolve*, const DevPage*, const DevSector*, const double*,
+                                   const cplx*, cplx*, cplx*, long long, cplx*);
+__global__ void sum_diff_kernel(long long, cplx*, cplx*);
+__global__ void smat_rhs3_kernel(int, const cplx*, const double*, const cplx*, const cplx*, long long, cplx*,
+                                 long long, cplx*);
+__global__ void rowscale_kernel(int, int, const cplx*, cplx*, long long);
+__global__ void eye_axpby_kernel(int, cplx, cplx, const cplx*, long long, cplx*, long long);
+__global__ void gather_pages_kernel(long long, const int*, const cplx*, long long, cplx*, long long);
+__global__ void eye_stack_kernel(int, long long, cplx*);
+__global__ void output_kernel(int, int, int, int, int, const DevSolve*, const DevPage*, const DevSector*,
+                              const DevSource*, const double*, const double*, const cplx*, const cplx*, long long,
+                              double*, double*, double*, double*, cplx*, cplx*, int);
+
+// ====================================================================== errors
+static thread_local std::string g_err;
+struct OrfError {
+  int code;
+  std::string msg;
+};
+static void fail(int code, const std::string& m) { throw OrfError{code, m}; }
+#define HIPCHECK(x)                                                                     \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess) {                                                             \
+      if (e_ == hipErrorOutOfMemory) fail(ORF_OOM, std::string("out of device memory: ") + #x); \
+      fail(ORF_DEVICE_ERROR, std::string(hipGetErrorString(e_)) + " at " + #x);         \
+    }                                                                                   \
+  } while (0)
+#define KCHECK() HIPCHECK(hipGetLastError())
+
+static inline cplx C(zc z) { return make_double2(z.real(), z.imag()); }
+static inline dim3 g1(long long n, int y = 1) { return dim3((unsigned)((n + 255) / 256), y); }
+
+// ====================================================================== timing
+struct StageTimer {
+  bool on = false;
+  struct Rec { const char* name; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, int>> acc;
+  std::vector<std::string> order;
+  hipEvent_t get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+  }
+  void begin(const char* name, hipStream_t st) {
+    if (!on) return;
+    Rec r{name, get(), get()};
+    HIPCHECK(hipEventRecord(r.a, st));
+    pending.push_back(r);
+  }
+  void end(hipStream_t st) {
+    if (!on) return;
+    HIPCHECK(hipEventRecord(pending.back().b, st));
+  }
+  void collect() {
+    for (auto& r : pending) {
+      float ms = 0.f;
+      HIPCHECK(hipEventSynchronize(r.b));
+      HIPCHECK(hipEventElapsedTime(&ms, r.a, r.b));
+      auto it = acc.find(r.name);
+      if (it == acc.end()) { order.push_back(r.name); acc[r.name] = {ms, 1}; }
+      else { it->second.first += ms; it->second.second += 1; }
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    pending.clear();
+  }
+  void reset() { acc.clear(); order.clear(); }
+  ~StageTimer() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+// ====================================================================== device memory
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() {}
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { if (p) (void)hipFree(p); }
+  void alloc(size_t b) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+    bytes = b;
+    if (b) HIPCHECK(hipMalloc(&p, b));
+  }
+  template <class T> T* as() const { return (T*)p; }
+  template <class T> void upload(const std::vector<T>& v) {
+    alloc(v.size() * sizeof(T));
+    if (!v.empty()) HIPCHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+};
+
+struct orf_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+// ====================================================================== host helpers
+static zc kz_branch_h(zc kz2) {
+  zc kz = std::sqrt(kz2);
+  if (kz.imag() < 0 || (kz.imag() == 0 && kz.real() < 0)) kz = -kz;
+  return kz;
+}
+static zc kz_nonzero_h(zc kz) { return std::abs(kz) < 1e-8 ? zc(1e-8, 1e-8) : kz; }
+
+// polarizationVector.m:4-20
+static void pol_vector(const double khat[3], zc jte, zc jtm, zc E[3]) {
+  double aTE[3];
+  if (std::fabs(khat[0]) < 1e-12 && std::fabs(khat[1]) < 1e-12) {
+    aTE[0] = 0; aTE[1] = 1; aTE[2] = 0;
+  } else {
+    aTE[0] = -khat[1]; aTE[1] = khat[0]; aTE[2] = 0;  // z x khat
+    double nn = std::sqrt(aTE[0] * aTE[0] + aTE[1] * aTE[1]);
+    aTE[0] /= nn; aTE[1] /= nn;
+  }
+  double aTM[3] = {aTE[1] * khat[2] - aTE[2] * khat[1], aTE[2] * khat[0] - aTE[0] * khat[2],
+                   aTE[0] * khat[1] - aTE[1] * khat[0]};
+  double nrm = 0;
+  for (int i = 0; i < 3; ++i) {
+    E[i] = jte * aTE[i] + jtm * aTM[i];
+    nrm += std::norm(E[i]);
+  }
+  nrm = std::sqrt(nrm);
+  for (int i = 0; i < 3; ++i) E[i] /= nrm;
+}
+
+// exact staircase Fourier weights (fourierWeights.m:7-12), (4N+1) x Ns, h fastest
+static std::vector<cplx> fourier_weights_h(int Ns, int N) {
+  const int H = 4 * N + 1;
+  std::vector<cplx> F((size_t)H * Ns);
+  for (int j = 0; j < Ns; ++j) {
+    const double xj = (j + 0.5) / Ns - 0.5;
+    for (int hi = 0; hi < H; ++hi) {
+      const double h = hi - 2 * N;
+      const double u = M_PI * h / Ns;
+      const double s = (u != 0) ? std::sin(u) / u : 1.0;
+      zc v = s * std::exp(zc(0, -2 * M_PI * h * xj)) / (double)Ns;
+      F[hi + (size_t)H * j] = C(v);
+    }
+  }
+  return F;
+}
+
+struct HostSector {
+  std::vector<int> ue_row, uh_row, ue_rowmap;
+  std::vector<double> ue_coef, uh_coef, ue_rowcoef;
+};
+
+// symmetryBasis.m:2-15: columns = even-even orbits then odd-odd orbits, each
+// ordered by (n >= 0 outer, m >= 0 inner).
+static HostSector build_sector(int Nx, int Ny, bool polP, int& nr) {
+  const int W = 2 * Nx + 1, M = W * (2 * Ny + 1), nf = 2 * M;
+  auto idx = [&](int m, int n) { return (m + Nx) + W * (n + Ny); };
+  struct Col { int rows[4]; double coef[4]; int cnt; };
+  std::vector<Col> ee, oo;
+  for (int b = 0; b <= Ny; ++b)
+    for (int a = 0; a <= Nx; ++a) {
+      int pts[4][2] = {{a, b}, {-a, b}, {a, -b}, {-a, -b}};
+      Col ce{{-1, -1, -1, -1}, {0, 0, 0, 0}, 0};
+      std::vector<int> seen;
+      for (auto& q : pts) {
+        int id = idx(q[0], q[1]);
+        if (std::find(seen.begin(), seen.end(), id) != seen.end()) continue;
+        seen.push_back(id);
+      }
+      double nrm = 1.0 / std::sqrt((double)seen.size());
+      for (size_t k = 0; k < seen.size(); ++k) { ce.rows[k] = seen[k]; ce.coef[k] = nrm; ce.cnt++; }
+      ee.push_back(ce);
+      if (a > 0 && b > 0) {
+        Col co{{-1, -1, -1, -1}, {0, 0, 0, 0}, 0};
+        for (int k = 0; k < 4; ++k) {
+          co.rows[k] = idx(pts[k][0], pts[k][1]);
+          co.coef[k] = 0.5 * ((pts[k][0] > 0) ? 1 : -1) * ((pts[k][1] > 0) ? 1 : -1);
+          co.cnt++;
+        }
+        oo.push_back(co);
+      }
+    }
+  nr = (int)(ee.size() + oo.size());
+  HostSector s;
+  s.ue_row.assign(4 * nr, -1); s.uh_row.assign(4 * nr, -1);
+  s.ue_coef.assign(4 * nr, 0); s.uh_coef.assign(4 * nr, 0);
+  s.ue_rowmap.assign(nf, -1); s.ue_rowcoef.assign(nf, 0);
+  // offsets of the field blocks: E = [Ex; Ey], H = [Hx; Hy]
+  // p: UE = [Ex ee | Ey oo], UH = [Hy ee | Hx oo];  s: UE = [Ey ee | Ex oo], UH = [Hx ee | Hy oo]
+  const int eeE = polP ? 0 : M, ooE = polP ? M : 0;
+  const int eeH = polP ? M : 0, ooH = polP ? 0 : M;
+  int a = 0;
+  auto put = [&](const Col& c, int offE, int offH) {
+    for (int k = 0; k < c.cnt; ++k) {
+      s.ue_row[4 * a + k] = offE + c.rows[k];
+      s.ue_coef[4 * a + k] = c.coef[k];
+      s.uh_row[4 * a + k] = offH + c.rows[k];
+      s.uh_coef[4 * a + k] = c.coef[k];
+      s.ue_rowmap[offE + c.rows[k]] = a;
+      s.ue_rowcoef[offE + c.rows[k]] = c.coef[k];
+    }
+    ++a;
+  };
+  for (auto& c : ee) put(c, eeE, eeH);
+  for (auto& c : oo) put(c, ooE, ooH);
+  return s;
+}
+
+// ====================================================================== plan
+struct orf_plan {
+  orf_ctx* ctx = nullptr;
+  hipStream_t st = nullptr;
+  // problem
+  double Lx = 0, Ly = 0, theta = 0, phi = 0;
+  int Nx = 0, Ny = 0, M = 0, nf = 0, nr = 0, Nsx = 0, Nsy = 0;
+  int B = 0, L = 0, nPol = 0, nLayers = 0, nSec = 1, reduced = 0;
+  int factorization = 0, precision = 0;
+  int nGrids = 0, nslotMax = 1, i0 = 0;
+  std::vector<double> lambda, k0;
+  std::vector<int> mIdx, nIdx;
+  std::vector<std::vector<int>> layerGrid;     // [layer][b] -> grid or -1
+  std::vector<std::vector<int>> layerMixed;    // unused placeholder
+  std::vector<std::vector<zc>> layerBgEps;     // [layer][b * L + l]
+  std::vector<int> layerAllPat, layerAnyPat;
+  std::vector<double> thick;                   // [layer * B + b]
+  // device-resident inputs
+  DBuf dShapes, dVerts, dGrids, dLabels, dFx, dFy, dChiX, dChiY, dChi2, dKxy, dKz;
+  std::vector<DBuf*> dPagesByLayer;            // [layer] DevPage[B*L]
+  std::vector<DBuf*> dSlotEps, dSlotInv;       // [layer] cplx[B*L*nslot]
+  DBuf dSectorData, dSectors, dSources;
+  std::vector<DBuf*> secArrays;
+  std::vector<DevSource> sources;
+  // shared per (lambda, sector) region data, nr x nr each
+  DBuf dV0, dSref, dStrn;                      // V0 [L*nSec], S [4][L*nSec]
+  // chunking
+  int chunkPages = 0;
+  // results (device)
+  DBuf dR, dT, dRtot, dTtot, dr, dt;
+  bool haveResult = false;
+  // workspace
+  std::vector<DBuf*> ws;
+  StageTimer timer;
+  ~orf_plan() {
+    for (auto* b : dPagesByLayer) delete b;
+    for (auto* b : dSlotEps) delete b;
+    for (auto* b : dSlotInv) delete b;
+    for (auto* b : secArrays) delete b;
+    for (auto* b : ws) delete b;
+  }
+};
+
+struct Timed {
+  StageTimer& t;
+  hipStream_t s;
+  Timed(StageTimer& t_, const char* n, hipStream_t s_) : t(t_), s(s_) { t.begin(n, s); }
+  ~Timed() { t.end(s); }
+};
+#define STAGE(name) Timed _tm_##__LINE__(P.timer, name, P.st)
+
+// ---------------------------------------------------------------- workspace set
+struct Work {
+  // mode-page buffers (per chunk of Cp mode pages)
+  cplx *E, *Exx, *Eyy, *G, *Omega, *Toe, *Inv, *GH, *HH;
+  int* ipivT;
+  // solve-page buffers (Cs = Cp * nSec)
+  cplx *A0, *Wm, *Vm, *Zw, *Yw, *mu, *lam, *Xd, *Wf, *QW, *T12;
+  cplx *X1, *X2, *XB, *R3, *Dm, *RHS2, *LUb;
+  cplx *Sa[4], *Sn[4], *Sb[4], *Tm, *M1, *Fm;
+  double* kL;
+  int *ipiv, *info;
+  DevSolve* solves;
+  int* groupOf;
+  cplx* epsSolve;
+};
+
+static size_t per_mode_bytes(const orf_plan& P) {
+  const size_t M2 = (size_t)P.M * P.M;
+  size_t b = 0;
+  b += 3 * M2 + 2 * M2 + (size_t)P.nf * P.nf;  // E, Exx, Eyy, G, Omega
+  if (P.factorization == ORF_FACT_LI) {
+    const size_t Wx = 2 * P.Nx + 1, Wy = 2 * P.Ny + 1;
+    b += 2 * std::max((size_t)P.Nsy * Wx * Wx, (size_t)P.Nsx * Wy * Wy);                 // Toe, Inv
+    b += (size_t)(4 * P.Ny + 1) * Wx * Wx + (size_t)(4 * P.Nx + 1) * Wy * Wy;            // GH, HH
+  }
+  b *= sizeof(cplx);
+  if (P.factorization == ORF_FACT_LI)
+    b += sizeof(int) * std::max((size_t)P.Nsy * (2 * P.Nx + 1), (size_t)P.Nsx * (2 * P.Ny + 1));
+  return b;
+}
+static size_t per_solve_bytes(const orf_plan& P) {
+  const size_t n2 = (size_t)P.nr * P.nr, nfr = (size_t)P.nf * P.nr;
+  size_t c = 0;
+  c += n2 * 5;               // A0, Wm, Vm, Zw, Yw
+  c += 3 * nfr;              // Wf, QW, T12
+  c += n2 * (2 + 1 + 3 + 1 + 2 + 1);  // X1 X2 XB R3 Dm RHS2 LUb
+  c += n2 * (12 + 3);        // Sa Sn Sb Tm M1 Fm
+  c += 3 * (size_t)P.nr + 1; // mu lam Xd eps
+  return c * sizeof(cplx) + sizeof(double) + sizeof(int) * (P.nr + 2) + sizeof(DevSolve);
+}
+
+static Work make_work(orf_plan& P, int Cp) {
+  for (auto* b : P.ws) delete b;
+  P.ws.clear();
+  auto get = [&](size_t bytes) -> void* {
+    DBuf* b = new DBuf();
+    P.ws.push_back(b);
+    b->alloc(std::max<size_t>(bytes, 16));
+    return b->p;
+  };
+  const int Cs = Cp * P.nSec;
+  const size_t M2 = (size_t)P.M * P.M, n2 = (size_t)P.nr * P.nr, nfr = (size_t)P.nf * P.nr;
+  const size_t cs = sizeof(cplx);
+  Work w{};
+  w.E = (cplx*)get(cs * M2 * Cp);
+  w.Exx = (cplx*)get(cs * M2 * Cp);
+  w.Eyy = (cplx*)get(cs * M2 * Cp);
+  w.G = (cplx*)get(cs * 2 * M2 * Cp);
+  w.Omega = (cplx*)get(cs * (size_t)P.nf * P.nf * Cp);
+  if (P.factorization == ORF_FACT_LI) {
+    const size_t Wx = 2 * P.Nx + 1, Wy = 2 * P.Ny + 1;
+    w.Toe = (cplx*)get(cs * std::max((size_t)P.Nsy * Wx * Wx, (size_t)P.Nsx * Wy * Wy) * Cp);
+    w.Inv = (cplx*)get(cs * std::max((size_t)P.Nsy * Wx * Wx, (size_t)P.Nsx * Wy * Wy) * Cp);
+    w.GH = (cplx*)get(cs * (size_t)(4 * P.Ny + 1) * Wx * Wx * Cp);
+    w.HH = (cplx*)get(cs * (size_t)(4 * P.Nx + 1) * Wy * Wy * Cp);
+    w.ipivT = (int*)get(sizeof(int) * std::max((size_t)P.Nsy * Wx, (size_t)P.Nsx * Wy) * Cp);
+  }
+  w.A0 = (cplx*)get(cs * n2 * Cs);
+  w.Wm = (cplx*)get(cs * n2 * Cs);
+  w.Vm = (cplx*)get(cs * n2 * Cs);
+  w.Zw = (cplx*)get(cs * n2 * Cs);
+  w.Yw = (cplx*)get(cs * n2 * Cs);
+  w.mu = (cplx*)get(cs * P.nr * Cs);
+  w.lam = (cplx*)get(cs * P.nr * Cs);
+  w.Xd = (cplx*)get(cs * P.nr * Cs);
+  w.Wf = (cplx*)get(cs * nfr * Cs);
+  w.QW = (cplx*)get(cs * nfr * Cs);
+  w.T12 = (cplx*)get(cs * nfr * Cs);
+  w.X1 = (cplx*)get(cs * n2 * Cs);
+  w.X2 = (cplx*)get(cs * n2 * Cs);
+  w.XB = (cplx*)get(cs * n2 * Cs);
+  w.R3 = (cplx*)get(cs * 3 * n2 * Cs);
+  w.Dm = (cplx*)get(cs * n2 * Cs);
+  w.RHS2 = (cplx*)get(cs * 2 * n2 * Cs);
+  w.LUb = (cplx*)get(cs * n2 * Cs);
+  for (int k = 0; k < 4; ++k) {
+    w.Sa[k] = (cplx*)get(cs * n2 * Cs);
+    w.Sn[k] = (cplx*)get(cs * n2 * Cs);
+    w.Sb[k] = (cplx*)get(cs * n2 * Cs);
+  }
+  w.Tm = (cplx*)get(cs * n2 * Cs);
+  w.M1 = (cplx*)get(cs * n2 * Cs);
+  w.Fm = (cplx*)get(cs * n2 * Cs);
+  w.kL = (double*)get(sizeof(double) * Cs);
+  w.ipiv = (int*)get(sizeof(int) * (size_t)std::max(P.nr, P.M) * std::max(Cs, Cp));
+  w.info = (int*)get(sizeof(int) * (size_t)std::max(Cs, Cp) * 64);
+  w.solves = (DevSolve*)get(sizeof(DevSolve) * Cs);
+  w.groupOf = (int*)get(sizeof(int) * Cs);
+  w.epsSolve = (cplx*)get(cs * Cs);
+  return w;
+}
+
+// ---------------------------------------------------------------- dense S-matrix algebra
+static const cplx ONE = {1.0, 0.0}, ZERO = {0.0, 0.0}, MONE = {-1.0, 0.0};
+
+static void check_info(orf_plan& P, const int* dinfo, int count, const char* what) {
+  std::vector<int> h(count);
+  HIPCHECK(hipMemcpyAsync(h.data(), dinfo, sizeof(int) * count, hipMemcpyDeviceToHost, P.st));
+  HIPCHECK(hipStreamSynchronize(P.st));
+  for (int i = 0; i < count; ++i)
+    if (h[i] != 0) fail(ORF_DEVICE_ERROR, std::string(what) + " failed on page " + std::to_string(i));
+}
+
+// Layer S-matrix (smatrixLayer.m:6-18): W, V, lam per solve page; W0 = I; V0 per
+// solve page in X2 on entry.  Out: S11 -> S11o, S12 -> S12o.
+static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx* lam, cplx* S11o, cplx* S12o) {
+  const int n = P.nr;
+  const long long n2 = (long long)n * n;
+  hipStream_t st = P.st;
+  {
+    STAGE("smat_lu");
+    orf_zgetrf(st, n, Wm, n, n2, w.ipiv, nullptr, Cs);
+    KCHECK();
+  }
+  eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.X1, n2);
+  {
+    STAGE("smat_solve");
+    orf_zgetrs(st, 'L', n, n, Wm, n, n2, w.ipiv, w.X1, n, n2, Cs);  // W \ W0
+  }
+  {
+    STAGE("smat_lu");
+    orf_zgetrf(st, n, Vm, n, n2, w.ipiv, nullptr, Cs);
+  }
+  {
+    STAGE("smat_solve");
+    orf_zgetrs(st, 'L', n, n, Vm, n, n2, w.ipiv, w.X2, n, n2, Cs);  // V \ V0
+  }
+  sum_diff_kernel<<<g1(n2 * Cs), 256, 0, st>>>(n2 * Cs, w.X1, w.X2);  // X1 = A, X2 = B
+  smat_rhs3_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, lam, w.kL, w.X1, w.X2, n2, w.R3, 3 * n2, w.Xd);
+  HIPCHECK(hipMemcpyAsync(w.XB, w.X2, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+  rowscale_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, n, w.Xd, w.XB, n2);
+  HIPCHECK(hipMemcpyAsync(w.LUb, w.X1, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+  {
+    STAGE("smat_lu");
+    orf_zgetrf(st, n, w.LUb, n, n2, w.ipiv, nullptr, Cs);
+  }
+  {
+    STAGE("smat_solve");
+    orf_zgetrs(st, 'L', n, 3 * n, w.LUb, n, n2, w.ipiv, w.R3, n, 3 * n2, Cs);  // [AiXB AiXA AiB]
+  }
+  cplx* AiXB = w.R3;
+  cplx* AiXA = w.R3 + n2;
+  cplx* AiB = w.R3 + 2 * n2;
+  // D = A - XB AiXB
+  HIPCHECK(hipMemcpyAsync(w.Dm, w.X1, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+  {
+    STAGE("smat_gemm");
+    orf_zgemm(st, 'N', 'N', n, n, n, MONE, w.XB, n, n2, AiXB, n, 3 * n2, ONE, w.Dm, n, n2, Cs);
+    // RHS2 = [XB AiXA - B, X (A - B AiB)]
+    eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ZERO, MONE, w.X2, n2, w.RHS2, 2 * n2);
+    orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.XB, n, n2, AiXA, n, 3 * n2, ONE, w.RHS2, n, 2 * n2, Cs);
+    eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ZERO, ONE, w.X1, n2, w.RHS2 + n2, 2 * n2);
+    orf_zgemm(st, 'N', 'N', n, n, n, MONE, w.X2, n, n2, AiB, n, 3 * n2, ONE, w.RHS2 + n2, n, 2 * n2, Cs);
+  }
+  rowscale_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, n, w.Xd, w.RHS2 + n2, 2 * n2);
+  {
+    STAGE("smat_lu");
+    orf_zgetrf(st, n, w.Dm, n, n2, w.ipiv, nullptr, Cs);
+  }
+  {
+    STAGE("smat_solve");
+    orf_zgetrs(st, 'L', n, 2 * n, w.Dm, n, n2, w.ipiv, w.RHS2, n, 2 * n2, Cs);
+  }
+  // split [S11 | S12] per page into separate page arrays
+  for (int k = 0; k < 2; ++k) {
+    cplx* dst = k == 0 ? S11o : S12o;
+    HIPCHECK(hipMemcpy2DAsync(dst, sizeof(cplx) * n2, w.RHS2 + k * n2, sizeof(cplx) * 2 * n2, sizeof(cplx) * n2,
+                              Cs, hipMemcpyDeviceToDevice, st));
+  }
+}
+
+// Redheffer star (smatrixStar.m:4-10): SA (4 page arrays) * SB -> out (4 page arrays).
+static void smatrix_star(orf_plan& P, Work& w, int Cs, cplx* const SA[4], cplx* const SB[4], cplx* const out[4]) {
+  const int n = P.nr;
+  const long long n2 = (long long)n * n;
+  hipStream_t st = P.st;
+  STAGE("star");
+  // D = SA12 / (I - SB11 SA22)
+  eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.M1, n2);
+  orf_zgemm(st, 'N', 'N', n, n, n, MONE, SB[0], n, n2, SA[3], n, n2, ONE, w.M1, n, n2, Cs);
+  orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs);
+  HIPCHECK(hipMemcpyAsync(w.Dm, SA[1], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+  orf_zgetrs(st, 'R', n, n, w.M1, n, n2, w.ipiv, w.Dm, n, n2, Cs);
+  // F = SB21 / (I - SA22 SB11)
+  eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.M1, n2);
+  orf_zgemm(st, 'N', 'N', n, n, n, MONE, SA[3], n, n2, SB[0], n, n2, ONE, w.M1, n, n2, Cs);
+  orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs);
+  HIPCHECK(hipMemcpyAsync(w.Fm, SB[2], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+  orf_zgetrs(st, 'R', n, n, w.M1, n, n2, w.ipiv, w.Fm, n, n2, Cs);
+  // S11 = SA11 + D SB11 SA21
+  orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Dm, n, n2, SB[0], n, n2, ZERO, w.Tm, n, n2, Cs);
+  HIPCHECK(hipMemcpyAsync(out[0], SA[0], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+  orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Tm, n, n2, SA[2], n, n2, ONE, out[0], n, n2, Cs);
+  // S12 = D SB12
+  orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Dm, n, n2, SB[1], n, n2, ZERO, out[1], n, n2, Cs);
+  // S21 = F SA21
+  orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Fm, n, n2, SA[2], n, n2, ZERO, out[2], n, n2, Cs);
+  // S22 = SB22 + F SA22 SB12
+  orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Fm, n, n2, SA[3], n, n2, ZERO, w.Tm, n, n2, Cs);
+  HIPCHECK(hipMemcpyAsync(out[3], SB[3], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+  orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Tm, n, n2, SB[1], n, n2, ONE, out[3], n, n2, Cs);
+}
+
+// ---------------------------------------------------------------- region S-matrices
+// smatrixRegion.m:8-20 with W = W0 = I for every (wavelength, sector) group.
+static void build_regions(orf_plan& P, const std::vector<zc>& epsInc, const std::vector<zc>& epsSub,
+                          const std::vector<zc>& epsGap) {
+  const int n = P.nr, G = P.L * P.nSec;
+  const long long n2 = (long long)n * n;
+  hipStream_t st = P.st;
+  // region pages: one DevPage per wavelength, DevSolve per group
+  std::vector<DevPage> pg(P.L);
+  for (int l = 0; l < P.L; ++l) pg[l] = DevPage{0, l, -1, 0};
+  std::vector<DevSolve> sv(3 * G);
+  std::vector<cplx> eps(3 * G);
+  for (int k = 0; k < 3; ++k)
+    for (int l = 0; l < P.L; ++l)
+      for (int s = 0; s < P.nSec; ++s) {
+        int g = k * G + l * P.nSec + s;
+        sv[g] = DevSolve{l, s};
+        eps[g] = C(k == 0 ? epsGap[l] : (k == 1 ? epsInc[l] : epsSub[l]));
+      }
+  DBuf dpg, dsv, deps, dW, dV, dlam, dX, dLU, dA, dB, dAi, dT, dip;
+  dpg.upload(pg);
+  dsv.upload(sv);
+  deps.upload(eps);
+  dW.alloc(sizeof(cplx) * n2 * 3 * G);
+  dV.alloc(sizeof(cplx) * n2 * 3 * G);
+  dlam.alloc(sizeof(cplx) * n * 3 * G);
+  homog_modes_kernel<<<g1(n2, 3 * G), 256, 0, st>>>(P.M, n, P.reduced, dsv.as<DevSolve>(), dpg.as<DevPage>(),
+                                                    P.dSectors.as<DevSector>(), P.dKxy.as<double>(),
+                                                    deps.as<cplx>(), dW.as<cplx>(), dV.as<cplx>(), n2,
+                                                    dlam.as<cplx>());
+  KCHECK();
+  P.dV0.alloc(sizeof(cplx) * n2 * G);
+  HIPCHECK(hipMemcpyAsync(P.dV0.p, dV.p, sizeof(cplx) * n2 * G, hipMemcpyDeviceToDevice, st));
+  // X = V0 \ V for inc and sub (2G pages); V0 broadcast per group
+  dLU.alloc(sizeof(cplx) * n2 * 2 * G);
+  for (int k = 0; k < 2; ++k)
+    HIPCHECK(hipMemcpyAsync(dLU.as<cplx>() + k * n2 * G, dV.p, sizeof(cplx) * n2 * G, hipMemcpyDeviceToDevice, st));
+  dip.alloc(sizeof(int) * n * 2 * G);
+  orf_zgetrf(st, n, dLU.as<cplx>(), n, n2, dip.as<int>(), nullptr, 2 * G);
+  dX.alloc(sizeof(cplx) * n2 * 2 * G);
+  HIPCHECK(hipMemcpyAsync(dX.p, dV.as<cplx>() + n2 * G, sizeof(cplx) * n2 * 2 * G, hipMemcpyDeviceToDevice, st));
+  orf_zgetrs(st, 'L', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), dX.as<cplx>(), n, n2, 2 * G);
+  // A = I + X, B = I - X
+  dA.alloc(sizeof(cplx) * n2 * 2 * G);
+  dB.alloc(sizeof(cplx) * n2 * 2 * G);
+  eye_axpby_kernel<<<g1(n2, 2 * G), 256, 0, st>>>(n, ONE, ONE, dX.as<cplx>(), n2, dA.as<cplx>(), n2);
+  eye_axpby_kernel<<<g1(n2, 2 * G), 256, 0, st>>>(n, ONE, MONE, dX.as<cplx>(), n2, dB.as<cplx>(), n2);
+  // LU(A); AiB = A \ B; Ai = A \ I; T = 0.5 (A - B AiB); BA = B / A
+  HIPCHECK(hipMemcpyAsync(dLU.p, dA.p, sizeof(cplx) * n2 * 2 * G, hipMemcpyDeviceToDevice, st));
+  orf_zgetrf(st, n, dLU.as<cplx>(), n, n2, dip.as<int>(), nullptr, 2 * G);
+  dAi.alloc(sizeof(cplx) * n2 * 2 * G * 3);
+  cplx* AiB = dAi.as<cplx>();
+  cplx* Ai = AiB + n2 * 2 * G;
+  cplx* BA = Ai + n2 * 2 * G;
+  HIPCHECK(hipMemcpyAsync(AiB, dB.p, sizeof(cplx) * n2 * 2 * G, hipMemcpyDeviceToDevice, st));
+  orf_zgetrs(st, 'L', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), AiB, n, n2, 2 * G);
+  eye_axpby_kernel<<<g1(n2, 2 * G), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, Ai, n2);
+  orf_zgetrs(st, 'L', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), Ai, n, n2, 2 * G);
+  HIPCHECK(hipMemcpyAsync(BA, dB.p, sizeof(cplx) * n2 * 2 * G, hipMemcpyDeviceToDevice, st));
+  orf_zgetrs(st, 'R', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), BA, n, n2, 2 * G);
+  dT.alloc(sizeof(cplx) * n2 * 2 * G);
+  eye_axpby_kernel<<<g1(n2, 2 * G), 256, 0, st>>>(n, ZERO, {0.5, 0.0}, dA.as<cplx>(), n2, dT.as<cplx>(), n2);
+  orf_zgemm(st, 'N', 'N', n, n, n, {-0.5, 0.0}, dB.as<cplx>(), n, n2, AiB, n, n2, ONE, dT.as<cplx>(), n, n2, 2 * G);
+  // assemble: ref uses the incident half (k = 0), trn the substrate half (k = 1)
+  P.dSref.alloc(sizeof(cplx) * n2 * G * 4);
+  P.dStrn.alloc(sizeof(cplx) * n2 * G * 4);
+  cplx* R = P.dSref.as<cplx>();
+  cplx* T = P.dStrn.as<cplx>();
+  const long long blk = n2 * G;
+  // ref: S11 = -AiB, S12 = 2 Ai, S21 = T, S22 = B / A
+  eye_axpby_kernel<<<g1(n2, G), 256, 0, st>>>(n, ZERO, MONE, AiB, n2, R + 0 * blk, n2);
+  eye_axpby_kernel<<<g1(n2, G), 256, 0, st>>>(n, ZERO, {2.0, 0.0}, Ai, n2, R + 1 * blk, n2);
+  HIPCHECK(hipMemcpyAsync(R + 2 * blk, dT.p, sizeof(cplx) * blk, hipMemcpyDeviceToDevice, st));
+  HIPCHECK(hipMemcpyAsync(R + 3 * blk, BA, sizeof(cplx) * blk, hipMemcpyDeviceToDevice, st));
+  // trn: S11 = B / A, S12 = T, S21 = 2 Ai, S22 = -AiB
+  HIPCHECK(hipMemcpyAsync(T + 0 * blk, BA + blk, sizeof(cplx) * blk, hipMemcpyDeviceToDevice, st));
+  HIPCHECK(hipMemcpyAsync(T + 1 * blk, dT.as<cplx>() + blk, sizeof(cplx) * blk, hipMemcpyDeviceToDevice, st));
+  eye_axpby_kernel<<<g1(n2, G), 256, 0, st>>>(n, ZERO, {2.0, 0.0}, Ai + blk, n2, T + 2 * blk, n2);
+  eye_axpby_kernel<<<g1(n2, G), 256, 0, st>>>(n, ZERO, MONE, AiB + blk, n2, T + 3 * blk, n2);
+  KCHECK();
+  HIPCHECK(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------- plan creation
+static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
+  if (!pr) fail(ORF_INVALID_ARG, "problem is NULL");
+  P.ctx = ctx;
+  P.st = ctx->stream;
+  P.Lx = pr->Lx;
+  P.Ly = pr->Ly;
+  if (!(P.Lx > 0) || !std::isfinite(P.Lx) || !(P.Ly > 0)) fail(ORF_INVALID_ARG, "period must be positive (Lx finite)");
+  P.Nx = pr->Nx;
+  P.Ny = std::isfinite(P.Ly) ? pr->Ny : 0;
+  if (P.Nx < 0 || P.Ny < 0) fail(ORF_INVALID_ARG, "Nx, Ny must be non-negative");
+  P.Nsx = pr->Nsx;
+  P.Nsy = std::isfinite(P.Ly) ? pr->Nsy : 1;
+  if (P.Nsx < 4 * P.Nx + 1 || P.Nsy < 4 * P.Ny + 1)
+    fail(ORF_SAMPLING, "Need at least 4N+1 samples per period (convMatrix1D:sampling)");
+  const int Wx = 2 * P.Nx + 1, Wy = 2 * P.Ny + 1;
+  P.M = Wx * Wy;
+  P.nf = 2 * P.M;
+  if (P.nf > 768) fail(ORF_INVALID_ARG, "2M > 768 is not supported");
+  P.B = pr->nStruct;
+  P.L = pr->nLambda;
+  P.nPol = pr->nPol;
+  P.nLayers = pr->nLayers;
+  if (P.B <= 0 || P.L <= 0 || P.nPol <= 0 || P.nLayers < 0) fail(ORF_INVALID_ARG, "empty batch");
+  P.theta = pr->theta;
+  P.phi = pr->phi;
+  P.factorization = pr->factorization;
+  P.precision = pr->precision;
+  if (pr->nMaterials <= 0 || !pr->matEps) fail(ORF_INVALID_ARG, "material table missing");
+  auto matEps = [&](int mat, int l) -> zc {
+    if (mat < 0 || mat >= pr->nMaterials) fail(ORF_INVALID_ARG, "material index out of range");
+    const double* e = pr->matEps + 2 * ((size_t)mat + (size_t)pr->nMaterials * l);
+    return zc(e[0], e[1]);
+  };
+  P.lambda.assign(pr->lambda0, pr->lambda0 + P.L);
+  for (double lm : P.lambda)
+    if (!(lm > 0)) fail(ORF_INVALID_ARG, "lambda0 must be positive");
+  // harmonics (harmonics2D.m:4-6)
+  P.mIdx.resize(P.M);
+  P.nIdx.resize(P.M);
+  for (int p = 0; p < P.M; ++p) {
+    P.mIdx[p] = p % Wx - P.Nx;
+    P.nIdx[p] = p / Wx - P.Ny;
+  }
+  P.i0 = P.Nx + Wx * P.Ny;
+  // k-vectors per wavelength (sign convention: +m lambda0 / Lx)
+  const double st_ = std::sin(P.theta), ct = std::cos(P.theta);
+  const double khat[3] = {st_ * std::cos(P.phi), st_ * std::sin(P.phi), ct};
+  std::vector<double> kxy((size_t)2 * P.M * P.L);
+  std::vector<cplx> kz((size_t)2 * P.M * P.L);
+  std::vector<zc> eInc(P.L), eSub(P.L), eGap(P.L);
+  std::vector<double> nInc(P.L);
+  P.k0.resize(P.L);
+  for (int l = 0; l < P.L; ++l) {
+    const double lm = P.lambda[l];
+    P.k0[l] = 2 * M_PI / lm;
+    eInc[l] = matEps(pr->incMaterial, l);
+    eSub[l] = matEps(pr->subMaterial, l);
+    const zc ninc = std::sqrt(eInc[l]);
+    if (std::fabs(ninc.imag()) > 1e-12) fail(ORF_LOSSY_INCIDENT, "Incident medium must be lossless.");
+    nInc[l] = ninc.real();
+    double kmax = 0;
+    for (int p = 0; p < P.M; ++p) {
+      const double kx = nInc[l] * khat[0] + P.mIdx[p] * lm / P.Lx;
+      const double ky = nInc[l] * khat[1] + (std::isfinite(P.Ly) ? P.nIdx[p] * lm / P.Ly : 0.0);
+      kxy[(size_t)l * 2 * P.M + p] = kx;
+      kxy[(size_t)l * 2 * P.M + P.M + p] = ky;
+      kz[(size_t)l * 2 * P.M + p] = C(kz_nonzero_h(kz_branch_h(eInc[l] - kx * kx - ky * ky)));
+      kz[(size_t)l * 2 * P.M + P.M + p] = C(kz_nonzero_h(kz_branch_h(eSub[l] - kx * kx - ky * ky)));
+      kmax = std::max(kmax, kx * kx + ky * ky);
+    }
+    eGap[l] = std::isnan(pr->gapEps) ? zc(1.0 + kmax, 0.0) : zc(pr->gapEps, 0.0);
+  }
+  P.dKxy.upload(kxy);
+  P.dKz.upload(kz);
+  // sources
+  bool allSP = true;
+  std::vector<zc> Ex(P.nPol), Ey(P.nPol);
+  for (int i = 0; i < P.nPol; ++i) {
+    const int k = pr->polKind[i];
+    zc jte, jtm;
+    if (k == ORF_POL_S) { jte = 1; jtm = 0; }
+    else if (k == ORF_POL_P) { jte = 0; jtm = 1; }
+    else if (k == ORF_POL_JONES) {
+      if (!pr->jones) fail(ORF_INVALID_POL, "Jones vector must be [aTE aTM].");
+      jte = zc(pr->jones[4 * i], pr->jones[4 * i + 1]);
+      jtm = zc(pr->jones[4 * i + 2], pr->jones[4 * i + 3]);
+      allSP = false;
+    } else fail(ORF_INVALID_POL, "pol must be \"s\", \"p\" or [aTE aTM].");
+    if (std::abs(jte) == 0 && std::abs(jtm) == 0) fail(ORF_INVALID_POL, "Jones vector must be nonzero.");
+    zc E[3];
+    pol_vector(khat, jte, jtm, E);
+    Ex[i] = E[0];
+    Ey[i] = E[1];
+  }
+  // layers and grids
+  P.layerGrid.assign(P.nLayers, std::vector<int>(P.B, -1));
+  P.layerAllPat.assign(P.nLayers, 0);
+  P.layerAnyPat.assign(P.nLayers, 0);
+  P.thick.resize((size_t)P.nLayers * P.B);
+  std::vector<DevGrid> grids;
+  std::vector<DevShape> shapes;
+  std::vector<std::vector<int>> gridSlotMat;  // slot -> material
+  for (int j = 0; j < P.nLayers; ++j) {
+    int npat = 0;
+    for (int b = 0; b < P.B; ++b) npat += pr->shapeCount[j + (size_t)P.nLayers * b] > 0;
+    P.layerAnyPat[j] = npat > 0;
+    P.layerAllPat[j] = npat == P.B;
+    for (int b = 0; b < P.B; ++b) {
+      const size_t jb = j + (size_t)P.nLayers * b;
+      P.thick[(size_t)j * P.B + b] = pr->thickness[jb];
+      if (pr->thickness[jb] < 0) fail(ORF_INVALID_ARG, "thickness must be non-negative");
+      const int cnt = pr->shapeCount[jb];
+      if (cnt == 0 && npat == 0) continue;
+      // patterned (or uniform inside a layer that is patterned elsewhere in the batch)
+      DevGrid g{(int)shapes.size(), cnt};
+      std::vector<int> slots{pr->background[jb]};
+      for (int s = 0; s < cnt; ++s) {
+        const orf_shape& sh = pr->shapes[pr->shapeStart[jb] + s];
+        int slot = (int)(std::find(slots.begin(), slots.end(), sh.material) - slots.begin());
+        if (slot == (int)slots.size()) slots.push_back(sh.material);
+        if ((int)slots.size() > ORF_MAX_SLOTS) fail(ORF_INVALID_ARG, "more than 8 materials in one layer");
+        DevShape d{sh.type, slot, sh.cx, sh.cy, sh.a, sh.b, 0, 0};
+        if (sh.type == ORF_SHAPE_POLYGON) {
+          if (sh.nvert < 3 || sh.vert0 < 0 || sh.vert0 + sh.nvert > pr->nVertices)
+            fail(ORF_INVALID_ARG, "A polygon needs at least 3 vertices.");
+          d.vert0 = sh.vert0;
+          d.nvert = sh.nvert;
+        } else if (sh.type != ORF_SHAPE_CIRCLE && sh.type != ORF_SHAPE_RECT) {
+          fail(ORF_INVALID_ARG, "unknown shape type");
+        }
+        (void)matEps(sh.material, 0);
+        shapes.push_back(d);
+      }
+      P.layerGrid[j][b] = (int)grids.size();
+      grids.push_back(g);
+      gridSlotMat.push_back(slots);
+      P.nslotMax = std::max(P.nslotMax, (int)slots.size());
+    }
+  }
+  P.nGrids = (int)grids.size();
+  // polygon vertices as separate x and y runs: [x(vert0..), y(vert0..)] per polygon
+  std::vector<double> verts;
+  for (auto& d : shapes)
+    if (d.type == ORF_SHAPE_POLYGON) {
+      int off = (int)verts.size();
+      for (int v = 0; v < d.nvert; ++v) verts.push_back(pr->vertices[2 * (size_t)(d.vert0 + v)]);
+      for (int v = 0; v < d.nvert; ++v) verts.push_back(pr->vertices[2 * (size_t)(d.vert0 + v) + 1]);
+      d.vert0 = off;
+    }
+  if (verts.empty()) verts.push_back(0.0);
+  hipStream_t st = P.st;
+  if (P.nGrids > 0) {
+    P.dShapes.upload(shapes.empty() ? std::vector<DevShape>(1) : shapes);
+    P.dVerts.upload(verts);
+    P.dGrids.upload(grids);
+    const long long cells = (long long)P.Nsx * P.Nsy;
+    P.dLabels.alloc((size_t)cells * P.nGrids);
+    raster_kernel<<<dim3((unsigned)((cells + 255) / 256), P.nGrids), 256, 0, st>>>(
+        P.Nsx, P.Nsy, P.Lx, P.Ly, P.dGrids.as<DevGrid>(), P.dShapes.as<DevShape>(), P.dVerts.as<double>(),
+        P.dLabels.as<uint8_t>());
+    KCHECK();
+  }
+  // symmetry decision (rcwa2D.m:24-27): theta = 0, pol "s"/"p", mirror-symmetric grids
+  P.reduced = 0;
+  if (pr->symmetry == ORF_SYM_AUTO && P.theta == 0.0 && allSP) {
+    int sym = 1;
+    if (P.nGrids > 0) {
+      DBuf flags;
+      flags.upload(std::vector<int>(P.nGrids, 1));
+      const long long cells = (long long)P.Nsx * P.Nsy;
+      grid_symmetry_kernel<<<dim3((unsigned)std::min<long long>((cells + 255) / 256, 1024), P.nGrids), 256, 0, st>>>(
+          P.Nsx, P.Nsy, P.dLabels.as<uint8_t>(), flags.as<int>());
+      KCHECK();
+      std::vector<int> h(P.nGrids);
+      HIPCHECK(hipMemcpy(h.data(), flags.p, sizeof(int) * P.nGrids, hipMemcpyDeviceToHost));
+      for (int v : h) sym &= v;
+    }
+    P.reduced = sym;
+  }
+  // sectors and sources
+  std::vector<HostSector> hs;
+  std::vector<int> srcSector(P.nPol, 0);
+  if (P.reduced) {
+    int secOf[2] = {-1, -1};  // [s, p]
+    for (int i = 0; i < P.nPol; ++i) {
+      const int k = pr->polKind[i];
+      if (secOf[k] < 0) {
+        int nr = 0;
+        hs.push_back(build_sector(P.Nx, P.Ny, k == ORF_POL_P, nr));
+        secOf[k] = (int)hs.size() - 1;
+        P.nr = nr;
+      }
+      srcSector[i] = secOf[k];
+    }
+    P.nSec = (int)hs.size();
+  } else {
+    P.nr = P.nf;
+    P.nSec = 1;
+  }
+  {
+    std::vector<DevSector> ds(std::max(1, P.nSec));
+    for (int s = 0; s < P.nSec && P.reduced; ++s) {
+      auto up = [&](auto& vec) {
+        DBuf* b = new DBuf();
+        P.secArrays.push_back(b);
+        b->upload(vec);
+        return b->p;
+      };
+      ds[s].ue_row = (const int*)up(hs[s].ue_row);
+      ds[s].ue_coef = (const double*)up(hs[s].ue_coef);
+      ds[s].uh_row = (const int*)up(hs[s].uh_row);
+      ds[s].uh_coef = (const double*)up(hs[s].uh_coef);
+      ds[s].ue_rowmap = (const int*)up(hs[s].ue_rowmap);
+      ds[s].ue_rowcoef = (const double*)up(hs[s].ue_rowcoef);
+    }
+    P.dSectors.upload(ds);
+  }
+  P.sources.resize(P.nPol);
+  for (int i = 0; i < P.nPol; ++i) {
+    DevSource so{};
+    so.pol = i;
+    so.sector = srcSector[i];
+    so.ex = C(Ex[i]);
+    so.ey = C(Ey[i]);
+    so.kzinc0 = nInc[0] * ct;
+    so.srcCol = 0;
+    so.srcVal = C(0);
+    if (P.reduced) {
+      const HostSector& s = hs[srcSector[i]];
+      int a1 = s.ue_rowmap[P.i0], a2 = s.ue_rowmap[P.M + P.i0];
+      zc v1 = a1 >= 0 ? s.ue_rowcoef[P.i0] * Ex[i] : 0.0;
+      zc v2 = a2 >= 0 ? s.ue_rowcoef[P.M + P.i0] * Ey[i] : 0.0;
+      if (a1 >= 0 && std::abs(v1) > 0) { so.srcCol = a1; so.srcVal = C(v1); }
+      else { so.srcCol = std::max(a2, 0); so.srcVal = C(v2); }
+    }
+    P.sources[i] = so;
+  }
+  // incident index may vary with lambda: kzinc0 per wavelength is folded into kz (output uses Re kz_inc / n cos)
+  for (int l = 1; l < P.L; ++l)
+    if (std::fabs(nInc[l] - nInc[0]) > 0) {
+      // different incident index per wavelength: store per source is insufficient
+      fail(ORF_INVALID_ARG, "dispersive incident medium across a multi-wavelength batch is not supported");
+    }
+  P.dSources.upload(P.sources);
+  // Fourier tables and geometry transforms
+  if (P.nGrids > 0) {
+    const int Hx = 4 * P.Nx + 1, Hy = 4 * P.Ny + 1;
+    P.dFx.upload(fourier_weights_h(P.Nsx, P.Nx));
+    P.dFy.upload(fourier_weights_h(P.Nsy, P.Ny));
+    const int ns = P.nslotMax;
+    P.dChiX.alloc(sizeof(cplx) * (size_t)P.nGrids * ns * Hx * P.Nsy);
+    P.dChiY.alloc(sizeof(cplx) * (size_t)P.nGrids * ns * Hy * P.Nsx);
+    P.dChi2.alloc(sizeof(cplx) * (size_t)P.nGrids * ns * Hx * Hy);
+    HIPCHECK(hipMemsetAsync(P.dChiX.p, 0, P.dChiX.bytes, st));
+    HIPCHECK(hipMemsetAsync(P.dChiY.p, 0, P.dChiY.bytes, st));
+    chi_x_kernel<<<dim3((unsigned)(((long long)P.Nsy * Hx + 255) / 256), P.nGrids), 256, 0, st>>>(
+        P.Nsx, P.Nsy, Hx, ns, P.dLabels.as<uint8_t>(), P.dFx.as<cplx>(), P.dChiX.as<cplx>());
+    chi_y_kernel<<<dim3((unsigned)(((long long)P.Nsx * Hy + 255) / 256), P.nGrids), 256, 0, st>>>(
+        P.Nsx, P.Nsy, Hy, ns, P.dLabels.as<uint8_t>(), P.dFy.as<cplx>(), P.dChiY.as<cplx>());
+    chi_2d_kernel<<<dim3((unsigned)(((long long)Hx * Hy * ns + 255) / 256), P.nGrids), 256, 0, st>>>(
+        P.Nsy, Hx, Hy, ns, P.dChiX.as<cplx>(), P.dFy.as<cplx>(), P.dChi2.as<cplx>());
+    KCHECK();
+  }
+  // per-layer page tables and slot permittivities
+  const int npages = P.B * P.L;
+  P.layerBgEps.assign(P.nLayers, std::vector<zc>(npages));
+  for (int j = 0; j < P.nLayers; ++j) {
+    std::vector<DevPage> pages(npages);
+    std::vector<cplx> se((size_t)npages * P.nslotMax, C(0)), si((size_t)npages * P.nslotMax, C(0));
+    for (int l = 0; l < P.L; ++l)
+      for (int b = 0; b < P.B; ++b) {
+        const int pg = b + P.B * l;
+        const size_t jb = j + (size_t)P.nLayers * b;
+        const int g = P.layerGrid[j][b];
+        pages[pg] = DevPage{b, l, g, g >= 0 ? (int)gridSlotMat[g].size() : 0};
+        P.layerBgEps[j][pg] = matEps(pr->background[jb], l);
+        if (g >= 0)
+          for (size_t s = 0; s < gridSlotMat[g].size(); ++s) {
+            zc e = matEps(gridSlotMat[g][s], l);
+            se[(size_t)pg * P.nslotMax + s] = C(e);
+            si[(size_t)pg * P.nslotMax + s] = C(1.0 / e);
+          }
+      }
+    DBuf* dp = new DBuf();
+    dp->upload(pages);
+    P.dPagesByLayer.push_back(dp);
+    DBuf* de = new DBuf();
+    de->upload(se);
+    P.dSlotEps.push_back(de);
+    DBuf* di = new DBuf();
+    di->upload(si);
+    P.dSlotInv.push_back(di);
+  }
+  {
+    std::vector<DevPage> base(npages);
+    for (int l = 0; l < P.L; ++l)
+      for (int b = 0; b < P.B; ++b) base[b + P.B * l] = DevPage{b, l, -1, 0};
+    DBuf* dp = new DBuf();
+    dp->upload(base);
+    P.dPagesByLayer.push_back(dp);  // index nLayers: base pages
+  }
+  // shared half-space S-matrices per (wavelength, sector)
+  build_regions(P, eInc, eSub, eGap);
+  // chunk size by device memory
+  size_t freeB = 0, totB = 0;
+  HIPCHECK(hipMemGetInfo(&freeB, &totB));
+  const size_t perMode = per_mode_bytes(P) + (size_t)P.nSec * per_solve_bytes(P);
+  size_t budget = (size_t)(0.6 * (double)freeB);
+  int cp = (int)std::max<size_t>(1, budget / std::max<size_t>(perMode, 1));
+  if (pr->batchSize > 0) cp = std::min(cp, pr->batchSize * P.L);
+  cp = std::min(cp, 32768 / P.nSec);  // grid-dimension limit of the page-batched launches
+  P.chunkPages = std::min(cp, npages);
+  // results
+  const size_t nout = (size_t)P.B * P.L * P.nPol;
+  P.dR.alloc(sizeof(double) * P.M * nout);
+  P.dT.alloc(sizeof(double) * P.M * nout);
+  P.dRtot.alloc(sizeof(double) * nout);
+  P.dTtot.alloc(sizeof(double) * nout);
+  P.dr.alloc(sizeof(cplx) * 3 * P.M * nout);
+  P.dt.alloc(sizeof(cplx) * 3 * P.M * nout);
+  HIPCHECK(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------- execution
+static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
+  hipStream_t st = P.st;
+  const int npages = P.B * P.L;
+  const int Cmax = P.chunkPages;
+  Work w = make_work(P, Cmax);
+  P.timer.reset();
+  const int nChunks = (npages + Cmax - 1) / Cmax;
+  const int M = P.M, nf = P.nf, nr = P.nr, nSec = P.nSec;
+  const long long M2 = (long long)M * M, nf2 = (long long)nf * nf, n2 = (long long)nr * nr;
+  const long long nfr = (long long)nf * nr;
+  const int Wx = 2 * P.Nx + 1, Wy = 2 * P.Ny + 1, Hx = 4 * P.Nx + 1, Hy = 4 * P.Ny + 1;
+  for (int ch = 0; ch < nChunks; ++ch) {
+    const int p0 = ch * Cmax, Cp = std::min(Cmax, npages - p0), Cs = Cp * nSec;
+    // solve pages: sp = mode * nSec + sector
+    std::vector<DevSolve> sv(Cs);
+    std::vector<int> grp(Cs);
+    for (int m = 0; m < Cp; ++m)
+      for (int s = 0; s < nSec; ++s) {
+        sv[m * nSec + s] = DevSolve{m, s};
+        const int l = (p0 + m) / P.B;
+        grp[m * nSec + s] = l * nSec + s;
+      }
+    HIPCHECK(hipMemcpyAsync(w.solves, sv.data(), sizeof(DevSolve) * Cs, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(w.groupOf, grp.data(), sizeof(int) * Cs, hipMemcpyHostToDevice, st));
+    const long long G = (long long)P.L * nSec;
+    // S_acc = S_ref of each page's (wavelength, sector)
+    for (int k = 0; k < 4; ++k)
+      gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dSref.as<cplx>() + k * n2 * G, n2, w.Sa[k], n2);
+    KCHECK();
+    const DevPage* basePages = P.dPagesByLayer[P.nLayers]->as<DevPage>() + p0;
+    for (int j = 0; j < P.nLayers; ++j) {
+      const DevPage* pages = P.dPagesByLayer[j]->as<DevPage>() + p0;
+      if (P.layerAnyPat[j]) {
+        const cplx* se = P.dSlotEps[j]->as<cplx>() + (size_t)p0 * P.nslotMax;
+        const cplx* si = P.dSlotInv[j]->as<cplx>() + (size_t)p0 * P.nslotMax;
+        {
+          STAGE("conv");
+          laurent_kernel<<<g1(M2, Cp), 256, 0, st>>>(P.Nx, P.Ny, M, pages, P.dChi2.as<cplx>(), se, P.nslotMax, w.E, M2);
+          if (P.factorization == ORF_FACT_LI) {
+            // Exx: inverse rule along x at each y-sample, Laurent along y
+            const long long perx = (long long)Wx * Wx * P.Nsy;
+            li_toeplitz_kernel<<<g1(perx, Cp), 256, 0, st>>>(0, P.Nx, P.Nsy, Hx, pages, P.dChiX.as<cplx>(),
+                                                              (long long)P.nslotMax * Hx * P.Nsy, si, P.nslotMax, w.Toe);
+            KCHECK();
+            // invert every Toeplitz block: LU then solve against the identity (in Yw scratch)
+            const int nb = Cp * P.Nsy;
+            cplx* inv = w.Inv;
+            eye_stack_kernel<<<g1((long long)Wx * Wx * nb), 256, 0, st>>>(Wx, (long long)nb, inv);
+            orf_zgetrf(st, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, nb);
+            orf_zgetrs(st, 'L', Wx, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, inv, Wx, (long long)Wx * Wx, nb);
+            // G = Fy * [A_j]^T : (Hy x Nsy) * (Nsy x Wx^2)
+            orf_zgemm(st, 'N', 'T', Hy, Wx * Wx, P.Nsy, ONE, P.dFy.as<cplx>(), Hy, 0, inv, Wx * Wx,
+                      (long long)Wx * Wx * P.Nsy, ZERO, w.GH, Hy, (long long)Hy * Wx * Wx, Cp);
+            // Eyy: inverse rule along y at each x-sample, Laurent along x
+            const long long pery = (long long)Wy * Wy * P.Nsx;
+            li_toeplitz_kernel<<<g1(pery, Cp), 256, 0, st>>>(1, P.Ny, P.Nsx, Hy, pages, P.dChiY.as<cplx>(),
+                                                              (long long)P.nslotMax * Hy * P.Nsx, si, P.nslotMax, w.Toe);
+            const int nb2 = Cp * P.Nsx;
+            eye_stack_kernel<<<g1((long long)Wy * Wy * nb2), 256, 0, st>>>(Wy, (long long)nb2, inv);
+            orf_zgetrf(st, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, nb2);
+            orf_zgetrs(st, 'L', Wy, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, inv, Wy, (long long)Wy * Wy, nb2);
+            orf_zgemm(st, 'N', 'T', Hx, Wy * Wy, P.Nsx, ONE, P.dFx.as<cplx>(), Hx, 0, inv, Wy * Wy,
+                      (long long)Wy * Wy * P.Nsx, ZERO, w.HH, Hx, (long long)Hx * Wy * Wy, Cp);
+            li_gather_kernel<<<g1(M2, Cp), 256, 0, st>>>(P.Nx, P.Ny, M, w.GH, w.HH, w.Exx, w.Eyy, M2);
+          } else {
+            HIPCHECK(hipMemcpyAsync(w.Exx, w.E, sizeof(cplx) * M2 * Cp, hipMemcpyDeviceToDevice, st));
+            HIPCHECK(hipMemcpyAsync(w.Eyy, w.E, sizeof(cplx) * M2 * Cp, hipMemcpyDeviceToDevice, st));
+          }
+          KCHECK();
+        }
+        {
+          STAGE("omega");
+          omega_rhs_kernel<<<g1(2 * M2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.Exx, w.Eyy, M2, w.G, 2 * M2);
+          orf_zgetrf(st, M, w.E, M, M2, w.ipiv, nullptr, Cp);
+          orf_zgetrs(st, 'L', M, 2 * M, w.E, M, M2, w.ipiv, w.G, M, 2 * M2, Cp);
+          omega_assemble_kernel<<<g1(nf2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.G, 2 * M2, w.Exx, w.Eyy,
+                                                             M2, w.Omega, nf2);
+          KCHECK();
+          if (P.reduced) {
+            sector_project_kernel<<<g1(n2, Cs), 256, 0, st>>>(nf, nr, w.solves, P.dSectors.as<DevSector>(), 0, 0,
+                                                               w.Omega, nf2, 0, w.A0, n2);
+          } else {
+            HIPCHECK(hipMemcpyAsync(w.A0, w.Omega, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+          }
+          KCHECK();
+        }
+        {
+          STAGE("eig");
+          orf_zgeev_timed(st, nr, w.A0, nr, n2, w.mu, w.Wm, nr, n2, w.Zw, w.Yw, w.info, Cs,
+                          [&](const char* nm, int phase) {
+                            if (phase == 0) P.timer.begin(nm, st); else P.timer.end(st);
+                          });
+          KCHECK();
+        }
+        check_info(P, w.info, Cs, "eigensolver (QR iteration did not converge)");
+        {
+          STAGE("modes");
+          lambda_kernel<<<g1((long long)nr * Cs), 256, 0, st>>>(nr, Cs, w.mu, w.lam);
+          cplx* Wf = w.Wm;
+          if (P.reduced) {
+            sector_expand_kernel<<<g1(nfr, Cs), 256, 0, st>>>(nf, nr, w.solves, P.dSectors.as<DevSector>(), w.Wm, n2,
+                                                               w.Wf, nfr);
+            Wf = w.Wf;
+          }
+          // T1 = Eyy W2, T2 = Exx W1 per sector
+          for (int s = 0; s < nSec; ++s) {
+            orf_zgemm(st, 'N', 'N', M, nr, M, ONE, w.Eyy, M, M2, Wf + s * nfr + M, nf, nSec * nfr, ZERO,
+                      w.T12 + s * nfr, nf, nSec * nfr, Cp);
+            orf_zgemm(st, 'N', 'N', M, nr, M, ONE, w.Exx, M, M2, Wf + s * nfr, nf, nSec * nfr, ZERO,
+                      w.T12 + s * nfr + M, nf, nSec * nfr, Cp);
+          }
+          qw_assemble_kernel<<<g1(nfr, Cs), 256, 0, st>>>(M, nr, w.solves, pages, P.dKxy.as<double>(), Wf, nfr, w.T12,
+                                                          nfr, w.QW, nfr);
+          if (P.reduced)
+            sector_rows_kernel<<<g1(n2, Cs), 256, 0, st>>>(nf, nr, nr, w.solves, P.dSectors.as<DevSector>(), w.QW, nfr,
+                                                           w.Vm, n2);
+          else
+            HIPCHECK(hipMemcpyAsync(w.Vm, w.QW, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+          colscale_inv_kernel<<<g1(n2, Cs), 256, 0, st>>>(nr, nr, w.lam, w.Vm, n2);
+          KCHECK();
+        }
+      } else {
+        STAGE("modes");
+        std::vector<cplx> eps(Cs);
+        for (int sp = 0; sp < Cs; ++sp) eps[sp] = C(P.layerBgEps[j][p0 + sp / nSec]);
+        HIPCHECK(hipMemcpyAsync(w.epsSolve, eps.data(), sizeof(cplx) * Cs, hipMemcpyHostToDevice, st));
+        homog_modes_kernel<<<g1(n2, Cs), 256, 0, st>>>(M, nr, P.reduced, w.solves, pages, P.dSectors.as<DevSector>(),
+                                                       P.dKxy.as<double>(), w.epsSolve, w.Wm, w.Vm, n2, w.lam);
+        KCHECK();
+        HIPCHECK(hipStreamSynchronize(st));
+      }
+      // layer S-matrix against the gap (W0 = I, V0 per wavelength and sector)
+      std::vector<double> kL(Cs);
+      for (int sp = 0; sp < Cs; ++sp) {
+        const int pg = p0 + sp / nSec;
+        kL[sp] = P.k0[pg / P.B] * P.thick[(size_t)j * P.B + pg % P.B];
+      }
+      HIPCHECK(hipMemcpyAsync(w.kL, kL.data(), sizeof(double) * Cs, hipMemcpyHostToDevice, st));
+      gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dV0.as<cplx>(), n2, w.X2, n2);
+      smatrix_layer(P, w, Cs, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1]);
+      cplx* SB[4] = {w.Sb[0], w.Sb[1], w.Sb[1], w.Sb[0]};
+      smatrix_star(P, w, Cs, w.Sa, SB, w.Sn);
+      for (int k = 0; k < 4; ++k) std::swap(w.Sa[k], w.Sn[k]);
+      HIPCHECK(hipStreamSynchronize(st));
+      if (progress) {
+        const double frac = (double)(ch * P.nLayers + j + 1) / (double)(nChunks * P.nLayers);
+        char msg[64];
+        snprintf(msg, sizeof msg, "layer %d/%d", j + 1, P.nLayers);
+        if (progress(frac, msg, user)) {
+          HIPCHECK(hipStreamSynchronize(st));
+          fail(ORF_CANCELLED, "cancelled by the progress callback");
+        }
+      }
+    }
+    // S = S * S_trn
+    for (int k = 0; k < 4; ++k)
+      gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dStrn.as<cplx>() + k * n2 * G, n2, w.Sb[k], n2);
+    smatrix_star(P, w, Cs, w.Sa, w.Sb, w.Sn);
+    {
+      STAGE("output");
+      output_kernel<<<dim3(Cs, P.nPol), 256, 0, st>>>(
+          M, nr, P.reduced, P.B, P.L, w.solves, basePages, P.dSectors.as<DevSector>(), P.dSources.as<DevSource>(),
+          P.dKxy.as<double>(), (const double*)P.dKz.p, w.Sn[0], w.Sn[2], n2, P.dR.as<double>(), P.dT.as<double>(),
+          P.dRtot.as<double>(), P.dTtot.as<double>(), P.dr.as<cplx>(), P.dt.as<cplx>(), P.i0);
+      KCHECK();
+    }
+    if (progress && P.nLayers == 0) {
+      if (progress((double)(ch + 1) / nChunks, "done", user)) fail(ORF_CANCELLED, "cancelled by the progress callback");
+    }
+  }
+  HIPCHECK(hipStreamSynchronize(st));
+  P.timer.collect();
+  P.haveResult = true;
+}
+
+static void plan_fetch(orf_plan& P, orf_result* out) {
+  if (!P.haveResult) fail(ORF_INVALID_ARG, "plan has not been executed");
+  if (!out) fail(ORF_INVALID_ARG, "result is NULL");
+  const size_t nout = (size_t)P.B * P.L * P.nPol;
+  if (out->m) for (int p = 0; p < P.M; ++p) out->m[p] = P.mIdx[p];
+  if (out->n) for (int p = 0; p < P.M; ++p) out->n[p] = P.nIdx[p];
+  if (out->R) HIPCHECK(hipMemcpy(out->R, P.dR.p, sizeof(double) * P.M * nout, hipMemcpyDeviceToHost));
+  if (out->T) HIPCHECK(hipMemcpy(out->T, P.dT.p, sizeof(double) * P.M * nout, hipMemcpyDeviceToHost));
+  if (out->Rtot) HIPCHECK(hipMemcpy(out->Rtot, P.dRtot.p, sizeof(double) * nout, hipMemcpyDeviceToHost));
+  if (out->Ttot) HIPCHECK(hipMemcpy(out->Ttot, P.dTtot.p, sizeof(double) * nout, hipMemcpyDeviceToHost));
+  if (out->r) HIPCHECK(hipMemcpy(out->r, P.dr.p, sizeof(cplx) * 3 * P.M * nout, hipMemcpyDeviceToHost));
+  if (out->t) HIPCHECK(hipMemcpy(out->t, P.dt.p, sizeof(cplx) * 3 * P.M * nout, hipMemcpyDeviceToHost));
+}
+
+// ====================================================================== C ABI
+#define ORF_TRY(...)                         \
+  try {                                      \
+    __VA_ARGS__;                             \
+    return ORF_OK;                           \
+  } catch (const OrfError& e) {              \
+    g_err = e.msg;                           \
+    return e.code;                           \
+  } catch (const std::bad_alloc&) {          \
+    g_err = "host out of memory";            \
+    return ORF_OOM;                          \
+  } catch (const std::exception& e) {        \
+    g_err = e.what();                        \
+    return ORF_DEVICE_ERROR;                 \
+  }
+
+extern "C" {
+
+const char* orf_last_error(void) { return g_err.c_str(); }
+
+int orf_init(int device, orf_ctx** out) {
+  ORF_TRY({
+    if (!out) fail(ORF_INVALID_ARG, "ctx is NULL");
+    int count = 0;
+    HIPCHECK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) fail(ORF_DEVICE_ERROR, "no such HIP device");
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      fail(ORF_DEVICE_ERROR, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+    HIPCHECK(hipSetDevice(device));
+    orf_ctx* c = new orf_ctx();
+    c->device = device;
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    // leave room for each kernel's static LDS
+    const int lim = (int)std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024) - 20 * 1024;
+    orf_set_lds_limit(std::max(lim, 32 * 1024));
+    *out = c;
+  })
+}
+
+void orf_release(orf_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int orf_harmonic_count(const orf_problem* p) {
+  if (!p) return 0;
+  const int Ny = std::isfinite(p->Ly) ? p->Ny : 0;
+  return (2 * p->Nx + 1) * (2 * Ny + 1);
+}
+
+int orf_plan_create(orf_ctx* ctx, const orf_problem* p, orf_plan** out) {
+  orf_plan* P = nullptr;
+  try {
+    if (!ctx || !out) fail(ORF_INVALID_ARG, "ctx or plan is NULL");
+    HIPCHECK(hipSetDevice(ctx->device));
+    P = new orf_plan();
+    plan_create(ctx, p, *P);
+    *out = P;
+    return ORF_OK;
+  } catch (const OrfError& e) {
+    delete P;
+    g_err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    delete P;
+    g_err = e.what();
+    return ORF_DEVICE_ERROR;
+  }
+}
+
+int orf_plan_execute(orf_plan* plan, orf_progress_fn progress, void* user) {
+  ORF_TRY({
+    if (!plan) fail(ORF_INVALID_ARG, "plan is NULL");
+    HIPCHECK(hipSetDevice(plan->ctx->device));
+    plan_execute(*plan, progress, user);
+  })
+}
+
+int orf_plan_fetch(orf_plan* plan, orf_result* out) {
+  ORF_TRY({
+    if (!plan) fail(ORF_INVALID_ARG, "plan is NULL");
+    plan_fetch(*plan, out);
+  })
+}
+
+void orf_plan_destroy(orf_plan* plan) { delete plan; }
+
+int orf_plan_info(const orf_plan* plan, int32_t* solved_size, int32_t* symmetric, int32_t* n_solve_pages) {
+  if (!plan) return ORF_INVALID_ARG;
+  if (solved_size) *solved_size = plan->nr;
+  if (symmetric) *symmetric = plan->reduced;
+  if (n_solve_pages) *n_solve_pages = plan->B * plan->L * plan->nSec;
+  return ORF_OK;
+}
+
+void orf_plan_enable_timing(orf_plan* plan, int32_t on) {
+  if (plan) plan->timer.on = on != 0;
+}
+
+int orf_plan_stage_times(const orf_plan* plan, int32_t max_stages, const char** names, double* total_ms,
+                         int32_t* launches, int32_t* n_stages) {
+  if (!plan) return ORF_INVALID_ARG;
+  int k = 0;
+  for (auto& nm : plan->timer.order) {
+    if (k >= max_stages) break;
+    auto it = plan->timer.acc.find(nm);
+    if (names) names[k] = it->first.c_str();
+    if (total_ms) total_ms[k] = it->second.first;
+    if (launches) launches[k] = it->second.second;
+    ++k;
+  }
+  if (n_stages) *n_stages = k;
+  return ORF_OK;
+}
+
+int orf_rcwa2d(orf_ctx* ctx, const orf_problem* p, orf_result* out, orf_progress_fn progress, void* user) {
+  orf_plan* plan = nullptr;
+  int rc = orf_plan_create(ctx, p, &plan);
+  if (rc != ORF_OK) return rc;
+  rc = orf_plan_execute(plan, progress, user);
+  if (rc == ORF_OK) rc = orf_plan_fetch(plan, out);
+  orf_plan_destroy(plan);
+  return rc;
+}
+
+// ---------------------------------------------------------------- page primitives
+int orf_pagemtimes(orf_ctx* ctx, char opA, char opB, int32_t m, int32_t n, int32_t k, const double* A,
+                   int64_t strideA, const double* B, int64_t strideB, double* Cout, int32_t batch) {
+  ORF_TRY({
+    if (!ctx || m <= 0 || n <= 0 || k <= 0 || batch <= 0) fail(ORF_INVALID_ARG, "bad sizes");
+    HIPCHECK(hipSetDevice(ctx->device));
+    const int lda = (opA == 'N') ? m : k, ldb = (opB == 'N') ? k : n;
+    const size_t na = (size_t)(strideA ? batch : 1) * (size_t)m * k, nb = (size_t)(strideB ? batch : 1) * (size_t)k * n;
+    DBuf a, b, c;
+    a.alloc(na * 16);
+    b.alloc(nb * 16);
+    c.alloc((size_t)batch * m * n * 16);
+    HIPCHECK(hipMemcpy(a.p, A, na * 16, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(b.p, B, nb * 16, hipMemcpyHostToDevice));
+    orf_zgemm(ctx->stream, opA, opB, m, n, k, ONE, a.as<cplx>(), lda, strideA ? (long long)m * k : 0, b.as<cplx>(), ldb,
+              strideB ? (long long)k * n : 0, ZERO, c.as<cplx>(), m, (long long)m * n, batch);
+    KCHECK();
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(Cout, c.p, (size_t)batch * m * n * 16, hipMemcpyDeviceToHost));
+  })
+}
+
+int orf_pagemldivide(orf_ctx* ctx, int32_t n, int32_t nrhs, const double* A, int64_t strideA, const double* B,
+                     double* X, int32_t batch) {
+  ORF_TRY({
+    if (!ctx || n <= 0 || nrhs <= 0 || batch <= 0) fail(ORF_INVALID_ARG, "bad sizes");
+    HIPCHECK(hipSetDevice(ctx->device));
+    const size_t n2 = (size_t)n * n;
+    DBuf a, b, ip;
+    a.alloc((size_t)batch * n2 * 16);
+    b.alloc((size_t)batch * n * nrhs * 16);
+    ip.alloc(sizeof(int) * (size_t)n * batch);
+    for (int q = 0; q < batch; ++q)
+      HIPCHECK(hipMemcpy(a.as<cplx>() + q * n2, A + (strideA ? 2 * n2 * q : 0), n2 * 16, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(b.p, B, (size_t)batch * n * nrhs * 16, hipMemcpyHostToDevice));
+    orf_zgetrf(ctx->stream, n, a.as<cplx>(), n, n2, ip.as<int>(), nullptr, batch);
+    orf_zgetrs(ctx->stream, 'L', n, nrhs, a.as<cplx>(), n, n2, ip.as<int>(), b.as<cplx>(), n, (long long)n * nrhs, batch);
+    KCHECK();
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(X, b.p, (size_t)batch * n * nrhs * 16, hipMemcpyDeviceToHost));
+  })
+}
+
+int orf_pagemrdivide(orf_ctx* ctx, int32_t n, int32_t nrows, const double* B, const double* A, int64_t strideA,
+                     double* X, int32_t batch) {
+  ORF_TRY({
+    if (!ctx || n <= 0 || nrows <= 0 || batch <= 0) fail(ORF_INVALID_ARG, "bad sizes");
+    HIPCHECK(hipSetDevice(ctx->device));
+    const size_t n2 = (size_t)n * n;
+    DBuf a, b, ip;
+    a.alloc((size_t)batch * n2 * 16);
+    b.alloc((size_t)batch * n * nrows * 16);
+    ip.alloc(sizeof(int) * (size_t)n * batch);
+    for (int q = 0; q < batch; ++q)
+      HIPCHECK(hipMemcpy(a.as<cplx>() + q * n2, A + (strideA ? 2 * n2 * q : 0), n2 * 16, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(b.p, B, (size_t)batch * n * nrows * 16, hipMemcpyHostToDevice));
+    orf_zgetrf(ctx->stream, n, a.as<cplx>(), n, n2, ip.as<int>(), nullptr, batch);
+    orf_zgetrs(ctx->stream, 'R', n, nrows, a.as<cplx>(), n, n2, ip.as<int>(), b.as<cplx>(), nrows,
+               (long long)n * nrows, batch);
+    KCHECK();
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(X, b.p, (size_t)batch * n * nrows * 16, hipMemcpyDeviceToHost));
+  })
+}
+
+int orf_pageeig(orf_ctx* ctx, int32_t n, const double* A, double* wout, double* Vout, int32_t batch) {
+  ORF_TRY({
+    if (!ctx || n <= 0 || batch <= 0) fail(ORF_INVALID_ARG, "bad sizes");
+    if (2 * ((n + 63) / 64) > 24 || n > 768) fail(ORF_INVALID_ARG, "n > 768 not supported");
+    HIPCHECK(hipSetDevice(ctx->device));
+    const size_t n2 = (size_t)n * n;
+    DBuf a, wv, v, z, y, info;
+    a.alloc(batch * n2 * 16);
+    wv.alloc((size_t)batch * n * 16);
+    v.alloc(batch * n2 * 16);
+    z.alloc(batch * n2 * 16);
+    y.alloc(batch * n2 * 16);
+    info.alloc(sizeof(int) * batch);
+    HIPCHECK(hipMemcpy(a.p, A, batch * n2 * 16, hipMemcpyHostToDevice));
+    orf_zgeev(ctx->stream, n, a.as<cplx>(), n, n2, wv.as<cplx>(), v.as<cplx>(), n, n2, z.as<cplx>(), y.as<cplx>(),
+              info.as<int>(), batch);
+    KCHECK();
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    std::vector<int> h(batch);
+    HIPCHECK(hipMemcpy(h.data(), info.p, sizeof(int) * batch, hipMemcpyDeviceToHost));
+    for (int q = 0; q < batch; ++q)
+      if (h[q]) fail(ORF_DEVICE_ERROR, "QR iteration did not converge on page " + std::to_string(q));
+    HIPCHECK(hipMemcpy(wout, wv.p, (size_t)batch * n * 16, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(Vout, v.p, batch * n2 * 16, hipMemcpyDeviceToHost));
+  })
+}
+
+}  // extern "C"
