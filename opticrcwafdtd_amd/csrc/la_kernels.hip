@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(
     }
     __syncthreads();
   }
-  // ---- epilogue: lane holds rows 4*(lane/16)+v, column lane%16 of each 16x16 block
+  // ---- epilogue: lane holds rows lane/16 + 4 v, column lane%16 of each 16x16 block
   const bool beta0 = (beta.x == 0.0 && beta.y == 0.0);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        int gr = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
+        int gr = m0 + wm + 16 * i + (lane >> 4) + 4 * v;
         int gc = n0 + wn + 16 * j + (lane & 15);
         if (gr < M && gc < N) {
           cplx acc = cmk(accR[i][j][v], accI[i][j][v]);
