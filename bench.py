@@ -1,0 +1,166 @@
+"""Throughput benchmark of the gfx950 RCWA solve on the metalens phase-library
+workload: a-Si pillars (n = 3.6 + 0.01i, h = 0.6 um) in air on SiO2 (lit from
+the SiO2 side), square period 0.4 um, 256 radii 40-180 nm x 8 wavelengths
+900-980 nm, harmonics +-5 (121 orders, 242 x 242 complex128 eigenproblem),
+eps grid 256 x 256, TE and TM: 2048 (cell, wavelength) pages = 4096 solves.
+
+One step = one execution of the resident plan over the whole batch (inputs
+already in HBM).  Prints ONE JSON line.
+"""
+import argparse
+import json
+import math
+import multiprocessing as mp
+import os
+import sys
+import time
+
+for _v in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS", "MKL_NUM_THREADS"):
+    os.environ[_v] = "1"   # one BLAS thread per CPU-baseline worker
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "unit-cell solves/s at ±5 harm. (|ΔT|≤1e-6 vs FP64) + p50/p99 request latency"
+PEAK_FP64_TFLOPS = 78.6
+
+N_HARM = 5
+RADII = np.linspace(0.040, 0.180, 256)
+LAMBDAS = np.linspace(0.900, 0.980, 8)
+
+
+def make_devs(orf, radii):
+    mc = orf.materialConst
+    sio2, air, asi = mc(1.45, "SiO2"), mc(1.0, "air"), mc(3.6 + 0.01j, "a-Si")
+    return [orf.makeStructure([0.4, 0.4], sio2, air, [orf.makeLayer(0.6, air, [orf.shapeCircle(r, asi)])])
+            for r in radii]
+
+
+# ---------------------------------------------------------------- CPU baseline (oracle)
+def _cpu_worker(args):
+    radii, lam, sym = args
+    import opticrcwafdtd_amd.structure as S
+    from oracle import rcwa_oracle as O
+
+    class _NS:
+        materialConst = staticmethod(S.materialConst)
+        makeStructure = staticmethod(S.makeStructure)
+        makeLayer = staticmethod(S.makeLayer)
+        shapeCircle = staticmethod(S.shapeCircle)
+    devs = make_devs(_NS, radii)
+    t0 = time.perf_counter()
+    O.rcwa2d(devs, lam, 0.0, 0.0, "s", Nx=N_HARM, Nsample=(256, 256), Symmetry=sym)
+    return time.perf_counter() - t0, len(radii)
+
+
+def cpu_baseline(sym, workers, per_worker=12):
+    """The FP64 oracle (numpy + LAPACK, one thread per process) on a bounded
+    sample of the workload: workers x per_worker single-polarisation solves."""
+    jobs = []
+    rng = np.random.default_rng(0)
+    for w in range(workers):
+        rs = rng.choice(RADII, size=per_worker, replace=False)
+        jobs.append((list(rs), float(LAMBDAS[w % len(LAMBDAS)]), sym))
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(workers) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    wall = time.perf_counter() - t0
+    solves = sum(r[1] for r in res)
+    return {"value": solves / wall, "unit": "solves/s", "cores": workers, "kind": "port",
+            "sample": f"{solves} single-polarisation solves (Symmetry={sym}) of the same workload, "
+                      f"{per_worker} cells per process, {workers} processes x 1 BLAS thread, "
+                      f"{wall:.1f} s wall"}
+
+
+# ---------------------------------------------------------------- algorithmic flops
+def schur_qr_flops(n):
+    """Complex QR iteration to Schur form with Schur vectors: Golub & Van Loan's
+    25 n^3 (Hessenberg + QR + Q) minus the Hessenberg/Q share (14/3 n^3),
+    complex arithmetic counted as 4 real flops."""
+    return 4.0 * (25.0 - 14.0 / 3.0) * n ** 3
+
+
+def solve_flops(n, M):
+    """Canonical real flops of one single-patterned-layer solve:
+    [eps]^-1 8 M^3 + P Q 8 n^3 + eig 100 n^3 + V 8 n^3 + layer S 90 2/3 n^3
+    + 2 star products 85 1/3 n^3 each."""
+    return 8.0 * M ** 3 + (8 + 100 + 8 + 90 + 2.0 / 3 + 2 * (85 + 1.0 / 3)) * n ** 3
+
+
+def run(args):
+    import opticrcwafdtd_amd as orf
+    devs = make_devs(orf, RADII)
+    sym = args.symmetry
+    prob = orf.Problem(devs, list(LAMBDAS), 0.0, 0.0, ["s", "p"], Nx=N_HARM, Nsample=(256, 256), Symmetry=sym)
+    plan = orf.Plan(prob)
+    for _ in range(args.warmup):
+        plan.execute()
+    plan.enable_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.execute()
+    t1 = time.perf_counter()
+    plan.enable_timing(False)
+    stages = plan.stage_times()
+    solves = len(RADII) * len(LAMBDAS) * 2
+    out = plan.fetch(amplitudes=False)
+    plan.close()
+    return {"plan": plan, "elapsed": t1 - t0, "stages": stages, "solves": solves, "n": plan.solved_size,
+            "symmetric": plan.symmetric, "pages": plan.solve_pages, "Ttot": out["Ttot"]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--symmetry", default="none", choices=["none", "auto"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every unit cell is an independent solve, so N GPUs would each run "
+              "their own share of the batch (see DESIGN.md); multi-GPU runs are not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+    cpu = None if args.no_cpu else cpu_baseline(args.symmetry, args.cpu_workers)
+    r = run(args)
+    ms = 1e3 * r["elapsed"] / args.steps
+    value = r["solves"] * args.steps / r["elapsed"]
+    # dominant kernel over the timed region (HIP events on the plan's stream)
+    kernels = {k: v for k, v in r["stages"].items() if k.startswith("eig_")}
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    tot_ms, launches = kernels[dom]
+    n = r["n"]
+    pages = r["pages"]
+    flops = schur_qr_flops(n) * pages if dom == "eig_schur_qr" else None
+    avg_s = tot_ms / launches / 1e3
+    achieved = flops / avg_s / 1e12 if flops else None
+    M = (2 * N_HARM + 1) ** 2
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "metalens phase library: a-Si pillars r 40-180 nm x 256, lambda 900-980 nm x 8, "
+                               "+-5 harmonics, eps 256x256, TE+TM, FP64",
+                   "global_batch": r["solves"], "solved_matrix": n, "symmetry": args.symmetry,
+                   "pages": pages, "parallelism": "single GPU, batched pages"},
+        "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 4) if achieved else None,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 5) if achieved else None,
+                     "traffic": None, "avg_launch_ms": round(tot_ms / launches, 3),
+                     "flops_per_launch": flops},
+        "whole_solve": {"canonical_flops_per_page": solve_flops(n, M if args.symmetry == "none" else 0),
+                        "achieved_tflops": round(solve_flops(n, M if args.symmetry == "none" else 0) *
+                                                 pages / (ms / 1e3) / 1e12, 4)},
+        "stages_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in r["stages"].items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

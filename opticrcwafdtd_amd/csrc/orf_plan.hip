@@ -87,6 +87,7 @@ struct StageTimer {
   bool on = false;
   struct Rec { const char* name; hipEvent_t a, b; };
   std::vector<Rec> pending;
+  std::vector<size_t> open;  // indices of begun, not yet ended records (stages nest)
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, int>> acc;
   std::vector<std::string> order;
@@ -100,11 +101,13 @@ struct StageTimer {
     if (!on) return;
     Rec r{name, get(), get()};
     HIPCHECK(hipEventRecord(r.a, st));
+    open.push_back(pending.size());
     pending.push_back(r);
   }
   void end(hipStream_t st) {
-    if (!on) return;
-    HIPCHECK(hipEventRecord(pending.back().b, st));
+    if (!on || open.empty()) return;
+    HIPCHECK(hipEventRecord(pending[open.back()].b, st));
+    open.pop_back();
   }
   void collect() {
     for (auto& r : pending) {
@@ -119,7 +122,7 @@ struct StageTimer {
     }
     pending.clear();
   }
-  void reset() { acc.clear(); order.clear(); }
+  void reset() { acc.clear(); order.clear(); open.clear(); }
   ~StageTimer() {
     for (auto e : pool) (void)hipEventDestroy(e);
   }
@@ -934,7 +937,7 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
   const int npages = P.B * P.L;
   const int Cmax = P.chunkPages;
   Work w = make_work(P, Cmax);
-  P.timer.reset();
+  if (P.timer.on) P.timer.reset();
   const int nChunks = (npages + Cmax - 1) / Cmax;
   const int M = P.M, nf = P.nf, nr = P.nr, nSec = P.nSec;
   const long long M2 = (long long)M * M, nf2 = (long long)nf * nf, n2 = (long long)nr * nr;
