@@ -354,6 +354,369 @@ __global__ __launch_bounds__(256) void zgetrs_right_kernel(int n, int nrows, int
     Bg[(size_t)(r0 + r) + (size_t)c * ldb] = X[r + c * RB];
   }
 }
+
+// ---------------------------------------------------------------- blocked kernels
+// One 16x16 complex tile on MFMA: acc += A (16 x K) * B (K x 16), K = 4 * ksteps.
+// fa(row, k) and fb(k, col) return operand entries (zero outside the matrix).
+// Lane layout of v_mfma_f64_16x16x4f64: A[l%16][l/16], B[l/16][l%16],
+// C/D rows l/16 + 4v (v = 0..3), column l%16.
+template <int KS, class FA, class FB>
+ORF_DEV void ztile_mfma(FA fa, FB fb, v4d& accR, v4d& accI) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15, kq = lane >> 4;
+  cplx av[KS], bv[KS];
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    av[s] = fa(r, 4 * s + kq);
+    bv[s] = fb(4 * s + kq, r);
+  }
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    accR = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].x, bv[s].x, accR, 0, 0, 0);
+    accR = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[s].y, bv[s].y, accR, 0, 0, 0);
+    accI = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].x, bv[s].y, accI, 0, 0, 0);
+    accI = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].y, bv[s].x, accI, 0, 0, 0);
+  }
+}
+
+// Right-looking blocked LU with partial pivoting, one workgroup per page.
+// The LB-wide panel and the LB rows of U12 live in LDS; the trailing update
+// A22 -= L21 U12 runs on MFMA tiles; row interchanges reach the other
+// columns in one pass per panel (LAPACK zgetrf order: ipiv[k] is the row
+// swapped with row k when column k is factored).
+template <int LB>
+__global__ __launch_bounds__(256) void zgetrf_blocked_kernel(int n, cplx* __restrict__ Ag, int lda,
+                                                             long long sA, int* __restrict__ ipiv,
+                                                             int* __restrict__ info) {
+  extern __shared__ cplx sm[];
+  __shared__ double rv[4];
+  __shared__ int ri[4];
+  __shared__ int spiv[LB];
+  cplx* Pn = sm;                  // panel, (n - kb) x LB, ld n
+  cplx* U = sm + (size_t)n * LB;  // LB x (n - kb - LB), row-major with ld n
+  const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
+  const int lane = t & 63, wv = t >> 6;
+  cplx* A = Ag + (size_t)page * sA;
+  ipiv += (size_t)page * n;
+  int sing = 0;
+  for (int kb = 0; kb < n; kb += LB) {
+    const int nb = min(LB, n - kb), m = n - kb;
+    for (int e = t; e < m * nb; e += nt) {
+      const int r = e % m, c = e / m;
+      Pn[r + (size_t)c * n] = A[(size_t)(kb + r) + (size_t)(kb + c) * lda];
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+      double best = -1.0;
+      int bi = m;
+      for (int r = j + t; r < m; r += nt) {
+        const double v = cabs1(Pn[r + (size_t)j * n]);
+        if (v > best) { best = v; bi = r; }
+      }
+      block_argmax(best, bi, rv, ri);
+      if (t == 0) { spiv[j] = bi; ipiv[kb + j] = kb + bi; }
+      if (best == 0.0) {
+        if (!sing) sing = kb + j + 1;
+        __syncthreads();
+        continue;
+      }
+      if (bi != j && t < nb) {
+        const cplx x = Pn[j + (size_t)t * n];
+        Pn[j + (size_t)t * n] = Pn[bi + (size_t)t * n];
+        Pn[bi + (size_t)t * n] = x;
+      }
+      __syncthreads();
+      const cplx rp = crecip(Pn[j + (size_t)j * n]);
+      for (int r = j + 1 + t; r < m; r += nt) Pn[r + (size_t)j * n] = cmul(Pn[r + (size_t)j * n], rp);
+      __syncthreads();
+      for (int e = t; e < (m - j - 1) * (nb - j - 1); e += nt) {
+        const int r = j + 1 + e % (m - j - 1), c = j + 1 + e / (m - j - 1);
+        Pn[r + (size_t)c * n] = csub(Pn[r + (size_t)c * n], cmul(Pn[r + (size_t)j * n], Pn[j + (size_t)c * n]));
+      }
+      __syncthreads();
+    }
+    // panel back to global
+    for (int e = t; e < m * nb; e += nt) {
+      const int r = e % m, c = e / m;
+      A[(size_t)(kb + r) + (size_t)(kb + c) * lda] = Pn[r + (size_t)c * n];
+    }
+    // other columns: apply the panel's interchanges; right of the panel also U12 = L11^-1 A12
+    const int nr = n - kb - nb;
+    for (int c = t; c < n - nb; c += nt) {
+      const int col = c < kb ? c : c + nb;
+      cplx* pc = A + (size_t)col * lda;
+      for (int j = 0; j < nb; ++j) {
+        const int p = kb + spiv[j];
+        if (p != kb + j) {
+          const cplx x = pc[kb + j];
+          pc[kb + j] = pc[p];
+          pc[p] = x;
+        }
+      }
+      if (col >= kb + nb) {
+        cplx x[LB];
+#pragma unroll
+        for (int j = 0; j < LB; ++j) x[j] = (j < nb) ? pc[kb + j] : cmk(0.0, 0.0);
+#pragma unroll
+        for (int j = 1; j < LB; ++j)
+#pragma unroll
+          for (int q = 0; q < j; ++q)
+            if (j < nb) x[j] = csub(x[j], cmul(Pn[j + (size_t)q * n], x[q]));
+#pragma unroll
+        for (int j = 0; j < LB; ++j)
+          if (j < nb) {
+            pc[kb + j] = x[j];
+            U[(size_t)j * n + (col - kb - nb)] = x[j];
+          }
+      }
+    }
+    __syncthreads();
+    // trailing update on MFMA tiles
+    if (nr > 0) {
+      const int tr = (nr + 15) / 16;
+      for (int tile = wv; tile < tr * tr; tile += nt >> 6) {
+        const int r0 = (tile % tr) * 16, c0 = (tile / tr) * 16;
+        v4d aR = {0, 0, 0, 0}, aI = {0, 0, 0, 0};
+        ztile_mfma<LB / 4>(
+                   [&](int r, int k) -> cplx {
+                     return (r0 + r < nr && k < nb) ? Pn[(nb + r0 + r) + (size_t)k * n] : cmk(0.0, 0.0);
+                   },
+                   [&](int k, int c) -> cplx {
+                     return (k < nb && c0 + c < nr) ? U[(size_t)k * n + c0 + c] : cmk(0.0, 0.0);
+                   },
+                   aR, aI);
+        const int cc = c0 + (lane & 15);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int rr = r0 + (lane >> 4) + 4 * v;
+          if (rr < nr && cc < nr) {
+            cplx* p = A + (size_t)(kb + nb + rr) + (size_t)(kb + nb + cc) * lda;
+            *p = csub(*p, cmk(aR[v], aI[v]));
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0 && info) info[page] = sing;
+}
+
+// Inverses of the 16x16 diagonal blocks of the LU factors, once per page:
+// Linv (unit lower) and Uinv (upper), padded with the identity beyond n.
+// Layout per page: [Linv blk 0..nblk-1][Uinv blk 0..nblk-1], 256 complex each.
+__global__ __launch_bounds__(256) void diag_inv_kernel(int n, const cplx* __restrict__ LU, int lda,
+                                                       long long sA, cplx* __restrict__ dinv, long long sD) {
+  __shared__ cplx T[256], X[256];
+  const int blk = blockIdx.x, page = blockIdx.y, t = threadIdx.x;
+  const int nblk = (n + 15) / 16, kb = blk * 16, nb = min(16, n - kb);
+  LU += (size_t)page * sA;
+  cplx* out = dinv + (size_t)page * sD;
+  const int r = t & 15, c = t >> 4;  // element (r, c)
+  T[t] = (r < nb && c < nb) ? LU[(size_t)(kb + r) + (size_t)(kb + c) * lda] : cmk(r == c ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  // lower unit: column c of L^-1 by forward substitution, 16 lanes per column
+  X[t] = cmk(r == c ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  for (int j = 1; j < 16; ++j) {
+    // x[j] -= sum_{q<j} L[j][q] x[q]; lane r = q contributes
+    cplx v = (r < j) ? cmul(T[j + 16 * r], X[r + 16 * c]) : cmk(0.0, 0.0);
+    for (int o = 8; o > 0; o >>= 1) { v.x += __shfl_xor(v.x, o, 16); v.y += __shfl_xor(v.y, o, 16); }
+    __syncthreads();
+    if (r == j) X[j + 16 * c] = csub(X[j + 16 * c], v);
+    __syncthreads();
+  }
+  out[(size_t)blk * 256 + t] = X[t];
+  __syncthreads();
+  // upper: column c of U^-1 by backward substitution
+  X[t] = cmk(r == c ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  for (int j = 15; j >= 0; --j) {
+    cplx v = (r > j) ? cmul(T[j + 16 * r], X[r + 16 * c]) : cmk(0.0, 0.0);
+    for (int o = 8; o > 0; o >>= 1) { v.x += __shfl_xor(v.x, o, 16); v.y += __shfl_xor(v.y, o, 16); }
+    __syncthreads();
+    if (r == j) X[j + 16 * c] = cdiv(csub(X[j + 16 * c], v), T[j + 16 * j]);
+    __syncthreads();
+  }
+  out[(size_t)(nblk + blk) * 256 + t] = X[t];
+}
+
+// Left solve A X = B with A = P L U: per 16-row block, X_blk = D^-1 X_blk then
+// the rows below (above) are updated; both steps are MFMA tile products on an
+// LDS block of CB right-hand sides.
+__global__ __launch_bounds__(256) void ztrsm_left_kernel(int n, int nrhs, int CB, const cplx* __restrict__ LU,
+                                                         int lda, long long sA, const int* __restrict__ ipiv,
+                                                         const cplx* __restrict__ dinv, long long sD,
+                                                         cplx* __restrict__ Bg, int ldb, long long sB) {
+  extern __shared__ cplx X[];  // n x CB, ld n
+  const int page = blockIdx.y, c0 = blockIdx.x * CB, t = threadIdx.x, nt = blockDim.x;
+  const int lane = t & 63, wv = t >> 6, nw = nt >> 6;
+  const int nblk = (n + 15) / 16;
+  LU += (size_t)page * sA;
+  ipiv += (size_t)page * n;
+  Bg += (size_t)page * sB;
+  const cplx* Linv = dinv + (size_t)page * sD;
+  const cplx* Uinv = Linv + (size_t)nblk * 256;
+  const int nc = min(CB, nrhs - c0);
+  for (int e = t; e < n * nc; e += nt) {
+    const int r = e % n, c = e / n;
+    X[r + c * n] = Bg[(size_t)r + (size_t)(c0 + c) * ldb];
+  }
+  __syncthreads();
+  if (t < 64) {
+    for (int k = 0; k < n; ++k) {
+      const int p = ipiv[k];
+      if (p != k)
+        for (int c = t; c < nc; c += 64) {
+          const cplx a = X[k + c * n];
+          X[k + c * n] = X[p + c * n];
+          X[p + c * n] = a;
+        }
+    }
+  }
+  __syncthreads();
+  const int tcn = (nc + 15) / 16;
+  for (int pass = 0; pass < 2; ++pass) {  // 0: forward with L, 1: backward with U
+    for (int bi = 0; bi < nblk; ++bi) {
+      const int bk = pass == 0 ? bi : nblk - 1 - bi;
+      const int kb = bk * 16, nb = min(16, n - kb);
+      const cplx* D = (pass == 0 ? Linv : Uinv) + (size_t)bk * 256;
+      // X_blk = D X_blk (in place: each wave owns whole column tiles)
+      for (int ct = wv; ct < tcn; ct += nw) {
+        const int cc0 = ct * 16;
+        v4d aR = {0, 0, 0, 0}, aI = {0, 0, 0, 0};
+        ztile_mfma<4>([&](int r, int k) -> cplx { return D[r + 16 * k]; },
+                      [&](int k, int c) -> cplx {
+                        return (k < nb && cc0 + c < nc) ? X[kb + k + (cc0 + c) * n] : cmk(0.0, 0.0);
+                      },
+                      aR, aI);
+        const int cc = cc0 + (lane & 15);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int rr = (lane >> 4) + 4 * v;
+          if (rr < nb && cc < nc) X[kb + rr + cc * n] = cmk(aR[v], aI[v]);
+        }
+      }
+      __syncthreads();
+      // rows below (forward) or above (backward)
+      const int rlo = pass == 0 ? kb + nb : 0, rhi = pass == 0 ? n : kb;
+      const int rows = rhi - rlo;
+      if (rows > 0) {
+        const int trn = (rows + 15) / 16;
+        for (int tile = wv; tile < trn * tcn; tile += nw) {
+          const int r0 = rlo + (tile % trn) * 16, cc0 = (tile / trn) * 16;
+          v4d aR = {0, 0, 0, 0}, aI = {0, 0, 0, 0};
+          ztile_mfma<4>(
+              [&](int r, int k) -> cplx {
+                return (r0 + r < rhi && k < nb) ? LU[(size_t)(r0 + r) + (size_t)(kb + k) * lda] : cmk(0.0, 0.0);
+              },
+              [&](int k, int c) -> cplx {
+                return (k < nb && cc0 + c < nc) ? X[kb + k + (cc0 + c) * n] : cmk(0.0, 0.0);
+              },
+              aR, aI);
+          const int cc = cc0 + (lane & 15);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int rr = r0 + (lane >> 4) + 4 * v;
+            if (rr < rhi && cc < nc) X[rr + cc * n] = csub(X[rr + cc * n], cmk(aR[v], aI[v]));
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = t; e < n * nc; e += nt) {
+    const int r = e % n, c = e / n;
+    Bg[(size_t)r + (size_t)(c0 + c) * ldb] = X[r + c * n];
+  }
+}
+
+// Right solve X A = B (B nrows x n) with A = P L U; LDS block of RB rows.
+// Z U = B left to right, then Y L = Z right to left, then the column swaps.
+__global__ __launch_bounds__(256) void ztrsm_right_kernel(int n, int nrows, int RB, const cplx* __restrict__ LU,
+                                                          int lda, long long sA, const int* __restrict__ ipiv,
+                                                          const cplx* __restrict__ dinv, long long sD,
+                                                          cplx* __restrict__ Bg, int ldb, long long sB) {
+  extern __shared__ cplx X[];  // RB x n, ld RB
+  const int page = blockIdx.y, r0g = blockIdx.x * RB, t = threadIdx.x, nt = blockDim.x;
+  const int lane = t & 63, wv = t >> 6, nw = nt >> 6;
+  const int nblk = (n + 15) / 16;
+  LU += (size_t)page * sA;
+  ipiv += (size_t)page * n;
+  Bg += (size_t)page * sB;
+  const cplx* Linv = dinv + (size_t)page * sD;
+  const cplx* Uinv = Linv + (size_t)nblk * 256;
+  const int nr = min(RB, nrows - r0g);
+  for (int e = t; e < nr * n; e += nt) {
+    const int r = e % nr, c = e / nr;
+    X[r + c * RB] = Bg[(size_t)(r0g + r) + (size_t)c * ldb];
+  }
+  __syncthreads();
+  const int trr = (nr + 15) / 16;
+  for (int pass = 0; pass < 2; ++pass) {  // 0: U left to right, 1: L right to left
+    for (int bi = 0; bi < nblk; ++bi) {
+      const int bk = pass == 0 ? bi : nblk - 1 - bi;
+      const int kb = bk * 16, nb = min(16, n - kb);
+      const cplx* D = (pass == 0 ? Uinv : Linv) + (size_t)bk * 256;
+      // X_blk = X_blk D (in place: each wave owns whole row tiles)
+      for (int rt = wv; rt < trr; rt += nw) {
+        const int rr0 = rt * 16;
+        v4d aR = {0, 0, 0, 0}, aI = {0, 0, 0, 0};
+        ztile_mfma<4>(
+            [&](int r, int k) -> cplx {
+              return (rr0 + r < nr && k < nb) ? X[rr0 + r + (kb + k) * RB] : cmk(0.0, 0.0);
+            },
+            [&](int k, int c) -> cplx { return D[k + 16 * c]; }, aR, aI);
+        const int cc = lane & 15;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int rr = rr0 + (lane >> 4) + 4 * v;
+          if (rr < nr && cc < nb) X[rr + (kb + cc) * RB] = cmk(aR[v], aI[v]);
+        }
+      }
+      __syncthreads();
+      const int clo = pass == 0 ? kb + nb : 0, chi = pass == 0 ? n : kb;
+      const int cols = chi - clo;
+      if (cols > 0) {
+        const int tcc = (cols + 15) / 16;
+        for (int tile = wv; tile < trr * tcc; tile += nw) {
+          const int rr0 = (tile % trr) * 16, cc0 = clo + (tile / trr) * 16;
+          v4d aR = {0, 0, 0, 0}, aI = {0, 0, 0, 0};
+          ztile_mfma<4>(
+              [&](int r, int k) -> cplx {
+                return (rr0 + r < nr && k < nb) ? X[rr0 + r + (kb + k) * RB] : cmk(0.0, 0.0);
+              },
+              [&](int k, int c) -> cplx {
+                return (k < nb && cc0 + c < chi) ? LU[(size_t)(kb + k) + (size_t)(cc0 + c) * lda] : cmk(0.0, 0.0);
+              },
+              aR, aI);
+          const int cc = cc0 + (lane & 15);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int rr = rr0 + (lane >> 4) + 4 * v;
+            if (rr < nr && cc < chi) X[rr + cc * RB] = csub(X[rr + cc * RB], cmk(aR[v], aI[v]));
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (t < 64) {
+    for (int k = n - 1; k >= 0; --k) {
+      const int p = ipiv[k];
+      if (p != k)
+        for (int r = t; r < nr; r += 64) {
+          const cplx a = X[r + k * RB];
+          X[r + k * RB] = X[r + p * RB];
+          X[r + p * RB] = a;
+        }
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < nr * n; e += nt) {
+    const int r = e % nr, c = e / nr;
+    Bg[(size_t)(r0g + r) + (size_t)c * ldb] = X[r + c * RB];
+  }
+}
 }  // namespace
 
 static int g_lds_limit = 65536;
@@ -361,12 +724,42 @@ static int g_lds_limit = 65536;
 void orf_set_lds_limit(int bytes) { g_lds_limit = bytes; }
 int orf_lds_limit() { return g_lds_limit; }
 
+long long orf_dinv_stride(int n) { return 2LL * ((n + 15) / 16) * 256; }
+
+static void launch_dinv(hipStream_t st, int n, const cplx* A, int lda, long long sA, cplx* dinv, int batch) {
+  while (batch > 0) {
+    const int b = batch > 65535 ? 65535 : batch;
+    diag_inv_kernel<<<dim3((n + 15) / 16, b), 256, 0, st>>>(n, A, lda, sA, dinv, orf_dinv_stride(n));
+    A += (size_t)sA * b;
+    dinv += (size_t)orf_dinv_stride(n) * b;
+    batch -= b;
+  }
+}
+
 void orf_zgetrf(hipStream_t st, int n, cplx* A, int lda, long long sA, int* ipiv, int* info,
-                int batch) {
+                int batch, cplx* dinv) {
+  if (n <= 0 || batch <= 0) return;
+  orf_zgetrf_only(st, n, A, lda, sA, ipiv, info, batch);
+  if (dinv && n >= 32) launch_dinv(st, n, A, lda, sA, dinv, batch);
+}
+
+void orf_zgetrf_only(hipStream_t st, int n, cplx* A, int lda, long long sA, int* ipiv, int* info,
+                     int batch) {
   if (n <= 0 || batch <= 0) return;
   size_t bytes = (size_t)n * n * sizeof(cplx);
   int use_lds = bytes <= (size_t)g_lds_limit;
-  zgetrf_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, ipiv, info, use_lds);
+  if (use_lds || n < 32) {
+    zgetrf_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, ipiv, info, use_lds);
+    return;
+  }
+  // panel + U rows in LDS: 2 * LB * n complex
+  const size_t per = 2 * (size_t)n * sizeof(cplx);
+  if (16 * per <= (size_t)g_lds_limit)
+    zgetrf_blocked_kernel<16><<<batch, 256, 16 * per, st>>>(n, A, lda, sA, ipiv, info);
+  else if (8 * per <= (size_t)g_lds_limit)
+    zgetrf_blocked_kernel<8><<<batch, 256, 8 * per, st>>>(n, A, lda, sA, ipiv, info);
+  else
+    zgetrf_blocked_kernel<4><<<batch, 256, 4 * per, st>>>(n, A, lda, sA, ipiv, info);
 }
 
 static int rhs_block(int n) {
@@ -378,25 +771,37 @@ static int rhs_block(int n) {
 }
 
 void orf_zgetrs(hipStream_t st, char side, int n, int nrhs, const cplx* LU, int lda, long long sA,
-                const int* ipiv, cplx* B, int ldb, long long sB, int batch) {
+                const int* ipiv, const cplx* dinv, cplx* B, int ldb, long long sB, int batch) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return;
   while (batch > 65535) {  // grid y limit
-    orf_zgetrs(st, side, n, nrhs, LU, lda, sA, ipiv, B, ldb, sB, 65535);
+    orf_zgetrs(st, side, n, nrhs, LU, lda, sA, ipiv, dinv, B, ldb, sB, 65535);
     LU += (size_t)sA * 65535;
     ipiv += (size_t)n * 65535;
+    if (dinv) dinv += (size_t)orf_dinv_stride(n) * 65535;
     B += (size_t)sB * 65535;
     batch -= 65535;
   }
   int cb = rhs_block(n);
-  if (side == 'L') {
+  if (n < 32 || !dinv) {  // tiny systems: per-column substitution in LDS
     dim3 grid((nrhs + cb - 1) / cb, batch);
-    zgetrs_left_kernel<<<grid, 256, (size_t)n * cb * sizeof(cplx), st>>>(n, nrhs, cb, LU, lda, sA,
-                                                                          ipiv, B, ldb, sB);
-  } else {
-    dim3 grid((nrhs + cb - 1) / cb, batch);
-    zgetrs_right_kernel<<<grid, 256, (size_t)n * cb * sizeof(cplx), st>>>(n, nrhs, cb, LU, lda, sA,
-                                                                           ipiv, B, ldb, sB);
+    if (side == 'L')
+      zgetrs_left_kernel<<<grid, 256, (size_t)n * cb * sizeof(cplx), st>>>(n, nrhs, cb, LU, lda, sA,
+                                                                            ipiv, B, ldb, sB);
+    else
+      zgetrs_right_kernel<<<grid, 256, (size_t)n * cb * sizeof(cplx), st>>>(n, nrhs, cb, LU, lda,
+                                                                             sA, ipiv, B, ldb, sB);
+    return;
   }
+  if (cb > 32) cb = 32;  // 16-wide MFMA tiles; moderate LDS footprint keeps several blocks per CU
+  if (cb > 16 && nrhs <= 16) cb = 16;
+  dim3 grid((nrhs + cb - 1) / cb, batch);
+  const long long sD = orf_dinv_stride(n);
+  if (side == 'L')
+    ztrsm_left_kernel<<<grid, 256, (size_t)n * cb * sizeof(cplx), st>>>(n, nrhs, cb, LU, lda, sA, ipiv,
+                                                                         dinv, sD, B, ldb, sB);
+  else
+    ztrsm_right_kernel<<<grid, 256, (size_t)n * cb * sizeof(cplx), st>>>(n, nrhs, cb, LU, lda, sA, ipiv,
+                                                                          dinv, sD, B, ldb, sB);
 }
 
 // ============================================================== eigen-decomposition
@@ -647,6 +1052,249 @@ __global__ __launch_bounds__(256) void schur_qr_kernel(int n, cplx* __restrict__
 #undef HH
 }
 
+// Windowed variant of the same single-shift QR.  A sweep is cut into segments
+// of QR_SEG rotations.  Wave 0 chases the bulge through a small diagonal window
+// of H held in LDS (every entry of the window updated at once); the whole
+// workgroup then applies the segment's rotations, kept in registers, to the
+// rows above the window, the columns right of it and the Schur vectors Z.
+// Each H/Z element is read and written once per segment instead of once per
+// rotation, and the rotation chain needs no block barrier.
+constexpr int QR_SEG = 32;
+constexpr int QR_WIN = QR_SEG + 3;
+
+ORF_DEV void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(256) void schur_qr_win_kernel(int n, cplx* __restrict__ Hg, int ldh,
+                                                           long long sH, cplx* __restrict__ Zg,
+                                                           long long sZ, cplx* __restrict__ wout,
+                                                           int* __restrict__ info, int hInLds,
+                                                           int zInLds) {
+  extern __shared__ cplx lds_mat[];
+  __shared__ cplx win[QR_WIN * QR_WIN];
+  __shared__ double rc[QR_SEG];
+  __shared__ cplx rs[QR_SEG];
+  __shared__ int redk[4];
+  const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
+  const int lane = t & 63, wv = t >> 6, nw = nt >> 6;
+  Hg += (size_t)page * sH;
+  cplx* Zp = Zg + (size_t)page * sZ;
+  wout += (size_t)page * n;
+  cplx* H = Hg;
+  int ld = ldh;
+  cplx* Z = Zp;
+  cplx* ldsNext = lds_mat;
+  if (hInLds) {
+    H = ldsNext;
+    ld = n;
+    ldsNext += (size_t)n * n;
+    for (int e = t; e < n * n; e += nt) H[e] = Hg[(size_t)(e % n) + (size_t)(e / n) * ldh];
+  }
+  if (zInLds) {
+    Z = ldsNext;
+    for (int e = t; e < n * n; e += nt) Z[e] = Zp[e];
+  }
+  __syncthreads();
+#define HH(r, c) H[(size_t)(r) + (size_t)(c) * ld]
+#define WN(r, c) win[(r) + (c) * QR_WIN]
+  const double smlnum = SAFMIN * ((double)n / ULP);
+  const int itmax = 100;
+  int failed = 0;
+  int i = n - 1;
+  while (i >= 0) {
+    int l = 0;
+    int its;
+    for (its = 0; its <= itmax; ++its) {
+      int kfound = l;
+      for (int k = i - t; k > l; k -= nt) {
+        const cplx hkk1 = HH(k, k - 1);
+        bool small = false;
+        if (cabs1(hkk1) <= smlnum) small = true;
+        else {
+          double tst = cabs1(HH(k - 1, k - 1)) + cabs1(HH(k, k));
+          if (tst == 0.0) {
+            if (k - 2 >= 0) tst += cabs1(HH(k - 1, k - 2));
+            if (k + 1 <= n - 1) tst += cabs1(HH(k + 1, k));
+          }
+          if (cabs1(hkk1) <= ULP * tst) {
+            const double ab = fmax(cabs1(hkk1), cabs1(HH(k - 1, k)));
+            const double ba = fmin(cabs1(hkk1), cabs1(HH(k - 1, k)));
+            const cplx dd = csub(HH(k - 1, k - 1), HH(k, k));
+            const double aa = fmax(cabs1(HH(k, k)), cabs1(dd));
+            const double bb = fmin(cabs1(HH(k, k)), cabs1(dd));
+            const double s = aa + ab;
+            if (ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)))) small = true;
+          }
+        }
+        if (small) { kfound = k; break; }
+      }
+      int kv = kfound;
+      for (int o = 32; o > 0; o >>= 1) kv = max(kv, __shfl_xor(kv, o, 64));
+      __syncthreads();
+      if (lane == 0) redk[wv] = kv;
+      __syncthreads();
+      int kmax = redk[0];
+      for (int q = 1; q < nw; ++q) kmax = max(kmax, redk[q]);
+      l = kmax;
+      if (l > 0) {
+        __syncthreads();
+        if (t == 0) HH(l, l - 1) = cmk(0.0, 0.0);
+        __syncthreads();
+      }
+      if (l >= i) break;
+      cplx sig;
+      if (its % 20 == 10) {
+        sig = cadd(cmk(0.75 * cabs1(HH(l + 1, l)), 0.0), HH(l, l));
+      } else if (its > 0 && its % 20 == 0) {
+        sig = cadd(cmk(0.75 * cabs1(HH(i, i - 1)), 0.0), HH(i, i));
+      } else {
+        cplx tt = HH(i, i);
+        const cplx u = cmul(csqrt(HH(i - 1, i)), csqrt(HH(i, i - 1)));
+        double s = cabs1(u);
+        if (s != 0.0) {
+          const cplx x = cscale(csub(HH(i - 1, i - 1), tt), 0.5);
+          const double sx = cabs1(x);
+          s = fmax(s, cabs1(x));
+          const cplx xs = cscale(x, 1.0 / s), us = cscale(u, 1.0 / s);
+          cplx y = cscale(csqrt(cadd(cmul(xs, xs), cmul(us, us))), s);
+          if (sx > 0.0) {
+            const cplx xx = cscale(x, 1.0 / sx);
+            if (xx.x * y.x + xx.y * y.y < 0.0) y = cneg(y);
+          }
+          tt = csub(tt, cmul(u, cdiv(u, cadd(x, y))));
+        }
+        sig = tt;
+      }
+      if (!(isfinite(sig.x) && isfinite(sig.y))) {
+        its = itmax + 1;
+        break;
+      }
+      // ---- one sweep l..i in segments of QR_SEG rotations
+      for (int k0 = l; k0 < i; k0 += QR_SEG) {
+        const int k1 = min(k0 + QR_SEG, i);       // rotations k0 .. k1-1
+        const int a = (k0 == l) ? l : k0 - 1;     // window rows/cols a..e
+        const int e = min(k1 + 1, i);
+        const int W = e - a + 1;
+        __syncthreads();
+        for (int q = t; q < W * W; q += nt) {
+          const int r = q % W, c = q / W;
+          WN(r, c) = HH(a + r, a + c);
+        }
+        __syncthreads();
+        if (wv == 0) {
+          for (int k = k0; k < k1; ++k) {
+            const int kk = k - a;
+            cplx x, y;
+            if (k == l) { x = csub(WN(kk, kk), sig); y = WN(kk + 1, kk); }
+            else { x = WN(kk, kk - 1); y = WN(kk + 1, kk - 1); }
+            double c;
+            cplx s, r;
+            givens(x, y, c, s, r);
+            const cplx sc = cconj(s);
+            if (lane == 0) { rc[k - k0] = c; rs[k - k0] = s; }
+            // rows kk, kk+1 for window columns kk..W-1
+            const int j = kk + lane;
+            cplx h1, h2;
+            if (j < W) { h1 = WN(kk, j); h2 = WN(kk + 1, j); }
+            wave_sync_lds();
+            if (j < W) {
+              WN(kk, j) = cadd(cscale(h1, c), cmul(s, h2));
+              WN(kk + 1, j) = csub(cscale(h2, c), cmul(sc, h1));
+            }
+            if (k > l && lane == 0) {
+              WN(kk, kk - 1) = r;
+              WN(kk + 1, kk - 1) = cmk(0.0, 0.0);
+            }
+            wave_sync_lds();
+            // columns kk, kk+1 for window rows 0..min(kk+2, i-a)
+            const int rmax = min(kk + 2, i - a);
+            const int q = lane;
+            if (q <= rmax) { h1 = WN(q, kk); h2 = WN(q, kk + 1); }
+            wave_sync_lds();
+            if (q <= rmax) {
+              WN(q, kk) = cadd(cscale(h1, c), cmul(sc, h2));
+              WN(q, kk + 1) = csub(cscale(h2, c), cmul(s, h1));
+            }
+            wave_sync_lds();
+          }
+        }
+        __syncthreads();
+        // write the window back
+        for (int q = t; q < W * W; q += nt) {
+          const int r = q % W, c = q / W;
+          HH(a + r, a + c) = WN(r, c);
+        }
+        // deferred rotations: columns right of the window (row ops on rows k0..k1),
+        // rows above the window and all rows of Z (column ops on columns k0..k1)
+        const int nk = k1 - k0;                   // rotations in this segment
+        const int ncR = n - 1 - e;                // columns e+1..n-1
+        const int nrA = a;                        // rows 0..a-1
+        const int tasks = ncR + nrA + n;
+        for (int task = t; task < tasks; task += nt) {
+          cplx v[QR_SEG + 1];
+          if (task < ncR) {
+            cplx* col = H + (size_t)(e + 1 + task) * ld + k0;
+#pragma unroll
+            for (int q = 0; q <= QR_SEG; ++q) v[q] = (q <= nk) ? col[q] : cmk(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < QR_SEG; ++q) {
+              if (q < nk) {
+                const double c = rc[q];
+                const cplx s = rs[q];
+                const cplx h1 = v[q], h2 = v[q + 1];
+                v[q] = cadd(cscale(h1, c), cmul(s, h2));
+                v[q + 1] = csub(cscale(h2, c), cmul(cconj(s), h1));
+              }
+            }
+#pragma unroll
+            for (int q = 0; q <= QR_SEG; ++q)
+              if (q <= nk) col[q] = v[q];
+          } else {
+            cplx* base;
+            size_t ldx;
+            int row;
+            if (task < ncR + nrA) { row = task - ncR; base = H; ldx = ld; }
+            else { row = task - ncR - nrA; base = Z; ldx = n; }
+            cplx* p0 = base + row + (size_t)k0 * ldx;
+#pragma unroll
+            for (int q = 0; q <= QR_SEG; ++q) v[q] = (q <= nk) ? p0[(size_t)q * ldx] : cmk(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < QR_SEG; ++q) {
+              if (q < nk) {
+                const double c = rc[q];
+                const cplx s = rs[q];
+                const cplx h1 = v[q], h2 = v[q + 1];
+                v[q] = cadd(cscale(h1, c), cmul(cconj(s), h2));
+                v[q + 1] = csub(cscale(h2, c), cmul(s, h1));
+              }
+            }
+#pragma unroll
+            for (int q = 0; q <= QR_SEG; ++q)
+              if (q <= nk) p0[(size_t)q * ldx] = v[q];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (its > itmax) {
+      if (!failed) failed = i + 1;
+      l = i;
+    }
+    if (t == 0) wout[i] = HH(i, i);
+    i = l - 1;
+    __syncthreads();
+  }
+  if (t == 0 && info) info[page] = failed;
+  if (hInLds)
+    for (int e = t; e < n * n; e += nt) Hg[(size_t)(e % n) + (size_t)(e / n) * ldh] = H[e];
+  if (zInLds)
+    for (int e = t; e < n * n; e += nt) Zp[e] = Z[e];
+#undef WN
+#undef HH
+}
+
 // Eigenvectors of upper-triangular T (ztrevc, side R, no back-transform):
 // Y[:, k] = x with (T - T[k,k]) x = 0, x[k] = 1, x[j>k] = 0, scaled to max |.|_1 = 1.
 // One wave per eigenvector; x distributed over lanes, 64 rows per slot.
@@ -734,8 +1382,14 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
   hessenberg_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, Zwork, sZ, use_lds);
   te("eig_hessenberg");
   tb("eig_schur_qr");
-  schur_qr_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, Zwork, sZ, w, info,
-                                                           use_lds);
+  {
+    // LDS residency: H and Z both, H only, or neither (static LDS ~21 KB)
+    const size_t stat = 22 * 1024;
+    const size_t lim = (size_t)g_lds_limit + 20 * 1024 - stat;
+    const int hIn = bytes <= lim, zIn = 2 * bytes <= lim;
+    schur_qr_win_kernel<<<batch, 256, (hIn + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w, info,
+                                                                  hIn, zIn);
+  }
   te("eig_schur_qr");
   dim3 g2((n + 3) / 4, batch);
   tb("eig_vectors");

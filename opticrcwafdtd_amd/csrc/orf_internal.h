@@ -14,11 +14,16 @@ void orf_zgemm(hipStream_t st, char opA, char opB, int M, int N, int K, cplx alp
                int lda, long long sA, const cplx* B, int ldb, long long sB, cplx beta, cplx* C,
                int ldc, long long sC, int batch);
 // LU with partial pivoting, in place; ipiv [n x batch] 0-based; info [batch] (0 ok, k+1 zero pivot).
+// dinv (optional, orf_dinv_stride(n) complex per page) receives the inverses of the 16x16
+// diagonal blocks of L and U, which orf_zgetrs uses for its MFMA block solves.
 void orf_zgetrf(hipStream_t st, int n, cplx* A, int lda, long long sA, int* ipiv, int* info,
-                int batch);
+                int batch, cplx* dinv);
+void orf_zgetrf_only(hipStream_t st, int n, cplx* A, int lda, long long sA, int* ipiv, int* info,
+                     int batch);
+long long orf_dinv_stride(int n);
 // side 'L': A X = B (B n x nrhs); side 'R': X A = B (B nrhs x n); B overwritten with X.
 void orf_zgetrs(hipStream_t st, char side, int n, int nrhs, const cplx* LU, int lda, long long sA,
-                const int* ipiv, cplx* B, int ldb, long long sB, int batch);
+                const int* ipiv, const cplx* dinv, cplx* B, int ldb, long long sB, int batch);
 // Eigen-decomposition; A (overwritten), w [n x batch], V (unit-norm eigenvectors),
 // Zwork/Ywork n*n per page; info [batch].
 void orf_zgeev(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V, int ldv,

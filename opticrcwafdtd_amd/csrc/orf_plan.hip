@@ -321,6 +321,7 @@ struct Work {
   cplx *X1, *X2, *XB, *R3, *Dm, *RHS2, *LUb;
   cplx *Sa[4], *Sn[4], *Sb[4], *Tm, *M1, *Fm;
   double* kL;
+  cplx* dinv;  // diagonal-block inverses of the most recent LU
   int *ipiv, *info;
   DevSolve* solves;
   int* groupOf;
@@ -349,6 +350,7 @@ static size_t per_solve_bytes(const orf_plan& P) {
   c += n2 * (2 + 1 + 3 + 1 + 2 + 1);  // X1 X2 XB R3 Dm RHS2 LUb
   c += n2 * (12 + 3);        // Sa Sn Sb Tm M1 Fm
   c += 3 * (size_t)P.nr + 1; // mu lam Xd eps
+  c += (size_t)orf_dinv_stride(P.nr); // diagonal-block inverses
   return c * sizeof(cplx) + sizeof(double) + sizeof(int) * (P.nr + 2) + sizeof(DevSolve);
 }
 
@@ -405,6 +407,7 @@ static Work make_work(orf_plan& P, int Cp) {
   w.M1 = (cplx*)get(cs * n2 * Cs);
   w.Fm = (cplx*)get(cs * n2 * Cs);
   w.kL = (double*)get(sizeof(double) * Cs);
+  w.dinv = (cplx*)get(cs * std::max((size_t)orf_dinv_stride(P.nr) * Cs, (size_t)orf_dinv_stride(P.M) * Cp));
   w.ipiv = (int*)get(sizeof(int) * (size_t)std::max(P.nr, P.M) * std::max(Cs, Cp));
   w.info = (int*)get(sizeof(int) * (size_t)std::max(Cs, Cp) * 64);
   w.solves = (DevSolve*)get(sizeof(DevSolve) * Cs);
@@ -432,21 +435,21 @@ static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx
   hipStream_t st = P.st;
   {
     STAGE("smat_lu");
-    orf_zgetrf(st, n, Wm, n, n2, w.ipiv, nullptr, Cs);
+    orf_zgetrf(st, n, Wm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
     KCHECK();
   }
   eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.X1, n2);
   {
     STAGE("smat_solve");
-    orf_zgetrs(st, 'L', n, n, Wm, n, n2, w.ipiv, w.X1, n, n2, Cs);  // W \ W0
+    orf_zgetrs(st, 'L', n, n, Wm, n, n2, w.ipiv, w.dinv, w.X1, n, n2, Cs);  // W \ W0
   }
   {
     STAGE("smat_lu");
-    orf_zgetrf(st, n, Vm, n, n2, w.ipiv, nullptr, Cs);
+    orf_zgetrf(st, n, Vm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   }
   {
     STAGE("smat_solve");
-    orf_zgetrs(st, 'L', n, n, Vm, n, n2, w.ipiv, w.X2, n, n2, Cs);  // V \ V0
+    orf_zgetrs(st, 'L', n, n, Vm, n, n2, w.ipiv, w.dinv, w.X2, n, n2, Cs);  // V \ V0
   }
   sum_diff_kernel<<<g1(n2 * Cs), 256, 0, st>>>(n2 * Cs, w.X1, w.X2);  // X1 = A, X2 = B
   smat_rhs3_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, lam, w.kL, w.X1, w.X2, n2, w.R3, 3 * n2, w.Xd);
@@ -455,11 +458,11 @@ static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx
   HIPCHECK(hipMemcpyAsync(w.LUb, w.X1, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
   {
     STAGE("smat_lu");
-    orf_zgetrf(st, n, w.LUb, n, n2, w.ipiv, nullptr, Cs);
+    orf_zgetrf(st, n, w.LUb, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   }
   {
     STAGE("smat_solve");
-    orf_zgetrs(st, 'L', n, 3 * n, w.LUb, n, n2, w.ipiv, w.R3, n, 3 * n2, Cs);  // [AiXB AiXA AiB]
+    orf_zgetrs(st, 'L', n, 3 * n, w.LUb, n, n2, w.ipiv, w.dinv, w.R3, n, 3 * n2, Cs);  // [AiXB AiXA AiB]
   }
   cplx* AiXB = w.R3;
   cplx* AiXA = w.R3 + n2;
@@ -478,11 +481,11 @@ static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx
   rowscale_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, n, w.Xd, w.RHS2 + n2, 2 * n2);
   {
     STAGE("smat_lu");
-    orf_zgetrf(st, n, w.Dm, n, n2, w.ipiv, nullptr, Cs);
+    orf_zgetrf(st, n, w.Dm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   }
   {
     STAGE("smat_solve");
-    orf_zgetrs(st, 'L', n, 2 * n, w.Dm, n, n2, w.ipiv, w.RHS2, n, 2 * n2, Cs);
+    orf_zgetrs(st, 'L', n, 2 * n, w.Dm, n, n2, w.ipiv, w.dinv, w.RHS2, n, 2 * n2, Cs);
   }
   // split [S11 | S12] per page into separate page arrays
   for (int k = 0; k < 2; ++k) {
@@ -501,15 +504,15 @@ static void smatrix_star(orf_plan& P, Work& w, int Cs, cplx* const SA[4], cplx* 
   // D = SA12 / (I - SB11 SA22)
   eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.M1, n2);
   orf_zgemm(st, 'N', 'N', n, n, n, MONE, SB[0], n, n2, SA[3], n, n2, ONE, w.M1, n, n2, Cs);
-  orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs);
+  orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   HIPCHECK(hipMemcpyAsync(w.Dm, SA[1], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-  orf_zgetrs(st, 'R', n, n, w.M1, n, n2, w.ipiv, w.Dm, n, n2, Cs);
+  orf_zgetrs(st, 'R', n, n, w.M1, n, n2, w.ipiv, w.dinv, w.Dm, n, n2, Cs);
   // F = SB21 / (I - SA22 SB11)
   eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.M1, n2);
   orf_zgemm(st, 'N', 'N', n, n, n, MONE, SA[3], n, n2, SB[0], n, n2, ONE, w.M1, n, n2, Cs);
-  orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs);
+  orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   HIPCHECK(hipMemcpyAsync(w.Fm, SB[2], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-  orf_zgetrs(st, 'R', n, n, w.M1, n, n2, w.ipiv, w.Fm, n, n2, Cs);
+  orf_zgetrs(st, 'R', n, n, w.M1, n, n2, w.ipiv, w.dinv, w.Fm, n, n2, Cs);
   // S11 = SA11 + D SB11 SA21
   orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Dm, n, n2, SB[0], n, n2, ZERO, w.Tm, n, n2, Cs);
   HIPCHECK(hipMemcpyAsync(out[0], SA[0], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
@@ -562,10 +565,12 @@ static void build_regions(orf_plan& P, const std::vector<zc>& epsInc, const std:
   for (int k = 0; k < 2; ++k)
     HIPCHECK(hipMemcpyAsync(dLU.as<cplx>() + k * n2 * G, dV.p, sizeof(cplx) * n2 * G, hipMemcpyDeviceToDevice, st));
   dip.alloc(sizeof(int) * n * 2 * G);
-  orf_zgetrf(st, n, dLU.as<cplx>(), n, n2, dip.as<int>(), nullptr, 2 * G);
+  DBuf dDinv;
+  dDinv.alloc(sizeof(cplx) * orf_dinv_stride(n) * 2 * G);
+  orf_zgetrf(st, n, dLU.as<cplx>(), n, n2, dip.as<int>(), nullptr, 2 * G, dDinv.as<cplx>());
   dX.alloc(sizeof(cplx) * n2 * 2 * G);
   HIPCHECK(hipMemcpyAsync(dX.p, dV.as<cplx>() + n2 * G, sizeof(cplx) * n2 * 2 * G, hipMemcpyDeviceToDevice, st));
-  orf_zgetrs(st, 'L', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), dX.as<cplx>(), n, n2, 2 * G);
+  orf_zgetrs(st, 'L', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), dDinv.as<cplx>(), dX.as<cplx>(), n, n2, 2 * G);
   // A = I + X, B = I - X
   dA.alloc(sizeof(cplx) * n2 * 2 * G);
   dB.alloc(sizeof(cplx) * n2 * 2 * G);
@@ -573,17 +578,17 @@ static void build_regions(orf_plan& P, const std::vector<zc>& epsInc, const std:
   eye_axpby_kernel<<<g1(n2, 2 * G), 256, 0, st>>>(n, ONE, MONE, dX.as<cplx>(), n2, dB.as<cplx>(), n2);
   // LU(A); AiB = A \ B; Ai = A \ I; T = 0.5 (A - B AiB); BA = B / A
   HIPCHECK(hipMemcpyAsync(dLU.p, dA.p, sizeof(cplx) * n2 * 2 * G, hipMemcpyDeviceToDevice, st));
-  orf_zgetrf(st, n, dLU.as<cplx>(), n, n2, dip.as<int>(), nullptr, 2 * G);
+  orf_zgetrf(st, n, dLU.as<cplx>(), n, n2, dip.as<int>(), nullptr, 2 * G, dDinv.as<cplx>());
   dAi.alloc(sizeof(cplx) * n2 * 2 * G * 3);
   cplx* AiB = dAi.as<cplx>();
   cplx* Ai = AiB + n2 * 2 * G;
   cplx* BA = Ai + n2 * 2 * G;
   HIPCHECK(hipMemcpyAsync(AiB, dB.p, sizeof(cplx) * n2 * 2 * G, hipMemcpyDeviceToDevice, st));
-  orf_zgetrs(st, 'L', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), AiB, n, n2, 2 * G);
+  orf_zgetrs(st, 'L', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), dDinv.as<cplx>(), AiB, n, n2, 2 * G);
   eye_axpby_kernel<<<g1(n2, 2 * G), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, Ai, n2);
-  orf_zgetrs(st, 'L', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), Ai, n, n2, 2 * G);
+  orf_zgetrs(st, 'L', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), dDinv.as<cplx>(), Ai, n, n2, 2 * G);
   HIPCHECK(hipMemcpyAsync(BA, dB.p, sizeof(cplx) * n2 * 2 * G, hipMemcpyDeviceToDevice, st));
-  orf_zgetrs(st, 'R', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), BA, n, n2, 2 * G);
+  orf_zgetrs(st, 'R', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), dDinv.as<cplx>(), BA, n, n2, 2 * G);
   dT.alloc(sizeof(cplx) * n2 * 2 * G);
   eye_axpby_kernel<<<g1(n2, 2 * G), 256, 0, st>>>(n, ZERO, {0.5, 0.0}, dA.as<cplx>(), n2, dT.as<cplx>(), n2);
   orf_zgemm(st, 'N', 'N', n, n, n, {-0.5, 0.0}, dB.as<cplx>(), n, n2, AiB, n, n2, ONE, dT.as<cplx>(), n, n2, 2 * G);
@@ -980,8 +985,8 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
             const int nb = Cp * P.Nsy;
             cplx* inv = w.Inv;
             eye_stack_kernel<<<g1((long long)Wx * Wx * nb), 256, 0, st>>>(Wx, (long long)nb, inv);
-            orf_zgetrf(st, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, nb);
-            orf_zgetrs(st, 'L', Wx, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, inv, Wx, (long long)Wx * Wx, nb);
+            orf_zgetrf(st, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, nb, nullptr);
+            orf_zgetrs(st, 'L', Wx, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, inv, Wx, (long long)Wx * Wx, nb);
             // G = Fy * [A_j]^T : (Hy x Nsy) * (Nsy x Wx^2)
             orf_zgemm(st, 'N', 'T', Hy, Wx * Wx, P.Nsy, ONE, P.dFy.as<cplx>(), Hy, 0, inv, Wx * Wx,
                       (long long)Wx * Wx * P.Nsy, ZERO, w.GH, Hy, (long long)Hy * Wx * Wx, Cp);
@@ -991,8 +996,8 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
                                                               (long long)P.nslotMax * Hy * P.Nsx, si, P.nslotMax, w.Toe);
             const int nb2 = Cp * P.Nsx;
             eye_stack_kernel<<<g1((long long)Wy * Wy * nb2), 256, 0, st>>>(Wy, (long long)nb2, inv);
-            orf_zgetrf(st, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, nb2);
-            orf_zgetrs(st, 'L', Wy, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, inv, Wy, (long long)Wy * Wy, nb2);
+            orf_zgetrf(st, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, nb2, nullptr);
+            orf_zgetrs(st, 'L', Wy, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, inv, Wy, (long long)Wy * Wy, nb2);
             orf_zgemm(st, 'N', 'T', Hx, Wy * Wy, P.Nsx, ONE, P.dFx.as<cplx>(), Hx, 0, inv, Wy * Wy,
                       (long long)Wy * Wy * P.Nsx, ZERO, w.HH, Hx, (long long)Hx * Wy * Wy, Cp);
             li_gather_kernel<<<g1(M2, Cp), 256, 0, st>>>(P.Nx, P.Ny, M, w.GH, w.HH, w.Exx, w.Eyy, M2);
@@ -1005,8 +1010,8 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         {
           STAGE("omega");
           omega_rhs_kernel<<<g1(2 * M2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.Exx, w.Eyy, M2, w.G, 2 * M2);
-          orf_zgetrf(st, M, w.E, M, M2, w.ipiv, nullptr, Cp);
-          orf_zgetrs(st, 'L', M, 2 * M, w.E, M, M2, w.ipiv, w.G, M, 2 * M2, Cp);
+          orf_zgetrf(st, M, w.E, M, M2, w.ipiv, nullptr, Cp, w.dinv);
+          orf_zgetrs(st, 'L', M, 2 * M, w.E, M, M2, w.ipiv, w.dinv, w.G, M, 2 * M2, Cp);
           omega_assemble_kernel<<<g1(nf2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.G, 2 * M2, w.Exx, w.Eyy,
                                                              M2, w.Omega, nf2);
           KCHECK();
@@ -1277,15 +1282,16 @@ int orf_pagemldivide(orf_ctx* ctx, int32_t n, int32_t nrhs, const double* A, int
     if (!ctx || n <= 0 || nrhs <= 0 || batch <= 0) fail(ORF_INVALID_ARG, "bad sizes");
     HIPCHECK(hipSetDevice(ctx->device));
     const size_t n2 = (size_t)n * n;
-    DBuf a, b, ip;
+    DBuf a, b, ip, dv;
+    dv.alloc(16 * (size_t)orf_dinv_stride(n) * batch);
     a.alloc((size_t)batch * n2 * 16);
     b.alloc((size_t)batch * n * nrhs * 16);
     ip.alloc(sizeof(int) * (size_t)n * batch);
     for (int q = 0; q < batch; ++q)
       HIPCHECK(hipMemcpy(a.as<cplx>() + q * n2, A + (strideA ? 2 * n2 * q : 0), n2 * 16, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(b.p, B, (size_t)batch * n * nrhs * 16, hipMemcpyHostToDevice));
-    orf_zgetrf(ctx->stream, n, a.as<cplx>(), n, n2, ip.as<int>(), nullptr, batch);
-    orf_zgetrs(ctx->stream, 'L', n, nrhs, a.as<cplx>(), n, n2, ip.as<int>(), b.as<cplx>(), n, (long long)n * nrhs, batch);
+    orf_zgetrf(ctx->stream, n, a.as<cplx>(), n, n2, ip.as<int>(), nullptr, batch, dv.as<cplx>());
+    orf_zgetrs(ctx->stream, 'L', n, nrhs, a.as<cplx>(), n, n2, ip.as<int>(), dv.as<cplx>(), b.as<cplx>(), n, (long long)n * nrhs, batch);
     KCHECK();
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     HIPCHECK(hipMemcpy(X, b.p, (size_t)batch * n * nrhs * 16, hipMemcpyDeviceToHost));
@@ -1298,15 +1304,16 @@ int orf_pagemrdivide(orf_ctx* ctx, int32_t n, int32_t nrows, const double* B, co
     if (!ctx || n <= 0 || nrows <= 0 || batch <= 0) fail(ORF_INVALID_ARG, "bad sizes");
     HIPCHECK(hipSetDevice(ctx->device));
     const size_t n2 = (size_t)n * n;
-    DBuf a, b, ip;
+    DBuf a, b, ip, dv;
+    dv.alloc(16 * (size_t)orf_dinv_stride(n) * batch);
     a.alloc((size_t)batch * n2 * 16);
     b.alloc((size_t)batch * n * nrows * 16);
     ip.alloc(sizeof(int) * (size_t)n * batch);
     for (int q = 0; q < batch; ++q)
       HIPCHECK(hipMemcpy(a.as<cplx>() + q * n2, A + (strideA ? 2 * n2 * q : 0), n2 * 16, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(b.p, B, (size_t)batch * n * nrows * 16, hipMemcpyHostToDevice));
-    orf_zgetrf(ctx->stream, n, a.as<cplx>(), n, n2, ip.as<int>(), nullptr, batch);
-    orf_zgetrs(ctx->stream, 'R', n, nrows, a.as<cplx>(), n, n2, ip.as<int>(), b.as<cplx>(), nrows,
+    orf_zgetrf(ctx->stream, n, a.as<cplx>(), n, n2, ip.as<int>(), nullptr, batch, dv.as<cplx>());
+    orf_zgetrs(ctx->stream, 'R', n, nrows, a.as<cplx>(), n, n2, ip.as<int>(), dv.as<cplx>(), b.as<cplx>(), nrows,
                (long long)n * nrows, batch);
     KCHECK();
     HIPCHECK(hipStreamSynchronize(ctx->stream));
