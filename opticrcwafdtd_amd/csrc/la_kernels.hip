@@ -720,6 +720,13 @@ __global__ __launch_bounds__(256) void ztrsm_right_kernel(int n, int nrows, int 
 }  // namespace
 
 static int g_lds_limit = 65536;
+static int g_qr_bulges = 4;      // bulges per multishift pass (4 or 8); 1 = per-rotation kernel
+static int g_qr_small_n = 96;    // up to this size the per-rotation LDS kernel is used
+
+void orf_set_qr_params(int bulges, int small_n) {
+  if (bulges > 0) g_qr_bulges = bulges;
+  if (small_n >= 0) g_qr_small_n = small_n;
+}
 
 void orf_set_lds_limit(int bytes) { g_lds_limit = bytes; }
 int orf_lds_limit() { return g_lds_limit; }
@@ -808,6 +815,7 @@ void orf_zgetrs(hipStream_t st, char side, int n, int nrhs, const cplx* LU, int 
 namespace {
 constexpr double ULP = 2.220446049250313e-16;   // LAPACK dlamch('P') = eps * base
 constexpr double SAFMIN = 2.2250738585072014e-308;
+constexpr int TREVC_SLOTS_H = 12;  // columns up to 768 rows in registers
 
 // Householder reduction to upper Hessenberg form, Z = H(0) H(1) ... accumulated.
 // A (n x n, ld) is reduced in place; entries below the subdiagonal are zeroed.
@@ -888,15 +896,165 @@ __global__ __launch_bounds__(256) void hessenberg_kernel(int n, cplx* __restrict
   }
 }
 
+// Householder reduction to Hessenberg form without forming Q: the reflectors
+// H(k) = I - tau_k v_k v_k^H (v_k[k+1] = 1) go to V (column k, rows k+1..n-1)
+// and tau; Q = H(0) H(1) ... is applied to the eigenvectors afterwards
+// (hess_q_apply_kernel).  The left update keeps each column in registers
+// (one read and one write per column and step).
+__global__ __launch_bounds__(256) void hessenberg_noq_kernel(int n, cplx* __restrict__ Ag, int lda,
+                                                             long long sA, cplx* __restrict__ Vg,
+                                                             cplx* __restrict__ taug, int use_lds) {
+  extern __shared__ cplx lds_mat[];
+  __shared__ cplx v[1024];
+  __shared__ double red[4];
+  const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
+  const int lane = t & 63, w = t >> 6;
+  Ag += (size_t)page * sA;
+  cplx* V = Vg + (size_t)page * n * n;
+  cplx* tau = taug + (size_t)page * n;
+  cplx* A = Ag;
+  int ld = lda;
+  if (use_lds) {
+    A = lds_mat;
+    ld = n;
+    for (int e = t; e < n * n; e += nt) A[e] = Ag[(size_t)(e % n) + (size_t)(e / n) * lda];
+  }
+  __syncthreads();
+  for (int k = 0; k < n - 2; ++k) {
+    const int m = n - k - 1;
+    cplx* colk = A + (size_t)k * ld;
+    double s2 = 0.0;
+    for (int i = k + 2 + t; i < n; i += nt) s2 += cnorm2(colk[i]);
+    s2 = block_sum(s2, red);
+    const cplx alpha = colk[k + 1];
+    if (s2 == 0.0 && alpha.y == 0.0) {
+      if (t == 0) tau[k] = cmk(0.0, 0.0);
+      __syncthreads();
+      continue;
+    }
+    const double beta = -copysign(sqrt(cnorm2(alpha) + s2), alpha.x);
+    const cplx tk = cmk((beta - alpha.x) / beta, -alpha.y / beta);
+    const cplx scal = crecip(csub(alpha, cmk(beta, 0.0)));
+    for (int i = t; i < m; i += nt) {
+      const cplx vi = (i == 0) ? cmk(1.0, 0.0) : cmul(colk[k + 1 + i], scal);
+      v[i] = vi;
+      V[(size_t)k * n + k + 1 + i] = vi;
+    }
+    if (t == 0) tau[k] = tk;
+    __syncthreads();
+    for (int i = k + 1 + t; i < n; i += nt) colk[i] = (i == k + 1) ? cmk(beta, 0.0) : cmk(0.0, 0.0);
+    // right: rows 0..n-1, columns k+1..n-1:  A -= tau (A v) v^H
+    for (int r = t; r < n; r += nt) {
+      cplx wa = cmk(0.0, 0.0);
+      for (int j = 0; j < m; ++j) wa = cadd(wa, cmul(A[r + (size_t)(k + 1 + j) * ld], v[j]));
+      wa = cmul(tk, wa);
+      for (int j = 0; j < m; ++j) {
+        cplx* pa = A + r + (size_t)(k + 1 + j) * ld;
+        *pa = csub(*pa, cmul(wa, cconj(v[j])));
+      }
+    }
+    __syncthreads();
+    // left: rows k+1..n-1, columns k+1..n-1: A -= conj(tau) v (v^H A); a wave per column
+    const cplx taub = cconj(tk);
+    for (int j = k + 1 + w; j < n; j += nt >> 6) {
+      cplx* col = A + (size_t)j * ld + k + 1;
+      cplx x[TREVC_SLOTS_H];
+      double zr = 0.0, zi = 0.0;
+#pragma unroll
+      for (int q = 0; q < TREVC_SLOTS_H; ++q) {
+        const int i = lane + 64 * q;
+        x[q] = (i < m) ? col[i] : cmk(0.0, 0.0);
+        if (i < m) {
+          const cplx p = cmulc(v[i], x[q]);
+          zr += p.x;
+          zi += p.y;
+        }
+      }
+      zr = wave_sum(zr);
+      zi = wave_sum(zi);
+      const cplx f = cmul(taub, cmk(zr, zi));
+#pragma unroll
+      for (int q = 0; q < TREVC_SLOTS_H; ++q) {
+        const int i = lane + 64 * q;
+        if (i < m) col[i] = csub(x[q], cmul(v[i], f));
+      }
+    }
+    __syncthreads();
+  }
+  if (use_lds) {
+    for (int e = t; e < n * n; e += nt) Ag[(size_t)(e % n) + (size_t)(e / n) * lda] = A[e];
+  }
+}
+
+// W = Q W with Q = H(0) H(1) ... H(n-3) from hessenberg_noq_kernel.  Columns
+// of W are independent: 16 columns per workgroup in LDS, 16 lanes per column,
+// reflectors applied last to first with no block barrier.
+__global__ __launch_bounds__(256) void hess_q_apply_kernel(int n, const cplx* __restrict__ Vg,
+                                                           const cplx* __restrict__ taug,
+                                                           cplx* __restrict__ Wg, int ldw,
+                                                           long long sW) {
+  extern __shared__ cplx Wc[];  // n x 16
+  const int page = blockIdx.y, c0 = blockIdx.x * 16, t = threadIdx.x;
+  const cplx* V = Vg + (size_t)page * n * n;
+  const cplx* tau = taug + (size_t)page * n;
+  cplx* W = Wg + (size_t)page * sW;
+  const int nc = min(16, n - c0);
+  for (int e = t; e < n * nc; e += blockDim.x) {
+    const int r = e % n, c = e / n;
+    Wc[r + c * n] = W[(size_t)r + (size_t)(c0 + c) * ldw];
+  }
+  __syncthreads();
+  const int cl = t >> 4, sub = t & 15;
+  if (cl < nc) {
+    cplx* col = Wc + cl * n;
+    for (int k = n - 3; k >= 0; --k) {
+      const cplx tk = tau[k];
+      if (tk.x == 0.0 && tk.y == 0.0) continue;
+      const int m = n - k - 1;
+      const cplx* vk = V + (size_t)k * n + k + 1;
+      double sr = 0.0, si = 0.0;
+      for (int i = sub; i < m; i += 16) {
+        const cplx p = cmulc(vk[i], col[k + 1 + i]);
+        sr += p.x;
+        si += p.y;
+      }
+      for (int o = 8; o > 0; o >>= 1) {
+        sr += __shfl_xor(sr, o, 16);
+        si += __shfl_xor(si, o, 16);
+      }
+      const cplx f = cmul(tk, cmk(sr, si));
+      for (int i = sub; i < m; i += 16) col[k + 1 + i] = csub(col[k + 1 + i], cmul(vk[i], f));
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < n * nc; e += blockDim.x) {
+    const int r = e % n, c = e / n;
+    W[(size_t)r + (size_t)(c0 + c) * ldw] = Wc[r + c * n];
+  }
+}
+
+// Complex Givens rotation G = [c s; -conj(s) c] with G [a; b] = [r; 0], c real.
+// Inputs are scaled by max(|a|_1, |b|_1) so the squared norms neither
+// overflow nor underflow; two square roots and three reciprocals.
 ORF_DEV void givens(cplx a, cplx b, double& c, cplx& s, cplx& r) {
-  const double aa = cabs(a), bb = cabs(b);
-  if (bb == 0.0) { c = 1.0; s = cmk(0.0, 0.0); r = a; return; }
-  if (aa == 0.0) { c = 0.0; s = cconj(cscale(b, 1.0 / bb)); r = cmk(bb, 0.0); return; }
-  const double rho = hypot(aa, bb);
-  c = aa / rho;
-  const cplx ph = cscale(a, 1.0 / aa);
-  s = cscale(cmul(ph, cconj(b)), 1.0 / rho);
-  r = cscale(ph, rho);
+  const double sc = fmax(cabs1(a), cabs1(b));
+  if (cabs1(b) == 0.0 || sc == 0.0) { c = 1.0; s = cmk(0.0, 0.0); r = a; return; }
+  const double isc = 1.0 / sc;
+  const cplx as = cscale(a, isc), bs = cscale(b, isc);
+  const double na2 = cnorm2(as), nb2 = cnorm2(bs);
+  const double rho = sqrt(na2 + nb2);
+  const double irho = 1.0 / rho;
+  if (na2 == 0.0) {
+    c = 0.0;
+    s = cscale(cconj(bs), irho);
+    r = cmk(rho * sc, 0.0);
+    return;
+  }
+  const double aa = sqrt(na2);
+  const cplx ph = cscale(as, 1.0 / aa);
+  c = aa * irho;
+  s = cscale(cmul(ph, cconj(bs)), irho);
+  r = cscale(ph, rho * sc);
 }
 
 // Complex Schur form by single-shift QR on a Hessenberg matrix (in place);
@@ -905,13 +1063,15 @@ ORF_DEV void givens(cplx a, cplx b, double& c, cplx& s, cplx& r) {
 __global__ __launch_bounds__(256) void schur_qr_kernel(int n, cplx* __restrict__ Hg, int ldh,
                                                        long long sH, cplx* __restrict__ Zg,
                                                        long long sZ, cplx* __restrict__ wout,
-                                                       int* __restrict__ info, int use_lds) {
+                                                       int* __restrict__ info, int use_lds,
+                                                       int zInLds) {
   extern __shared__ cplx lds_mat[];
   __shared__ int redk[4];
   const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
   const int lane = t & 63, wv = t >> 6, nw = nt >> 6;
   Hg += (size_t)page * sH;
-  cplx* Z = Zg + (size_t)page * sZ;
+  cplx* Zp = Zg + (size_t)page * sZ;
+  cplx* Z = Zp;
   wout += (size_t)page * n;
   cplx* H = Hg;
   int ld = ldh;
@@ -919,8 +1079,12 @@ __global__ __launch_bounds__(256) void schur_qr_kernel(int n, cplx* __restrict__
     H = lds_mat;
     ld = n;
     for (int e = t; e < n * n; e += nt) H[e] = Hg[(size_t)(e % n) + (size_t)(e / n) * ldh];
-    __syncthreads();
   }
+  if (zInLds) {
+    Z = lds_mat + (size_t)n * n;
+    for (int e = t; e < n * n; e += nt) Z[e] = Zp[e];
+  }
+  __syncthreads();
 #define HH(r, c) H[(size_t)(r) + (size_t)(c) * ld]
   const double smlnum = SAFMIN * ((double)n / ULP);
   const int itmax = 100;  // sweeps per eigenvalue before giving up (reported via info)
@@ -1049,6 +1213,8 @@ __global__ __launch_bounds__(256) void schur_qr_kernel(int n, cplx* __restrict__
   if (use_lds) {
     for (int e = t; e < n * n; e += nt) Hg[(size_t)(e % n) + (size_t)(e / n) * ldh] = H[e];
   }
+  if (zInLds)
+    for (int e = t; e < n * n; e += nt) Zp[e] = Z[e];
 #undef HH
 }
 
@@ -1295,6 +1461,356 @@ __global__ __launch_bounds__(256) void schur_qr_win_kernel(int n, cplx* __restri
 #undef HH
 }
 
+// Multi-shift variant: each pass chases up to MB_NB single-shift bulges at
+// once, with shifts taken as the eigenvalues of the trailing MB_NB x MB_NB
+// block (the choice of LAPACK's small-bulge multishift QR).  Bulge b trails
+// bulge b-1 by three positions; within a segment the bulges are processed one
+// after another (a reordering of commuting rotations), which gives every
+// rotation a fixed slot in a register strip of MB_SW columns.  The deferred
+// updates of the rows above the window, the columns right of it and Z then
+// serve all shifts of the pass with one load and one store per segment.
+constexpr int MB_SEG = 16;
+constexpr int MB_NBMAX = 8;
+
+// Eigenvalues of a small m x m (m <= 8) complex matrix held in LDS (ld 8), by
+// single-shift QR without vectors; run by one thread.
+ORF_DEV void small_eigvals(int m, cplx* A, cplx* ev) {
+  const double smlnum = SAFMIN * ((double)m / ULP);
+  int i = m - 1;
+  int guard = 0;
+  while (i >= 0 && guard < 60 * m) {
+    int l = 0;
+    for (int k = i; k > 0; --k) {
+      const double tst = cabs1(A[(k - 1) + 8 * (k - 1)]) + cabs1(A[k + 8 * k]);
+      if (cabs1(A[k + 8 * (k - 1)]) <= fmax(smlnum, ULP * tst)) { l = k; break; }
+    }
+    if (l >= i) { ev[i] = A[i + 8 * i]; --i; guard = 0; continue; }
+    ++guard;
+    cplx tt = A[i + 8 * i];
+    if (guard % 10 == 0) {
+      tt = cadd(tt, cmk(0.75 * cabs1(A[i + 8 * (i - 1)]), 0.0));
+    } else {
+      const cplx u = cmul(csqrt(A[(i - 1) + 8 * i]), csqrt(A[i + 8 * (i - 1)]));
+      double sm = cabs1(u);
+      if (sm != 0.0) {
+        const cplx x = cscale(csub(A[(i - 1) + 8 * (i - 1)], tt), 0.5);
+        const double sx = cabs1(x);
+        sm = fmax(sm, sx);
+        const cplx xs = cscale(x, 1.0 / sm), us = cscale(u, 1.0 / sm);
+        cplx y = cscale(csqrt(cadd(cmul(xs, xs), cmul(us, us))), sm);
+        if (sx > 0.0) {
+          const cplx xx = cscale(x, 1.0 / sx);
+          if (xx.x * y.x + xx.y * y.y < 0.0) y = cneg(y);
+        }
+        tt = csub(tt, cmul(u, cdiv(u, cadd(x, y))));
+      }
+    }
+    for (int k = l; k < i; ++k) {
+      cplx a, b;
+      if (k == l) { a = csub(A[l + 8 * l], tt); b = A[(l + 1) + 8 * l]; }
+      else { a = A[k + 8 * (k - 1)]; b = A[(k + 1) + 8 * (k - 1)]; }
+      double c;
+      cplx sn, r;
+      givens(a, b, c, sn, r);
+      const cplx sc = cconj(sn);
+      if (k > l) { A[k + 8 * (k - 1)] = r; A[(k + 1) + 8 * (k - 1)] = cmk(0.0, 0.0); }
+      for (int j = k; j <= i; ++j) {
+        const cplx h1 = A[k + 8 * j], h2 = A[(k + 1) + 8 * j];
+        A[k + 8 * j] = cadd(cscale(h1, c), cmul(sn, h2));
+        A[(k + 1) + 8 * j] = csub(cscale(h2, c), cmul(sc, h1));
+      }
+      for (int q = l; q <= min(k + 2, i); ++q) {
+        const cplx h1 = A[q + 8 * k], h2 = A[q + 8 * (k + 1)];
+        A[q + 8 * k] = cadd(cscale(h1, c), cmul(sc, h2));
+        A[q + 8 * (k + 1)] = csub(cscale(h2, c), cmul(sn, h1));
+      }
+    }
+  }
+  for (; i >= 0; --i) ev[i] = A[i + 8 * i];  // not converged: use the diagonal
+}
+
+template <int MB_NB>
+__global__ __launch_bounds__(256) void schur_qr_mb_kernel(int n, cplx* __restrict__ Hg, int ldh,
+                                                          long long sH, cplx* __restrict__ Zg,
+                                                          long long sZ, cplx* __restrict__ wout,
+                                                          int* __restrict__ info, int hInLds,
+                                                          int zInLds) {
+  constexpr int MB_SW = MB_SEG + 3 * (MB_NB - 1) + 3;
+  extern __shared__ cplx lds_mat[];
+  __shared__ cplx win[MB_SW * MB_SW];
+  __shared__ double rc[MB_NB * MB_SEG];
+  __shared__ cplx rs[MB_NB * MB_SEG];
+  __shared__ cplx sm8[64];
+  __shared__ cplx shifts[MB_NB];
+  __shared__ cplx rr8[MB_NB];
+  __shared__ int redk[4];
+  const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
+  const int lane = t & 63, wv = t >> 6, nw = nt >> 6;
+  Hg += (size_t)page * sH;
+  cplx* Zp = Zg + (size_t)page * sZ;
+  wout += (size_t)page * n;
+  cplx* H = Hg;
+  int ld = ldh;
+  cplx* Z = Zp;
+  cplx* ldsNext = lds_mat;
+  if (hInLds) {
+    H = ldsNext;
+    ld = n;
+    ldsNext += (size_t)n * n;
+    for (int e = t; e < n * n; e += nt) H[e] = Hg[(size_t)(e % n) + (size_t)(e / n) * ldh];
+  }
+  if (zInLds) {
+    Z = ldsNext;
+    for (int e = t; e < n * n; e += nt) Z[e] = Zp[e];
+  }
+  __syncthreads();
+#define HH(r, c) H[(size_t)(r) + (size_t)(c) * ld]
+#define WN(r, c) win[(r) + (c) * MB_SW]
+  const double smlnum = SAFMIN * ((double)n / ULP);
+  const int itmax = 60;
+  int failed = 0;
+  int i = n - 1;
+  while (i >= 0) {
+    int l = 0;
+    int its;
+    for (its = 0; its <= itmax; ++its) {
+      int kfound = l;
+      for (int k = i - t; k > l; k -= nt) {
+        const cplx hkk1 = HH(k, k - 1);
+        bool small = false;
+        if (cabs1(hkk1) <= smlnum) small = true;
+        else {
+          double tst = cabs1(HH(k - 1, k - 1)) + cabs1(HH(k, k));
+          if (tst == 0.0) {
+            if (k - 2 >= 0) tst += cabs1(HH(k - 1, k - 2));
+            if (k + 1 <= n - 1) tst += cabs1(HH(k + 1, k));
+          }
+          if (cabs1(hkk1) <= ULP * tst) {
+            const double ab = fmax(cabs1(hkk1), cabs1(HH(k - 1, k)));
+            const double ba = fmin(cabs1(hkk1), cabs1(HH(k - 1, k)));
+            const cplx dd = csub(HH(k - 1, k - 1), HH(k, k));
+            const double aa = fmax(cabs1(HH(k, k)), cabs1(dd));
+            const double bb = fmin(cabs1(HH(k, k)), cabs1(dd));
+            const double s = aa + ab;
+            if (ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)))) small = true;
+          }
+        }
+        if (small) { kfound = k; break; }
+      }
+      int kv = kfound;
+      for (int o = 32; o > 0; o >>= 1) kv = max(kv, __shfl_xor(kv, o, 64));
+      __syncthreads();
+      if (lane == 0) redk[wv] = kv;
+      __syncthreads();
+      int kmax = redk[0];
+      for (int q = 1; q < nw; ++q) kmax = max(kmax, redk[q]);
+      l = kmax;
+      if (l > 0) {
+        __syncthreads();
+        if (t == 0) HH(l, l - 1) = cmk(0.0, 0.0);
+        __syncthreads();
+      }
+      if (l >= i) break;
+      // ---- shifts
+      const int na = i - l + 1;
+      const bool exc = (its > 0 && its % 10 == 0);
+      const int nbul = (exc || na < 3 * MB_NB + 4) ? 1 : MB_NB;
+      if (nbul == 1) {
+        cplx sig;
+        if (its % 20 == 10) {
+          sig = cadd(cmk(0.75 * cabs1(HH(l + 1, l)), 0.0), HH(l, l));
+        } else if (its > 0 && its % 20 == 0) {
+          sig = cadd(cmk(0.75 * cabs1(HH(i, i - 1)), 0.0), HH(i, i));
+        } else {
+          cplx tt = HH(i, i);
+          const cplx u = cmul(csqrt(HH(i - 1, i)), csqrt(HH(i, i - 1)));
+          double s = cabs1(u);
+          if (s != 0.0) {
+            const cplx x = cscale(csub(HH(i - 1, i - 1), tt), 0.5);
+            const double sx = cabs1(x);
+            s = fmax(s, cabs1(x));
+            const cplx xs = cscale(x, 1.0 / s), us = cscale(u, 1.0 / s);
+            cplx y = cscale(csqrt(cadd(cmul(xs, xs), cmul(us, us))), s);
+            if (sx > 0.0) {
+              const cplx xx = cscale(x, 1.0 / sx);
+              if (xx.x * y.x + xx.y * y.y < 0.0) y = cneg(y);
+            }
+            tt = csub(tt, cmul(u, cdiv(u, cadd(x, y))));
+          }
+          sig = tt;
+        }
+        __syncthreads();
+        if (t == 0) shifts[0] = sig;
+        __syncthreads();
+      } else {
+        __syncthreads();
+        if (t < MB_NB * MB_NB) {
+          const int r = t % MB_NB, c = t / MB_NB;
+          sm8[r + 8 * c] = HH(i - MB_NB + 1 + r, i - MB_NB + 1 + c);
+        }
+        __syncthreads();
+        if (t == 0) small_eigvals(MB_NB, sm8, shifts);
+        __syncthreads();
+      }
+      bool bad = false;
+      for (int b = 0; b < nbul; ++b) bad |= !(isfinite(shifts[b].x) && isfinite(shifts[b].y));
+      if (bad) {
+        its = itmax + 1;
+        break;
+      }
+      // ---- one pass: segments of MB_SEG positions of bulge 0
+      for (int p0 = l; p0 <= i - 1 + 3 * (nbul - 1); p0 += MB_SEG) {
+        const int a0 = p0 - 3 * (MB_NB - 1) - 1;  // window origin (may be < 0)
+        __syncthreads();
+        for (int q = t; q < MB_SW * MB_SW; q += nt) {
+          const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
+          win[q] = (r >= 0 && r < n && c >= 0 && c < n) ? HH(r, c) : cmk(0.0, 0.0);
+        }
+        __syncthreads();
+        if (wv == 0) {
+          // lockstep: at step j bulge b sits at position p0 - 3b + j; the rotations
+          // of one step act on disjoint row and column pairs
+          for (int j = 0; j < MB_SEG; ++j) {
+            // A: lane b computes bulge b's rotation
+            if (lane < MB_NB) {
+              const int b = lane, k = p0 - 3 * b + j;
+              const bool valid = b < nbul && k >= l && k <= i - 1;
+              double c = 1.0;
+              cplx sn = cmk(0.0, 0.0), r = cmk(0.0, 0.0);
+              if (valid) {
+                const int kk = k - a0;
+                cplx x, y;
+                if (k == l) { x = csub(WN(kk, kk), shifts[b]); y = WN(kk + 1, kk); }
+                else { x = WN(kk, kk - 1); y = WN(kk + 1, kk - 1); }
+                givens(x, y, c, sn, r);
+              }
+              rc[b * MB_SEG + j] = c;
+              rs[b * MB_SEG + j] = sn;
+              rr8[b] = r;
+            }
+            wave_sync_lds();
+            // B: row ops of every bulge on window columns kk..MB_SW-1
+            for (int e = lane; e < MB_NB * MB_SW; e += 64) {
+              const int b = e / MB_SW, jc = e % MB_SW;
+              const int k = p0 - 3 * b + j;
+              if (!(b < nbul && k >= l && k <= i - 1)) continue;
+              const int kk = k - a0;
+              if (jc < kk || a0 + jc >= n) continue;
+              const double c = rc[b * MB_SEG + j];
+              const cplx sn = rs[b * MB_SEG + j];
+              const cplx h1 = WN(kk, jc), h2 = WN(kk + 1, jc);
+              WN(kk, jc) = cadd(cscale(h1, c), cmul(sn, h2));
+              WN(kk + 1, jc) = csub(cscale(h2, c), cmul(cconj(sn), h1));
+            }
+            if (lane < MB_NB) {
+              const int b = lane, k = p0 - 3 * b + j;
+              if (b < nbul && k > l && k <= i - 1) {
+                const int kk = k - a0;
+                WN(kk, kk - 1) = rr8[b];
+                WN(kk + 1, kk - 1) = cmk(0.0, 0.0);
+              }
+            }
+            wave_sync_lds();
+            // C: column ops of every bulge on window rows 0..min(kk+2, i-a0)
+            for (int e = lane; e < MB_NB * MB_SW; e += 64) {
+              const int b = e / MB_SW, q = e % MB_SW;
+              const int k = p0 - 3 * b + j;
+              if (!(b < nbul && k >= l && k <= i - 1)) continue;
+              const int kk = k - a0;
+              if (q > min(kk + 2, i - a0) || a0 + q < 0) continue;
+              const double c = rc[b * MB_SEG + j];
+              const cplx sn = rs[b * MB_SEG + j];
+              const cplx h1 = WN(q, kk), h2 = WN(q, kk + 1);
+              WN(q, kk) = cadd(cscale(h1, c), cmul(cconj(sn), h2));
+              WN(q, kk + 1) = csub(cscale(h2, c), cmul(sn, h1));
+            }
+            wave_sync_lds();
+          }
+        }
+        __syncthreads();
+        for (int q = t; q < MB_SW * MB_SW; q += nt) {
+          const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
+          if (r >= 0 && r < n && c >= 0 && c < n) HH(r, c) = win[q];
+        }
+        // deferred: columns right of the window, rows above it, rows of Z
+        const int e0 = a0 + MB_SW;                    // first column right of the window
+        const int ncR = e0 < n ? n - e0 : 0;
+        const int nrA = a0 > 0 ? a0 : 0;
+        const int tasks = ncR + nrA + n;
+        for (int task = t; task < tasks; task += nt) {
+          cplx v[MB_SW];
+          cplx* base;
+          size_t stride;
+          bool rowop;
+          if (task < ncR) {
+            base = H + (size_t)(e0 + task) * ld + a0;  // column e0+task, rows a0..
+            stride = 1;
+            rowop = true;
+          } else if (task < ncR + nrA) {
+            base = H + (task - ncR) + (size_t)a0 * ld;  // row task-ncR, columns a0..
+            stride = ld;
+            rowop = false;
+          } else {
+            base = Z + (task - ncR - nrA) + (ptrdiff_t)a0 * n;
+            stride = n;
+            rowop = false;
+          }
+#pragma unroll
+          for (int q = 0; q < MB_SW; ++q) {
+            const int g = a0 + q;
+            v[q] = (g >= 0 && g < n) ? base[(ptrdiff_t)q * (ptrdiff_t)stride] : cmk(0.0, 0.0);
+          }
+          if (rowop) {
+#pragma unroll
+            for (int b = 0; b < MB_NB; ++b)
+#pragma unroll
+              for (int j = 0; j < MB_SEG; ++j) {
+                const int q = 3 * (MB_NB - 1 - b) + 1 + j;
+                const double c = rc[b * MB_SEG + j];
+                const cplx sn = rs[b * MB_SEG + j];
+                const cplx h1 = v[q], h2 = v[q + 1];
+                v[q] = cadd(cscale(h1, c), cmul(sn, h2));
+                v[q + 1] = csub(cscale(h2, c), cmul(cconj(sn), h1));
+              }
+          } else {
+#pragma unroll
+            for (int b = 0; b < MB_NB; ++b)
+#pragma unroll
+              for (int j = 0; j < MB_SEG; ++j) {
+                const int q = 3 * (MB_NB - 1 - b) + 1 + j;
+                const double c = rc[b * MB_SEG + j];
+                const cplx sn = rs[b * MB_SEG + j];
+                const cplx h1 = v[q], h2 = v[q + 1];
+                v[q] = cadd(cscale(h1, c), cmul(cconj(sn), h2));
+                v[q + 1] = csub(cscale(h2, c), cmul(sn, h1));
+              }
+          }
+#pragma unroll
+          for (int q = 0; q < MB_SW; ++q) {
+            const int g = a0 + q;
+            if (g >= 0 && g < n) base[(ptrdiff_t)q * (ptrdiff_t)stride] = v[q];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (its > itmax) {
+      if (!failed) failed = i + 1;
+      l = i;
+    }
+    if (t == 0) wout[i] = HH(i, i);
+    i = l - 1;
+    __syncthreads();
+  }
+  if (t == 0 && info) info[page] = failed;
+  if (hInLds)
+    for (int e = t; e < n * n; e += nt) Hg[(size_t)(e % n) + (size_t)(e / n) * ldh] = H[e];
+  if (zInLds)
+    for (int e = t; e < n * n; e += nt) Zp[e] = Z[e];
+#undef WN
+#undef HH
+}
+
 // Eigenvectors of upper-triangular T (ztrevc, side R, no back-transform):
 // Y[:, k] = x with (T - T[k,k]) x = 0, x[k] = 1, x[j>k] = 0, scaled to max |.|_1 = 1.
 // One wave per eigenvector; x distributed over lanes, 64 rows per slot.
@@ -1365,30 +1881,60 @@ __global__ __launch_bounds__(256) void colnorm_kernel(int n, cplx* __restrict__ 
 
 // A is overwritten by the Schur factor T; Zwork and Ywork hold n x n per page.
 void orf_zgeev(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V, int ldv,
-               long long sV, cplx* Zwork, cplx* Ywork, int* info, int batch) {
-  orf_zgeev_timed(st, n, A, lda, sA, w, V, ldv, sV, Zwork, Ywork, info, batch, OrfTimeHook());
+               long long sV, cplx* Zwork, cplx* Ywork, cplx* Hwork, int* info, int batch) {
+  orf_zgeev_timed(st, n, A, lda, sA, w, V, ldv, sV, Zwork, Ywork, Hwork, info, batch, OrfTimeHook());
+}
+
+static __global__ void eye_batch_kernel(int n, cplx* Z, long long sZ) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  Z[(size_t)blockIdx.y * sZ + e] = cmk((e % n) == (e / n) ? 1.0 : 0.0, 0.0);
+}
+
+static void eye_batch(hipStream_t st, int n, cplx* Z, long long sZ, int batch) {
+  for (int b0 = 0; b0 < batch; b0 += 65535) {
+    const int bb = batch - b0 < 65535 ? batch - b0 : 65535;
+    eye_batch_kernel<<<dim3((unsigned)(((long long)n * n + 255) / 256), bb), 256, 0, st>>>(n, Z + (size_t)b0 * sZ, sZ);
+  }
 }
 
 void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V,
-                     int ldv, long long sV, cplx* Zwork, cplx* Ywork, int* info, int batch,
-                     const OrfTimeHook& hook) {
+                     int ldv, long long sV, cplx* Zwork, cplx* Ywork, cplx* Hwork, int* info,
+                     int batch, const OrfTimeHook& hook) {
   if (n <= 0 || batch <= 0) return;
   auto tb = [&](const char* nm) { if (hook) hook(nm, 0); };
   auto te = [&](const char* nm) { if (hook) hook(nm, 1); };
   const long long sZ = (long long)n * n;
   const size_t bytes = (size_t)n * n * sizeof(cplx);
   const int use_lds = bytes <= (size_t)g_lds_limit;
+  // Householder vectors and tau live in Vh (n*n + n per page), the first
+  // n*n of which double as scratch for nothing else until Q is applied.
+  cplx* Vh = Hwork;
+  cplx* tau = Hwork + (size_t)n * n * batch;
   tb("eig_hessenberg");
-  hessenberg_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, Zwork, sZ, use_lds);
+  hessenberg_noq_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, Vh, tau, use_lds);
+  eye_batch(st, n, Zwork, sZ, batch);
   te("eig_hessenberg");
   tb("eig_schur_qr");
   {
-    // LDS residency: H and Z both, H only, or neither (static LDS ~21 KB)
-    const size_t stat = 22 * 1024;
-    const size_t lim = (size_t)g_lds_limit + 20 * 1024 - stat;
-    const int hIn = bytes <= lim, zIn = 2 * bytes <= lim;
-    schur_qr_win_kernel<<<batch, 256, (hIn + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w, info,
-                                                                  hIn, zIn);
+    // Small matrices: one rotation at a time with H (and Z when it fits) in LDS.
+    // Larger ones: multi-bulge passes over a window, off-window work deferred.
+    const size_t lim = (size_t)g_lds_limit;
+    const int nbul = g_qr_bulges;
+    if (bytes <= lim && (n <= g_qr_small_n || nbul == 1)) {
+      const int zIn = 2 * bytes <= lim;
+      schur_qr_kernel<<<batch, 256, (1 + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w, info, 1, zIn);
+    } else {
+      const size_t stat = 32 * 1024;
+      const size_t lim2 = lim + 20 * 1024 - stat;
+      const int hIn = bytes <= lim2, zIn = 2 * bytes <= lim2;
+      if (nbul >= 8)
+        schur_qr_mb_kernel<8><<<batch, 256, (hIn + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w, info,
+                                                                        hIn, zIn);
+      else
+        schur_qr_mb_kernel<4><<<batch, 256, (hIn + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w, info,
+                                                                        hIn, zIn);
+    }
   }
   te("eig_schur_qr");
   dim3 g2((n + 3) / 4, batch);
@@ -1396,6 +1942,11 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
   trevc_kernel<<<g2, 256, 0, st>>>(n, A, lda, sA, Ywork, n, sZ);
   orf_zgemm(st, 'N', 'N', n, n, n, cmk(1.0, 0.0), Zwork, n, sZ, Ywork, n, sZ, cmk(0.0, 0.0), V,
             ldv, sV, batch);
+  for (int b0 = 0; b0 < batch; b0 += 65535) {
+    const int bb = batch - b0 < 65535 ? batch - b0 : 65535;
+    hess_q_apply_kernel<<<dim3((n + 15) / 16, bb), 256, (size_t)n * 16 * sizeof(cplx), st>>>(
+        n, Vh + (size_t)b0 * n * n, tau + (size_t)b0 * n, V + (size_t)b0 * sV, ldv, sV);
+  }
   colnorm_kernel<<<g2, 256, 0, st>>>(n, V, ldv, sV);
   te("eig_vectors");
 }

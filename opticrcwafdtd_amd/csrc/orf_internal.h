@@ -25,9 +25,12 @@ long long orf_dinv_stride(int n);
 void orf_zgetrs(hipStream_t st, char side, int n, int nrhs, const cplx* LU, int lda, long long sA,
                 const int* ipiv, const cplx* dinv, cplx* B, int ldb, long long sB, int batch);
 // Eigen-decomposition; A (overwritten), w [n x batch], V (unit-norm eigenvectors),
-// Zwork/Ywork n*n per page; info [batch].
+// Zwork/Ywork n*n per page, Hwork (n*n + n) per page; info [batch].
 void orf_zgeev(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V, int ldv,
-               long long sV, cplx* Zwork, cplx* Ywork, int* info, int batch);
+               long long sV, cplx* Zwork, cplx* Ywork, cplx* Hwork, int* info, int batch);
 void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V,
-                     int ldv, long long sV, cplx* Zwork, cplx* Ywork, int* info, int batch,
-                     const OrfTimeHook& hook);
+                     int ldv, long long sV, cplx* Zwork, cplx* Ywork, cplx* Hwork, int* info,
+                     int batch, const OrfTimeHook& hook);
+// Schur QR dispatch knobs: bulges per multishift pass and the size below which the
+// per-rotation LDS kernel is used.
+void orf_set_qr_params(int bulges, int small_n);

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -317,7 +318,7 @@ struct Work {
   cplx *E, *Exx, *Eyy, *G, *Omega, *Toe, *Inv, *GH, *HH;
   int* ipivT;
   // solve-page buffers (Cs = Cp * nSec)
-  cplx *A0, *Wm, *Vm, *Zw, *Yw, *mu, *lam, *Xd, *Wf, *QW, *T12;
+  cplx *A0, *Wm, *Vm, *Zw, *Yw, *Hv, *mu, *lam, *Xd, *Wf, *QW, *T12;
   cplx *X1, *X2, *XB, *R3, *Dm, *RHS2, *LUb;
   cplx *Sa[4], *Sn[4], *Sb[4], *Tm, *M1, *Fm;
   double* kL;
@@ -345,7 +346,7 @@ static size_t per_mode_bytes(const orf_plan& P) {
 static size_t per_solve_bytes(const orf_plan& P) {
   const size_t n2 = (size_t)P.nr * P.nr, nfr = (size_t)P.nf * P.nr;
   size_t c = 0;
-  c += n2 * 5;               // A0, Wm, Vm, Zw, Yw
+  c += n2 * 6 + P.nr;        // A0, Wm, Vm, Zw, Yw, Hv (+tau)
   c += 3 * nfr;              // Wf, QW, T12
   c += n2 * (2 + 1 + 3 + 1 + 2 + 1);  // X1 X2 XB R3 Dm RHS2 LUb
   c += n2 * (12 + 3);        // Sa Sn Sb Tm M1 Fm
@@ -385,6 +386,7 @@ static Work make_work(orf_plan& P, int Cp) {
   w.Vm = (cplx*)get(cs * n2 * Cs);
   w.Zw = (cplx*)get(cs * n2 * Cs);
   w.Yw = (cplx*)get(cs * n2 * Cs);
+  w.Hv = (cplx*)get(cs * (n2 + P.nr) * Cs);
   w.mu = (cplx*)get(cs * P.nr * Cs);
   w.lam = (cplx*)get(cs * P.nr * Cs);
   w.Xd = (cplx*)get(cs * P.nr * Cs);
@@ -1025,7 +1027,7 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         }
         {
           STAGE("eig");
-          orf_zgeev_timed(st, nr, w.A0, nr, n2, w.mu, w.Wm, nr, n2, w.Zw, w.Yw, w.info, Cs,
+          orf_zgeev_timed(st, nr, w.A0, nr, n2, w.mu, w.Wm, nr, n2, w.Zw, w.Yw, w.Hv, w.info, Cs,
                           [&](const char* nm, int phase) {
                             if (phase == 0) P.timer.begin(nm, st); else P.timer.end(st);
                           });
@@ -1163,6 +1165,8 @@ int orf_init(int device, orf_ctx** out) {
     // leave room for each kernel's static LDS
     const int lim = (int)std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024) - 20 * 1024;
     orf_set_lds_limit(std::max(lim, 32 * 1024));
+    if (const char* e = getenv("ORF_QR_BULGES")) orf_set_qr_params(atoi(e), -1);
+    if (const char* e = getenv("ORF_QR_SMALL_N")) orf_set_qr_params(0, atoi(e));
     *out = c;
   })
 }
@@ -1327,7 +1331,8 @@ int orf_pageeig(orf_ctx* ctx, int32_t n, const double* A, double* wout, double* 
     if (2 * ((n + 63) / 64) > 24 || n > 768) fail(ORF_INVALID_ARG, "n > 768 not supported");
     HIPCHECK(hipSetDevice(ctx->device));
     const size_t n2 = (size_t)n * n;
-    DBuf a, wv, v, z, y, info;
+    DBuf a, wv, v, z, y, hv, info;
+    hv.alloc((n2 + n) * 16 * batch);
     a.alloc(batch * n2 * 16);
     wv.alloc((size_t)batch * n * 16);
     v.alloc(batch * n2 * 16);
@@ -1336,7 +1341,7 @@ int orf_pageeig(orf_ctx* ctx, int32_t n, const double* A, double* wout, double* 
     info.alloc(sizeof(int) * batch);
     HIPCHECK(hipMemcpy(a.p, A, batch * n2 * 16, hipMemcpyHostToDevice));
     orf_zgeev(ctx->stream, n, a.as<cplx>(), n, n2, wv.as<cplx>(), v.as<cplx>(), n, n2, z.as<cplx>(), y.as<cplx>(),
-              info.as<int>(), batch);
+              hv.as<cplx>(), info.as<int>(), batch);
     KCHECK();
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     std::vector<int> h(batch);
