@@ -59,6 +59,7 @@ __global__ void rowscale_kernel(int, int, const cplx*, cplx*, long long);
 __global__ void eye_axpby_kernel(int, cplx, cplx, const cplx*, long long, cplx*, long long);
 __global__ void gather_pages_kernel(long long, const int*, const cplx*, long long, cplx*, long long);
 __global__ void eye_stack_kernel(int, long long, cplx*);
+__global__ void select_cols_kernel(int, int, const int*, const cplx*, long long, cplx*, long long);
 __global__ void output_kernel(int, int, int, int, int, const DevSolve*, const DevPage*, const DevSector*,
                               const DevSource*, const double*, const double*, const cplx*, const cplx*, long long,
                               double*, double*, double*, double*, cplx*, cplx*, int);
@@ -285,6 +286,7 @@ struct orf_plan {
   DBuf dSectorData, dSectors, dSources;
   std::vector<DBuf*> secArrays;
   std::vector<DevSource> sources;
+  std::vector<int> sectorSrcCol;               // reduced basis: source column per sector
   // shared per (lambda, sector) region data, nr x nr each
   DBuf dV0, dSref, dStrn;                      // V0 [L*nSec], S [4][L*nSec]
   // chunking
@@ -326,6 +328,7 @@ struct Work {
   int *ipiv, *info;
   DevSolve* solves;
   int* groupOf;
+  int* colIdx;
   cplx* epsSolve;
 };
 
@@ -414,6 +417,7 @@ static Work make_work(orf_plan& P, int Cp) {
   w.info = (int*)get(sizeof(int) * (size_t)std::max(Cs, Cp) * 64);
   w.solves = (DevSolve*)get(sizeof(DevSolve) * Cs);
   w.groupOf = (int*)get(sizeof(int) * Cs);
+  w.colIdx = (int*)get(sizeof(int) * 2 * Cs);
   w.epsSolve = (cplx*)get(cs * Cs);
   return w;
 }
@@ -527,6 +531,33 @@ static void smatrix_star(orf_plan& P, Work& w, int Cs, cplx* const SA[4], cplx* 
   orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Fm, n, n2, SA[3], n, n2, ZERO, w.Tm, n, n2, Cs);
   HIPCHECK(hipMemcpyAsync(out[3], SB[3], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
   orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Tm, n, n2, SB[1], n, n2, ONE, out[3], n, n2, Cs);
+}
+
+// Last star product S = SA * S_trn restricted to the source columns (rcwa2D
+// needs r = S11 c and t = S21 c only, smatrixStar.m:5-10 applied to vectors):
+// R = SA11 C + SA12 (I - SB11 SA22)^-1 SB11 SA21 C,  T = SB21 (I - SA22 SB11)^-1 SA21 C.
+static void smatrix_star_cols(orf_plan& P, Work& w, int Cs, cplx* const SA[4], cplx* const SB[4], int ncol,
+                              const int* colIdx, cplx* R, cplx* T) {
+  const int n = P.nr;
+  const long long n2 = (long long)n * n, nc2 = (long long)n * ncol;
+  hipStream_t st = P.st;
+  STAGE("star_final");
+  cplx* U = w.Tm;          // SA21 C
+  cplx* Vv = w.Tm + nc2 * Cs;  // SB11 U, then X = M1^-1 V
+  select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, SA[2], n2, U, nc2);
+  select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, SA[0], n2, R, nc2);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, SB[0], n, n2, U, n, nc2, ZERO, Vv, n, nc2, Cs);
+  eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.M1, n2);
+  orf_zgemm(st, 'N', 'N', n, n, n, MONE, SB[0], n, n2, SA[3], n, n2, ONE, w.M1, n, n2, Cs);
+  orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs, w.dinv);
+  orf_zgetrs(st, 'L', n, ncol, w.M1, n, n2, w.ipiv, w.dinv, Vv, n, nc2, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, SA[1], n, n2, Vv, n, nc2, ONE, R, n, nc2, Cs);
+  eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.M1, n2);
+  orf_zgemm(st, 'N', 'N', n, n, n, MONE, SA[3], n, n2, SB[0], n, n2, ONE, w.M1, n, n2, Cs);
+  orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs, w.dinv);
+  orf_zgetrs(st, 'L', n, ncol, w.M1, n, n2, w.ipiv, w.dinv, U, n, nc2, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, SB[2], n, n2, U, n, nc2, ZERO, T, n, nc2, Cs);
+  KCHECK();
 }
 
 // ---------------------------------------------------------------- region S-matrices
@@ -859,6 +890,8 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
       fail(ORF_INVALID_ARG, "dispersive incident medium across a multi-wavelength batch is not supported");
     }
   P.dSources.upload(P.sources);
+  P.sectorSrcCol.assign(std::max(1, P.nSec), 0);
+  for (auto& so : P.sources) P.sectorSrcCol[so.sector] = so.srcCol;
   // Fourier tables and geometry transforms
   if (P.nGrids > 0) {
     const int Hx = 4 * P.Nx + 1, Hy = 4 * P.Ny + 1;
@@ -1093,16 +1126,27 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         }
       }
     }
-    // S = S * S_trn
-    for (int k = 0; k < 4; ++k)
+    // S = S * S_trn, source columns only
+    for (int k = 0; k < 4; k += 2)
       gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dStrn.as<cplx>() + k * n2 * G, n2, w.Sb[k], n2);
-    smatrix_star(P, w, Cs, w.Sa, w.Sb, w.Sn);
+    const int ncol = P.reduced ? 1 : 2;
+    {
+      std::vector<int> cols((size_t)Cs * ncol);
+      for (int sp = 0; sp < Cs; ++sp) {
+        if (P.reduced) cols[sp] = P.sectorSrcCol[sv[sp].sector];
+        else { cols[2 * sp] = P.i0; cols[2 * sp + 1] = P.M + P.i0; }
+      }
+      HIPCHECK(hipMemcpyAsync(w.colIdx, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, st));
+    }
+    cplx* Rsel = w.Sn[0];
+    cplx* Tsel = w.Sn[2];
+    smatrix_star_cols(P, w, Cs, w.Sa, w.Sb, ncol, w.colIdx, Rsel, Tsel);
     {
       STAGE("output");
       output_kernel<<<dim3(Cs, P.nPol), 256, 0, st>>>(
           M, nr, P.reduced, P.B, P.L, w.solves, basePages, P.dSectors.as<DevSector>(), P.dSources.as<DevSource>(),
-          P.dKxy.as<double>(), (const double*)P.dKz.p, w.Sn[0], w.Sn[2], n2, P.dR.as<double>(), P.dT.as<double>(),
-          P.dRtot.as<double>(), P.dTtot.as<double>(), P.dr.as<cplx>(), P.dt.as<cplx>(), P.i0);
+          P.dKxy.as<double>(), (const double*)P.dKz.p, Rsel, Tsel, (long long)nr * ncol, P.dR.as<double>(),
+          P.dT.as<double>(), P.dRtot.as<double>(), P.dTtot.as<double>(), P.dr.as<cplx>(), P.dt.as<cplx>(), P.i0);
       KCHECK();
     }
     if (progress && P.nLayers == 0) {

@@ -492,17 +492,19 @@ __global__ void output_kernel(int M, int nr, int reduced, int B, int L, const De
     for (int comp = 0; comp < 2; ++comp) {
       const int i = comp * M + p;
       cplx rv = cmk(0.0, 0.0), tv = cmk(0.0, 0.0);
+      // s11 / s21 hold the columns of S11 / S21 selected for the sources:
+      // full basis: columns (i0, M + i0); parity sector: the source column srcCol
       if (!reduced) {
-        rv = cadd(cmul(s11[i + (size_t)nr * i0], so.ex), cmul(s11[i + (size_t)nr * (M + i0)], so.ey));
-        tv = cadd(cmul(s21[i + (size_t)nr * i0], so.ex), cmul(s21[i + (size_t)nr * (M + i0)], so.ey));
+        rv = cadd(cmul(s11[i], so.ex), cmul(s11[i + (size_t)nr], so.ey));
+        tv = cadd(cmul(s21[i], so.ex), cmul(s21[i + (size_t)nr], so.ey));
       } else {
         const DevSector& sec = sectors[S.sector];
         const int a = sec.ue_rowmap[i];
         if (a >= 0) {
           const double co = sec.ue_rowcoef[i];
-          // c_r = UE' c has one nonzero at column srcCol with value srcVal
-          rv = cscale(cmul(s11[a + (size_t)nr * so.srcCol], so.srcVal), co);
-          tv = cscale(cmul(s21[a + (size_t)nr * so.srcCol], so.srcVal), co);
+          // c_r = UE' c has one nonzero, at the selected column, of value srcVal
+          rv = cscale(cmul(s11[a], so.srcVal), co);
+          tv = cscale(cmul(s21[a], so.srcVal), co);
         }
       }
       rxy[comp] = rv;
@@ -540,4 +542,14 @@ __global__ void eye_stack_kernel(int n, long long count, cplx* __restrict__ Y) {
   if (e >= n2 * count) return;
   const long long r = e % n2;
   Y[e] = cmk((r % n) == (r / n) ? 1.0 : 0.0, 0.0);
+}
+
+// out[:, q] = S[:, colIdx[page * ncol + q]]  (n x ncol per page)
+__global__ void select_cols_kernel(int n, int ncol, const int* __restrict__ colIdx, const cplx* __restrict__ S,
+                                   long long sS, cplx* __restrict__ out, long long sO) {
+  const int sp = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * ncol) return;
+  const int r = e % n, q = e / n;
+  out[(size_t)sp * sO + e] = S[(size_t)sp * sS + r + (size_t)n * colIdx[sp * ncol + q]];
 }
