@@ -92,10 +92,17 @@ typedef struct orf_problem {
   double gapEps;                /* NaN: every gap harmonic propagates           */
   int32_t precision;            /* ORF_PREC_*                                    */
   int32_t batchSize;            /* structures per chunk; 0 = auto by memory     */
+  /* ---- uneven batches (build extension; zero / NULL = reference behaviour) */
+  const double* structPeriod;   /* [2 x nStruct] period of each structure, NULL: Lx, Ly */
+  int32_t nPages;               /* explicit (structure, wavelength) pages; 0: all pairs */
+  const int32_t* pageStruct;    /* [nPages] structure index                     */
+  const int32_t* pageLambda;    /* [nPages] wavelength index                    */
 } orf_problem;
 
 /* Results, column-major like the reference's out struct (rcwa2D.m:31-34),
- * with wavelength and polarisation appended as trailing dimensions. */
+ * with wavelength and polarisation appended as trailing dimensions.  With
+ * explicit pages the (structure, wavelength) pair is replaced by the page:
+ * R is [M x nPages x nPol], Rtot [nPages x nPol], r [3 x M x nPages x nPol]. */
 typedef struct orf_result {
   int32_t* m;     /* [M]                                                     */
   int32_t* n;     /* [M]                                                     */

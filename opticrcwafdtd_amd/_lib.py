@@ -48,6 +48,8 @@ class orf_problem(C.Structure):
         ("nVertices", C.c_int32), ("vertices", C.POINTER(C.c_double)),
         ("factorization", C.c_int32), ("symmetry", C.c_int32), ("gapEps", C.c_double),
         ("precision", C.c_int32), ("batchSize", C.c_int32),
+        ("structPeriod", C.POINTER(C.c_double)), ("nPages", C.c_int32),
+        ("pageStruct", C.POINTER(C.c_int32)), ("pageLambda", C.POINTER(C.c_int32)),
     ]
 
 

@@ -19,6 +19,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/orf_pageops.h"
@@ -29,7 +30,7 @@
 typedef std::complex<double> zc;
 
 // kernels from rcwa_kernels.hip
-__global__ void raster_kernel(int, int, double, double, const DevGrid*, const DevShape*, const double*, uint8_t*);
+__global__ void raster_kernel(int, int, const DevGrid*, const DevShape*, const double*, uint8_t*);
 __global__ void grid_symmetry_kernel(int, int, const uint8_t*, int*);
 __global__ void chi_x_kernel(int, int, int, int, const uint8_t*, const cplx*, cplx*);
 __global__ void chi_y_kernel(int, int, int, int, const uint8_t*, const cplx*, cplx*);
@@ -61,8 +62,8 @@ __global__ void gather_pages_kernel(long long, const int*, const cplx*, long lon
 __global__ void eye_stack_kernel(int, long long, cplx*);
 __global__ void select_cols_kernel(int, int, const int*, const cplx*, long long, cplx*, long long);
 __global__ void output_kernel(int, int, int, int, int, const DevSolve*, const DevPage*, const DevSector*,
-                              const DevSource*, const double*, const double*, const cplx*, const cplx*, long long,
-                              double*, double*, double*, double*, cplx*, cplx*, int);
+                              const DevSource*, const double*, const double*, const double*, const cplx*,
+                              const cplx*, long long, double*, double*, double*, double*, cplx*, cplx*, int);
 
 // ====================================================================== errors
 static thread_local std::string g_err;
@@ -262,58 +263,6 @@ static HostSector build_sector(int Nx, int Ny, bool polP, int& nr) {
   return s;
 }
 
-// ====================================================================== plan
-struct orf_plan {
-  orf_ctx* ctx = nullptr;
-  hipStream_t st = nullptr;
-  // problem
-  double Lx = 0, Ly = 0, theta = 0, phi = 0;
-  int Nx = 0, Ny = 0, M = 0, nf = 0, nr = 0, Nsx = 0, Nsy = 0;
-  int B = 0, L = 0, nPol = 0, nLayers = 0, nSec = 1, reduced = 0;
-  int factorization = 0, precision = 0;
-  int nGrids = 0, nslotMax = 1, i0 = 0;
-  std::vector<double> lambda, k0;
-  std::vector<int> mIdx, nIdx;
-  std::vector<std::vector<int>> layerGrid;     // [layer][b] -> grid or -1
-  std::vector<std::vector<int>> layerMixed;    // unused placeholder
-  std::vector<std::vector<zc>> layerBgEps;     // [layer][b * L + l]
-  std::vector<int> layerAllPat, layerAnyPat;
-  std::vector<double> thick;                   // [layer * B + b]
-  // device-resident inputs
-  DBuf dShapes, dVerts, dGrids, dLabels, dFx, dFy, dChiX, dChiY, dChi2, dKxy, dKz;
-  std::vector<DBuf*> dPagesByLayer;            // [layer] DevPage[B*L]
-  std::vector<DBuf*> dSlotEps, dSlotInv;       // [layer] cplx[B*L*nslot]
-  DBuf dSectorData, dSectors, dSources;
-  std::vector<DBuf*> secArrays;
-  std::vector<DevSource> sources;
-  std::vector<int> sectorSrcCol;               // reduced basis: source column per sector
-  // shared per (lambda, sector) region data, nr x nr each
-  DBuf dV0, dSref, dStrn;                      // V0 [L*nSec], S [4][L*nSec]
-  // chunking
-  int chunkPages = 0;
-  // results (device)
-  DBuf dR, dT, dRtot, dTtot, dr, dt;
-  bool haveResult = false;
-  // workspace
-  std::vector<DBuf*> ws;
-  StageTimer timer;
-  ~orf_plan() {
-    for (auto* b : dPagesByLayer) delete b;
-    for (auto* b : dSlotEps) delete b;
-    for (auto* b : dSlotInv) delete b;
-    for (auto* b : secArrays) delete b;
-    for (auto* b : ws) delete b;
-  }
-};
-
-struct Timed {
-  StageTimer& t;
-  hipStream_t s;
-  Timed(StageTimer& t_, const char* n, hipStream_t s_) : t(t_), s(s_) { t.begin(n, s); }
-  ~Timed() { t.end(s); }
-};
-#define STAGE(name) Timed _tm_##__LINE__(P.timer, name, P.st)
-
 // ---------------------------------------------------------------- workspace set
 struct Work {
   // mode-page buffers (per chunk of Cp mode pages)
@@ -331,6 +280,65 @@ struct Work {
   int* colIdx;
   cplx* epsSolve;
 };
+
+// ====================================================================== plan
+struct orf_plan {
+  orf_ctx* ctx = nullptr;
+  hipStream_t st = nullptr;
+  // problem
+  double Lx = 0, Ly = 0, theta = 0, phi = 0;
+  int Nx = 0, Ny = 0, M = 0, nf = 0, nr = 0, Nsx = 0, Nsy = 0;
+  int B = 0, L = 0, nPol = 0, nLayers = 0, nSec = 1, reduced = 0;
+  int factorization = 0, precision = 0;
+  int nGrids = 0, nslotMax = 1, i0 = 0;
+  // pages: (structure, context); a context is one (wavelength, period) pair
+  int nPages = 0, nCtx = 0;
+  std::vector<int> pageB, pageCtx;
+  std::vector<double> lambda, k0;             // k0 per context
+  std::vector<int> mIdx, nIdx;
+  std::vector<std::vector<int>> layerGrid;     // [layer][b] -> grid or -1
+  std::vector<std::vector<int>> layerMixed;    // unused placeholder
+  std::vector<std::vector<zc>> layerBgEps;     // [layer][page]
+  std::vector<int> layerAllPat, layerAnyPat;
+  std::vector<double> thick;                   // [layer * B + b]
+  // device-resident inputs
+  DBuf dShapes, dVerts, dGrids, dLabels, dFx, dFy, dChiX, dChiY, dChi2, dKxy, dKz, dKzi0;
+  std::vector<DBuf*> dPagesByLayer;            // [layer] DevPage[nPages]
+  std::vector<DBuf*> dSlotEps, dSlotInv;       // [layer] cplx[nPages*nslot]
+  DBuf dSectorData, dSectors, dSources;
+  std::vector<DBuf*> secArrays;
+  std::vector<DevSource> sources;
+  std::vector<int> sectorSrcCol;               // reduced basis: source column per sector
+  // shared per (context, sector) region data, nr x nr each
+  DBuf dV0, dSref, dStrn;                      // V0 [nCtx*nSec], S [4][nCtx*nSec]
+  // chunking
+  int chunkPages = 0;
+  // results (device)
+  DBuf dR, dT, dRtot, dTtot, dr, dt;
+  bool haveResult = false;
+  // workspace of one chunk, allocated at creation and reused by every execute
+  std::vector<DBuf*> ws;
+  Work work{};
+  bool ownStream = false;
+  StageTimer timer;
+  ~orf_plan() {
+    for (auto* b : dPagesByLayer) delete b;
+    for (auto* b : dSlotEps) delete b;
+    for (auto* b : dSlotInv) delete b;
+    for (auto* b : secArrays) delete b;
+    for (auto* b : ws) delete b;
+    if (ownStream && st) (void)hipStreamDestroy(st);
+  }
+};
+
+struct Timed {
+  StageTimer& t;
+  hipStream_t s;
+  Timed(StageTimer& t_, const char* n, hipStream_t s_) : t(t_), s(s_) { t.begin(n, s); }
+  ~Timed() { t.end(s); }
+};
+#define STAGE(name) Timed _tm_##__LINE__(P.timer, name, P.st)
+
 
 static size_t per_mode_bytes(const orf_plan& P) {
   const size_t M2 = (size_t)P.M * P.M;
@@ -561,23 +569,23 @@ static void smatrix_star_cols(orf_plan& P, Work& w, int Cs, cplx* const SA[4], c
 }
 
 // ---------------------------------------------------------------- region S-matrices
-// smatrixRegion.m:8-20 with W = W0 = I for every (wavelength, sector) group.
+// smatrixRegion.m:8-20 with W = W0 = I for every (context, sector) group.
 static void build_regions(orf_plan& P, const std::vector<zc>& epsInc, const std::vector<zc>& epsSub,
                           const std::vector<zc>& epsGap) {
-  const int n = P.nr, G = P.L * P.nSec;
+  const int n = P.nr, G = P.nCtx * P.nSec;
   const long long n2 = (long long)n * n;
   hipStream_t st = P.st;
-  // region pages: one DevPage per wavelength, DevSolve per group
-  std::vector<DevPage> pg(P.L);
-  for (int l = 0; l < P.L; ++l) pg[l] = DevPage{0, l, -1, 0};
+  // region pages: one DevPage per context, DevSolve per group
+  std::vector<DevPage> pg(P.nCtx);
+  for (int c = 0; c < P.nCtx; ++c) pg[c] = DevPage{0, c, -1, 0};
   std::vector<DevSolve> sv(3 * G);
   std::vector<cplx> eps(3 * G);
   for (int k = 0; k < 3; ++k)
-    for (int l = 0; l < P.L; ++l)
+    for (int c = 0; c < P.nCtx; ++c)
       for (int s = 0; s < P.nSec; ++s) {
-        int g = k * G + l * P.nSec + s;
-        sv[g] = DevSolve{l, s};
-        eps[g] = C(k == 0 ? epsGap[l] : (k == 1 ? epsInc[l] : epsSub[l]));
+        int g = k * G + c * P.nSec + s;
+        sv[g] = DevSolve{c, s};
+        eps[g] = C(k == 0 ? epsGap[c] : (k == 1 ? epsInc[c] : epsSub[c]));
       }
   DBuf dpg, dsv, deps, dW, dV, dlam, dX, dLU, dA, dB, dAi, dT, dip;
   dpg.upload(pg);
@@ -649,7 +657,10 @@ static void build_regions(orf_plan& P, const std::vector<zc>& epsInc, const std:
 static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   if (!pr) fail(ORF_INVALID_ARG, "problem is NULL");
   P.ctx = ctx;
-  P.st = ctx->stream;
+  // one stream per plan (default priority) so that plans on different host
+  // threads overlap on the device
+  HIPCHECK(hipStreamCreateWithFlags(&P.st, hipStreamNonBlocking));
+  P.ownStream = true;
   P.Lx = pr->Lx;
   P.Ly = pr->Ly;
   if (!(P.Lx > 0) || !std::isfinite(P.Lx) || !(P.Ly > 0)) fail(ORF_INVALID_ARG, "period must be positive (Lx finite)");
@@ -690,36 +701,86 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
     P.nIdx[p] = p / Wx - P.Ny;
   }
   P.i0 = P.Nx + Wx * P.Ny;
-  // k-vectors per wavelength (sign convention: +m lambda0 / Lx)
+  // period of every structure (rcwa2D.m:4-7 shares one; uneven batches give one per structure)
+  std::vector<double> perX(P.B, P.Lx), perY(P.B, P.Ly);
+  if (pr->structPeriod) {
+    for (int b = 0; b < P.B; ++b) {
+      perX[b] = pr->structPeriod[2 * (size_t)b];
+      perY[b] = pr->structPeriod[2 * (size_t)b + 1];
+      if (!(perX[b] > 0) || !std::isfinite(perX[b]) || !(perY[b] > 0) ||
+          std::isfinite(perY[b]) != std::isfinite(P.Ly))
+        fail(ORF_INVALID_ARG, "structPeriod must be positive, with Ly finite exactly when the problem's Ly is");
+    }
+  }
+  // pages: every (structure, wavelength) pair, or the caller's explicit list
+  if (pr->nPages > 0) {
+    if (!pr->pageStruct || !pr->pageLambda) fail(ORF_INVALID_ARG, "pageStruct and pageLambda are required");
+    P.nPages = pr->nPages;
+    for (int q = 0; q < P.nPages; ++q)
+      if (pr->pageStruct[q] < 0 || pr->pageStruct[q] >= P.B || pr->pageLambda[q] < 0 || pr->pageLambda[q] >= P.L)
+        fail(ORF_INVALID_ARG, "page index out of range");
+  } else if (pr->nPages == 0) {
+    P.nPages = P.B * P.L;
+  } else {
+    fail(ORF_INVALID_ARG, "nPages must be non-negative");
+  }
+  // contexts: distinct (wavelength, period) pairs in page order
+  P.pageB.resize(P.nPages);
+  P.pageCtx.resize(P.nPages);
+  std::map<std::tuple<int, double, double>, int> ctxOf;
+  std::vector<int> ctxLam;
+  std::vector<double> ctxLx, ctxLy;
+  for (int q = 0; q < P.nPages; ++q) {
+    const int b = pr->nPages > 0 ? pr->pageStruct[q] : q % P.B;
+    const int l = pr->nPages > 0 ? pr->pageLambda[q] : q / P.B;
+    auto key = std::make_tuple(l, perX[b], std::isfinite(perY[b]) ? perY[b] : -1.0);
+    auto it = ctxOf.find(key);
+    int c;
+    if (it == ctxOf.end()) {
+      c = (int)ctxLam.size();
+      ctxOf[key] = c;
+      ctxLam.push_back(l);
+      ctxLx.push_back(perX[b]);
+      ctxLy.push_back(perY[b]);
+    } else {
+      c = it->second;
+    }
+    P.pageB[q] = b;
+    P.pageCtx[q] = c;
+  }
+  P.nCtx = (int)ctxLam.size();
+  // k-vectors per context (sign convention: +m lambda0 / Lx)
   const double st_ = std::sin(P.theta), ct = std::cos(P.theta);
   const double khat[3] = {st_ * std::cos(P.phi), st_ * std::sin(P.phi), ct};
-  std::vector<double> kxy((size_t)2 * P.M * P.L);
-  std::vector<cplx> kz((size_t)2 * P.M * P.L);
-  std::vector<zc> eInc(P.L), eSub(P.L), eGap(P.L);
-  std::vector<double> nInc(P.L);
-  P.k0.resize(P.L);
-  for (int l = 0; l < P.L; ++l) {
+  std::vector<double> kxy((size_t)2 * P.M * P.nCtx), kzi0(P.nCtx);
+  std::vector<cplx> kz((size_t)2 * P.M * P.nCtx);
+  std::vector<zc> eInc(P.nCtx), eSub(P.nCtx), eGap(P.nCtx);
+  P.k0.resize(P.nCtx);
+  for (int c = 0; c < P.nCtx; ++c) {
+    const int l = ctxLam[c];
     const double lm = P.lambda[l];
-    P.k0[l] = 2 * M_PI / lm;
-    eInc[l] = matEps(pr->incMaterial, l);
-    eSub[l] = matEps(pr->subMaterial, l);
-    const zc ninc = std::sqrt(eInc[l]);
+    P.k0[c] = 2 * M_PI / lm;
+    eInc[c] = matEps(pr->incMaterial, l);
+    eSub[c] = matEps(pr->subMaterial, l);
+    const zc ninc = std::sqrt(eInc[c]);
     if (std::fabs(ninc.imag()) > 1e-12) fail(ORF_LOSSY_INCIDENT, "Incident medium must be lossless.");
-    nInc[l] = ninc.real();
+    const double nInc = ninc.real();
+    kzi0[c] = nInc * ct;
     double kmax = 0;
     for (int p = 0; p < P.M; ++p) {
-      const double kx = nInc[l] * khat[0] + P.mIdx[p] * lm / P.Lx;
-      const double ky = nInc[l] * khat[1] + (std::isfinite(P.Ly) ? P.nIdx[p] * lm / P.Ly : 0.0);
-      kxy[(size_t)l * 2 * P.M + p] = kx;
-      kxy[(size_t)l * 2 * P.M + P.M + p] = ky;
-      kz[(size_t)l * 2 * P.M + p] = C(kz_nonzero_h(kz_branch_h(eInc[l] - kx * kx - ky * ky)));
-      kz[(size_t)l * 2 * P.M + P.M + p] = C(kz_nonzero_h(kz_branch_h(eSub[l] - kx * kx - ky * ky)));
+      const double kx = nInc * khat[0] + P.mIdx[p] * lm / ctxLx[c];
+      const double ky = nInc * khat[1] + (std::isfinite(ctxLy[c]) ? P.nIdx[p] * lm / ctxLy[c] : 0.0);
+      kxy[(size_t)c * 2 * P.M + p] = kx;
+      kxy[(size_t)c * 2 * P.M + P.M + p] = ky;
+      kz[(size_t)c * 2 * P.M + p] = C(kz_nonzero_h(kz_branch_h(eInc[c] - kx * kx - ky * ky)));
+      kz[(size_t)c * 2 * P.M + P.M + p] = C(kz_nonzero_h(kz_branch_h(eSub[c] - kx * kx - ky * ky)));
       kmax = std::max(kmax, kx * kx + ky * ky);
     }
-    eGap[l] = std::isnan(pr->gapEps) ? zc(1.0 + kmax, 0.0) : zc(pr->gapEps, 0.0);
+    eGap[c] = std::isnan(pr->gapEps) ? zc(1.0 + kmax, 0.0) : zc(pr->gapEps, 0.0);
   }
   P.dKxy.upload(kxy);
   P.dKz.upload(kz);
+  P.dKzi0.upload(kzi0);
   // sources
   bool allSP = true;
   std::vector<zc> Ex(P.nPol), Ey(P.nPol);
@@ -760,7 +821,7 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
       const int cnt = pr->shapeCount[jb];
       if (cnt == 0 && npat == 0) continue;
       // patterned (or uniform inside a layer that is patterned elsewhere in the batch)
-      DevGrid g{(int)shapes.size(), cnt};
+      DevGrid g{(int)shapes.size(), cnt, perX[b], perY[b]};
       std::vector<int> slots{pr->background[jb]};
       for (int s = 0; s < cnt; ++s) {
         const orf_shape& sh = pr->shapes[pr->shapeStart[jb] + s];
@@ -804,7 +865,7 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
     const long long cells = (long long)P.Nsx * P.Nsy;
     P.dLabels.alloc((size_t)cells * P.nGrids);
     raster_kernel<<<dim3((unsigned)((cells + 255) / 256), P.nGrids), 256, 0, st>>>(
-        P.Nsx, P.Nsy, P.Lx, P.Ly, P.dGrids.as<DevGrid>(), P.dShapes.as<DevShape>(), P.dVerts.as<double>(),
+        P.Nsx, P.Nsy, P.dGrids.as<DevGrid>(), P.dShapes.as<DevShape>(), P.dVerts.as<double>(),
         P.dLabels.as<uint8_t>());
     KCHECK();
   }
@@ -870,7 +931,6 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
     so.sector = srcSector[i];
     so.ex = C(Ex[i]);
     so.ey = C(Ey[i]);
-    so.kzinc0 = nInc[0] * ct;
     so.srcCol = 0;
     so.srcVal = C(0);
     if (P.reduced) {
@@ -883,12 +943,6 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
     }
     P.sources[i] = so;
   }
-  // incident index may vary with lambda: kzinc0 per wavelength is folded into kz (output uses Re kz_inc / n cos)
-  for (int l = 1; l < P.L; ++l)
-    if (std::fabs(nInc[l] - nInc[0]) > 0) {
-      // different incident index per wavelength: store per source is insufficient
-      fail(ORF_INVALID_ARG, "dispersive incident medium across a multi-wavelength batch is not supported");
-    }
   P.dSources.upload(P.sources);
   P.sectorSrcCol.assign(std::max(1, P.nSec), 0);
   for (auto& so : P.sources) P.sectorSrcCol[so.sector] = so.srcCol;
@@ -912,25 +966,24 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
     KCHECK();
   }
   // per-layer page tables and slot permittivities
-  const int npages = P.B * P.L;
+  const int npages = P.nPages;
   P.layerBgEps.assign(P.nLayers, std::vector<zc>(npages));
   for (int j = 0; j < P.nLayers; ++j) {
     std::vector<DevPage> pages(npages);
     std::vector<cplx> se((size_t)npages * P.nslotMax, C(0)), si((size_t)npages * P.nslotMax, C(0));
-    for (int l = 0; l < P.L; ++l)
-      for (int b = 0; b < P.B; ++b) {
-        const int pg = b + P.B * l;
-        const size_t jb = j + (size_t)P.nLayers * b;
-        const int g = P.layerGrid[j][b];
-        pages[pg] = DevPage{b, l, g, g >= 0 ? (int)gridSlotMat[g].size() : 0};
-        P.layerBgEps[j][pg] = matEps(pr->background[jb], l);
-        if (g >= 0)
-          for (size_t s = 0; s < gridSlotMat[g].size(); ++s) {
-            zc e = matEps(gridSlotMat[g][s], l);
-            se[(size_t)pg * P.nslotMax + s] = C(e);
-            si[(size_t)pg * P.nslotMax + s] = C(1.0 / e);
-          }
-      }
+    for (int pg = 0; pg < npages; ++pg) {
+      const int b = P.pageB[pg], c = P.pageCtx[pg], l = ctxLam[c];
+      const size_t jb = j + (size_t)P.nLayers * b;
+      const int g = P.layerGrid[j][b];
+      pages[pg] = DevPage{b, c, g, g >= 0 ? (int)gridSlotMat[g].size() : 0};
+      P.layerBgEps[j][pg] = matEps(pr->background[jb], l);
+      if (g >= 0)
+        for (size_t s = 0; s < gridSlotMat[g].size(); ++s) {
+          zc e = matEps(gridSlotMat[g][s], l);
+          se[(size_t)pg * P.nslotMax + s] = C(e);
+          si[(size_t)pg * P.nslotMax + s] = C(1.0 / e);
+        }
+    }
     DBuf* dp = new DBuf();
     dp->upload(pages);
     P.dPagesByLayer.push_back(dp);
@@ -943,13 +996,12 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   }
   {
     std::vector<DevPage> base(npages);
-    for (int l = 0; l < P.L; ++l)
-      for (int b = 0; b < P.B; ++b) base[b + P.B * l] = DevPage{b, l, -1, 0};
+    for (int pg = 0; pg < npages; ++pg) base[pg] = DevPage{P.pageB[pg], P.pageCtx[pg], -1, 0};
     DBuf* dp = new DBuf();
     dp->upload(base);
     P.dPagesByLayer.push_back(dp);  // index nLayers: base pages
   }
-  // shared half-space S-matrices per (wavelength, sector)
+  // shared half-space S-matrices per (context, sector)
   build_regions(P, eInc, eSub, eGap);
   // chunk size by device memory
   size_t freeB = 0, totB = 0;
@@ -957,11 +1009,12 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   const size_t perMode = per_mode_bytes(P) + (size_t)P.nSec * per_solve_bytes(P);
   size_t budget = (size_t)(0.6 * (double)freeB);
   int cp = (int)std::max<size_t>(1, budget / std::max<size_t>(perMode, 1));
-  if (pr->batchSize > 0) cp = std::min(cp, pr->batchSize * P.L);
+  if (pr->batchSize > 0) cp = std::min(cp, pr->nPages > 0 ? pr->batchSize : pr->batchSize * P.L);
   cp = std::min(cp, 32768 / P.nSec);  // grid-dimension limit of the page-batched launches
   P.chunkPages = std::min(cp, npages);
+  P.work = make_work(P, P.chunkPages);
   // results
-  const size_t nout = (size_t)P.B * P.L * P.nPol;
+  const size_t nout = (size_t)P.nPages * P.nPol;
   P.dR.alloc(sizeof(double) * P.M * nout);
   P.dT.alloc(sizeof(double) * P.M * nout);
   P.dRtot.alloc(sizeof(double) * nout);
@@ -974,9 +1027,9 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
 // ---------------------------------------------------------------- execution
 static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
   hipStream_t st = P.st;
-  const int npages = P.B * P.L;
+  const int npages = P.nPages;
   const int Cmax = P.chunkPages;
-  Work w = make_work(P, Cmax);
+  Work& w = P.work;
   if (P.timer.on) P.timer.reset();
   const int nChunks = (npages + Cmax - 1) / Cmax;
   const int M = P.M, nf = P.nf, nr = P.nr, nSec = P.nSec;
@@ -991,13 +1044,12 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
     for (int m = 0; m < Cp; ++m)
       for (int s = 0; s < nSec; ++s) {
         sv[m * nSec + s] = DevSolve{m, s};
-        const int l = (p0 + m) / P.B;
-        grp[m * nSec + s] = l * nSec + s;
+        grp[m * nSec + s] = P.pageCtx[p0 + m] * nSec + s;
       }
     HIPCHECK(hipMemcpyAsync(w.solves, sv.data(), sizeof(DevSolve) * Cs, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(w.groupOf, grp.data(), sizeof(int) * Cs, hipMemcpyHostToDevice, st));
-    const long long G = (long long)P.L * nSec;
-    // S_acc = S_ref of each page's (wavelength, sector)
+    const long long G = (long long)P.nCtx * nSec;
+    // S_acc = S_ref of each page's (context, sector)
     for (int k = 0; k < 4; ++k)
       gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dSref.as<cplx>() + k * n2 * G, n2, w.Sa[k], n2);
     KCHECK();
@@ -1103,11 +1155,11 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         KCHECK();
         HIPCHECK(hipStreamSynchronize(st));
       }
-      // layer S-matrix against the gap (W0 = I, V0 per wavelength and sector)
+      // layer S-matrix against the gap (W0 = I, V0 per context and sector)
       std::vector<double> kL(Cs);
       for (int sp = 0; sp < Cs; ++sp) {
         const int pg = p0 + sp / nSec;
-        kL[sp] = P.k0[pg / P.B] * P.thick[(size_t)j * P.B + pg % P.B];
+        kL[sp] = P.k0[P.pageCtx[pg]] * P.thick[(size_t)j * P.B + P.pageB[pg]];
       }
       HIPCHECK(hipMemcpyAsync(w.kL, kL.data(), sizeof(double) * Cs, hipMemcpyHostToDevice, st));
       gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dV0.as<cplx>(), n2, w.X2, n2);
@@ -1144,8 +1196,9 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
     {
       STAGE("output");
       output_kernel<<<dim3(Cs, P.nPol), 256, 0, st>>>(
-          M, nr, P.reduced, P.B, P.L, w.solves, basePages, P.dSectors.as<DevSector>(), P.dSources.as<DevSource>(),
-          P.dKxy.as<double>(), (const double*)P.dKz.p, Rsel, Tsel, (long long)nr * ncol, P.dR.as<double>(),
+          M, nr, P.reduced, p0, npages, w.solves, basePages, P.dSectors.as<DevSector>(), P.dSources.as<DevSource>(),
+          P.dKxy.as<double>(), (const double*)P.dKz.p, P.dKzi0.as<double>(), Rsel, Tsel, (long long)nr * ncol,
+          P.dR.as<double>(),
           P.dT.as<double>(), P.dRtot.as<double>(), P.dTtot.as<double>(), P.dr.as<cplx>(), P.dt.as<cplx>(), P.i0);
       KCHECK();
     }
@@ -1161,7 +1214,7 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
 static void plan_fetch(orf_plan& P, orf_result* out) {
   if (!P.haveResult) fail(ORF_INVALID_ARG, "plan has not been executed");
   if (!out) fail(ORF_INVALID_ARG, "result is NULL");
-  const size_t nout = (size_t)P.B * P.L * P.nPol;
+  const size_t nout = (size_t)P.nPages * P.nPol;
   if (out->m) for (int p = 0; p < P.M; ++p) out->m[p] = P.mIdx[p];
   if (out->n) for (int p = 0; p < P.M; ++p) out->n[p] = P.nIdx[p];
   if (out->R) HIPCHECK(hipMemcpy(out->R, P.dR.p, sizeof(double) * P.M * nout, hipMemcpyDeviceToHost));
@@ -1268,7 +1321,7 @@ int orf_plan_info(const orf_plan* plan, int32_t* solved_size, int32_t* symmetric
   if (!plan) return ORF_INVALID_ARG;
   if (solved_size) *solved_size = plan->nr;
   if (symmetric) *symmetric = plan->reduced;
-  if (n_solve_pages) *n_solve_pages = plan->B * plan->L * plan->nSec;
+  if (n_solve_pages) *n_solve_pages = plan->nPages * plan->nSec;
   return ORF_OK;
 }
 

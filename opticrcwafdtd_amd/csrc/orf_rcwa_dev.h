@@ -15,11 +15,13 @@ struct DevShape {
 
 struct DevGrid {
   int shape0, nshape;
+  double Lx, Ly;  // period of the structure this grid belongs to
 };
 
-// One (structure, wavelength) page for a given layer.
+// One (structure, context) page for a given layer; a context is one
+// (wavelength, period) pair with its own k-vectors and half-space S-matrices.
 struct DevPage {
-  int b, lam;
+  int b, lam;   // lam: context index
   int grid;    // label grid of this layer (-1: homogeneous)
   int nslot;
 };
@@ -46,5 +48,4 @@ struct DevSource {
   cplx ex, ey;      // transverse incident E at the zero order
   int srcCol;       // reduced basis: the one nonzero of UE' c
   cplx srcVal;
-  double kzinc0;    // n_inc cos(theta)
 };

@@ -39,16 +39,17 @@ __device__ bool inpolygon_rn(double px, double py, const double* vx, const doubl
   return inside || on;
 }
 
-__global__ void raster_kernel(int Nsx, int Nsy, double Lx, double Ly, const DevGrid* __restrict__ grids,
+__global__ void raster_kernel(int Nsx, int Nsy, const DevGrid* __restrict__ grids,
                               const DevShape* __restrict__ shapes, const double* __restrict__ verts,
                               uint8_t* __restrict__ labels) {
   const int g = blockIdx.y;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= Nsx * Nsy) return;
   const int i = e % Nsx, j = e / Nsx;
+  const DevGrid G = grids[g];
+  const double Lx = G.Lx, Ly = G.Ly;
   const double x = isfinite(Lx) ? __dadd_rn(-Lx / 2, __dmul_rn(Lx / Nsx, i + 0.5)) : 0.0;
   const double y = isfinite(Ly) ? __dadd_rn(-Ly / 2, __dmul_rn(Ly / Nsy, j + 0.5)) : 0.0;
-  const DevGrid G = grids[g];
   int lab = 0;
   for (int s = 0; s < G.nshape; ++s) {
     const DevShape S = shapes[G.shape0 + s];
@@ -464,10 +465,11 @@ __global__ void gather_pages_kernel(long long elems, const int* __restrict__ gro
 // One block per (solve page, source).  r_xy = S11 c, t_xy = S21 c (full basis
 // or UE (S c_r) in a parity sector); z from transversality; R, T per order;
 // Rtot, Ttot.  Out index of (b, l, pol): b + B (l + L pol).
-__global__ void output_kernel(int M, int nr, int reduced, int B, int L, const DevSolve* __restrict__ solves,
+__global__ void output_kernel(int M, int nr, int reduced, int page0, int npages, const DevSolve* __restrict__ solves,
                               const DevPage* __restrict__ pages, const DevSector* __restrict__ sectors,
                               const DevSource* __restrict__ src, const double* __restrict__ kxy,
-                              const double* __restrict__ kzIncSub, const cplx* __restrict__ S11,
+                              const double* __restrict__ kzIncSub, const double* __restrict__ kzinc0s,
+                              const cplx* __restrict__ S11,
                               const cplx* __restrict__ S21, long long sS, double* __restrict__ R,
                               double* __restrict__ T, double* __restrict__ Rtot, double* __restrict__ Ttot,
                               cplx* __restrict__ rOut, cplx* __restrict__ tOut, int i0) {
@@ -482,10 +484,10 @@ __global__ void output_kernel(int M, int nr, int reduced, int B, int L, const De
   const double* ky = kx + M;
   const cplx* kzi = (const cplx*)kzIncSub + (size_t)P.lam * 2 * M;
   const cplx* kzs = kzi + M;
-  const double kzi0 = so.kzinc0;
+  const double kzi0 = kzinc0s[P.lam];
   const cplx* s11 = S11 + (size_t)sp * sS;
   const cplx* s21 = S21 + (size_t)sp * sS;
-  const long long outIdx = (long long)P.b + (long long)B * ((long long)P.lam + (long long)L * so.pol);
+  const long long outIdx = (long long)(page0 + S.mode) + (long long)npages * so.pol;
   double rs = 0.0, ts = 0.0;
   for (int p = threadIdx.x; p < M; p += blockDim.x) {
     cplx rxy[2], txy[2];

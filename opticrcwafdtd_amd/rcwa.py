@@ -9,7 +9,9 @@ through liborf_rcwa.so; this module only flattens the device descriptions.
 Output arrays follow MATLAB's order: ``R``, ``T`` are M x B, ``Rtot``,
 ``Ttot`` are B, ``r``, ``t`` are 3 x M x B.  Two build extensions append
 trailing dimensions: a vector ``lambda0`` (wavelength axis) and a list of
-polarisations (polarisation axis).
+polarisations (polarisation axis).  A third, ``Pages``, replaces the
+(structure, wavelength) product by an explicit list of pairs and lets the
+structures differ in period; the outputs are then M x nPages.
 """
 from __future__ import annotations
 
@@ -47,7 +49,7 @@ class Problem:
 
     def __init__(self, devs, lambda0, theta, phi, pol, Nx=5, Ny=None, Nsample=0,
                  Factorization="li", GapEps=float("nan"), Symmetry="auto", BatchSize=0,
-                 Precision="fp64"):
+                 Precision="fp64", Pages=None):
         if isinstance(devs, dict):
             devs = [devs]
         devs = list(devs)
@@ -55,9 +57,13 @@ class Problem:
             raise OrfError("orf:invalidArgument", "devs is empty")
         self.devs = devs
         Lx, Ly = devs[0]["period"]
+        self.pages = None if Pages is None else np.asarray(Pages, np.int32).reshape(-1, 2)
+        periods = np.array([list(d["period"]) for d in devs], float)
         for d in devs:
-            if list(d["period"]) != [Lx, Ly]:
+            if list(d["period"]) != [Lx, Ly] and self.pages is None:
                 raise OrfError("orf:invalidArgument", "structures must share the period")
+            if math.isfinite(d["period"][1]) != math.isfinite(Ly):
+                raise OrfError("orf:invalidArgument", "structures must all be 1D or all 2D")
         if Ny is None:
             Ny = Nx
         if not math.isfinite(Ly):
@@ -162,6 +168,19 @@ class Problem:
         pr.gapEps = float(GapEps)
         pr.precision = 0 if Precision == "fp64" else 1
         pr.batchSize = int(BatchSize)
+        if self.pages is not None:
+            if self.pages.shape[0] == 0:
+                raise OrfError("orf:invalidArgument", "Pages is empty")
+            if (self.pages[:, 0].min() < 0 or self.pages[:, 0].max() >= len(devs)
+                    or self.pages[:, 1].min() < 0 or self.pages[:, 1].max() >= lam.size):
+                raise OrfError("orf:invalidArgument", "Pages index out of range")
+            self._period = np.ascontiguousarray(periods.reshape(-1))
+            self._pstruct = np.ascontiguousarray(self.pages[:, 0])
+            self._plam = np.ascontiguousarray(self.pages[:, 1])
+            pr.structPeriod = self._period.ctypes.data_as(P(C.c_double))
+            pr.nPages = int(self.pages.shape[0])
+            pr.pageStruct = self._pstruct.ctypes.data_as(P(C.c_int32))
+            pr.pageLambda = self._plam.ctypes.data_as(P(C.c_int32))
         self.struct = pr
         self.M = (2 * int(Nx) + 1) * (2 * int(Ny) + 1)
         self.B = len(devs)
@@ -227,6 +246,8 @@ class Plan:
     def fetch(self, amplitudes=True):
         pr = self.problem
         M, B, L, Pn = pr.M, pr.B, pr.lambdas.size, pr.npol
+        if pr.pages is not None:  # one "structure" axis of nPages, no wavelength axis
+            B, L = pr.pages.shape[0], 1
         m = np.zeros(M, np.int32)
         n = np.zeros(M, np.int32)
         R = np.zeros((Pn, L, B, M))
@@ -265,9 +286,9 @@ def _pack(pr, m, n, R, T, Rt, Tt, r, t, symmetric):
             a = np.swapaxes(a, 1, 2)          # (3, M, B, L, P)
         else:
             a = np.swapaxes(a, 0, 1) if a.ndim == 4 else a  # (M, B, L, P)
+        if not pr.multi_lambda or pr.pages is not None:
+            a = a[..., 0, :]
         if not pr.multi_pol:
-            a = a[..., 0]
-        if not pr.multi_lambda:
             a = a[..., 0]
         return a
     out = {"m": m.astype(int), "n": n.astype(int), "R": fix(R), "T": fix(T), "Rtot": fix(Rt),
@@ -278,15 +299,17 @@ def _pack(pr, m, n, R, T, Rt, Tt, r, t, symmetric):
 
 def rcwa2D(devs, lambda0, theta, phi, pol, Nx=5, Ny=None, Nsample=0, Factorization="li",
            GapEps=float("nan"), UseGPU=True, GPUEig=True, BatchSize=0, Symmetry="auto",
-           Progress: Optional[Callable] = None, Precision="fp64"):
+           Progress: Optional[Callable] = None, Precision="fp64", Pages=None):
     """orf.rcwa2D (rcwa2D.m:2-35) on the GPU.
 
     ``UseGPU`` / ``GPUEig`` are accepted for signature compatibility; every
-    stage, eig included, runs on the gfx950 device.
+    stage, eig included, runs on the gfx950 device.  ``Pages`` (build
+    extension) is a list of (structure, wavelength-index) pairs to solve
+    instead of every pair; the structures may then have different periods.
     """
     prob = Problem(devs, lambda0, theta, phi, pol, Nx=Nx, Ny=Ny, Nsample=Nsample,
                    Factorization=Factorization, GapEps=GapEps, Symmetry=Symmetry,
-                   BatchSize=BatchSize, Precision=Precision)
+                   BatchSize=BatchSize, Precision=Precision, Pages=Pages)
     plan = Plan(prob)
     try:
         plan.execute(Progress)

@@ -229,10 +229,14 @@ def default_sample_counts(devs: Sequence[dict], Nx: int, Ny: int) -> tuple:
     rectangles, and a single sample along an infinite period.
     """
     period = devs[0]["period"]
+    shared = all(list(d["period"]) == list(period) for d in devs)
     out = []
     for dim, N in ((1, Nx), (2, Ny)):
+        ns_min = max(256, 8 * (4 * N + 1))
         if not math.isfinite(period[dim - 1]):
             out.append(1)
-        else:
-            out.append(alignedSampleCount(devs, period, dim, max(256, 8 * (4 * N + 1))))
+        elif shared:
+            out.append(alignedSampleCount(devs, period, dim, ns_min))
+        else:  # structures of different periods cannot all be edge-aligned by one count
+            out.append(ns_min)
     return tuple(out)

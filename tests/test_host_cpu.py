@@ -1,0 +1,76 @@
+"""Host-side logic that needs no GPU: explicit-page problems, the uneven-batch
+grouping and the seeded mixed-cell workload."""
+import math
+
+import numpy as np
+import pytest
+
+import opticrcwafdtd_amd as orf
+from opticrcwafdtd_amd import uneven
+
+mc = orf.materialConst
+
+
+def _cell(P, r):
+    return orf.makeStructure([P, P], mc(1.45), mc(1), [orf.makeLayer(0.5, mc(1), [orf.shapeCircle(r, mc(2.0))])])
+
+
+def test_pages_flatten_into_the_problem():
+    devs = [_cell(0.4, 0.1), _cell(0.5, 0.12)]
+    pr = orf.Problem(devs, [0.9, 1.0, 1.1], 0, 0, "s", Nx=2, Nsample=(64, 64), Pages=[(1, 2), (0, 0)])
+    s = pr.struct
+    assert s.nPages == 2
+    assert [s.pageStruct[i] for i in range(2)] == [1, 0]
+    assert [s.pageLambda[i] for i in range(2)] == [2, 0]
+    assert [s.structPeriod[i] for i in range(4)] == [0.4, 0.4, 0.5, 0.5]
+
+
+def test_pages_validation():
+    devs = [_cell(0.4, 0.1), _cell(0.5, 0.12)]
+    with pytest.raises(orf.OrfError, match="share the period"):
+        orf.Problem(devs, 0.9, 0, 0, "s", Nx=2, Nsample=(64, 64))
+    with pytest.raises(orf.OrfError, match="out of range"):
+        orf.Problem(devs, 0.9, 0, 0, "s", Nx=2, Nsample=(64, 64), Pages=[(2, 0)])
+    with pytest.raises(orf.OrfError, match="out of range"):
+        orf.Problem(devs, 0.9, 0, 0, "s", Nx=2, Nsample=(64, 64), Pages=[(0, 1)])
+    one_d = orf.makeStructure([0.4, math.inf], mc(1), mc(1), [orf.makeLayer(0.1, mc(1))])
+    with pytest.raises(orf.OrfError, match="1D or all 2D"):
+        orf.Problem([devs[0], one_d], 0.9, 0, 0, "s", Nx=2, Nsample=(64, 64), Pages=[(0, 0)])
+
+
+def test_mixed_period_default_sampling_is_not_edge_aligned():
+    devs = [_cell(0.4, 0.1), _cell(0.5, 0.12)]
+    pr = orf.Problem(devs, 0.9, 0, 0, "s", Nx=3, Pages=[(0, 0), (1, 0)])
+    assert pr.Nsample == (256, 256)
+
+
+def test_random_cells_are_seeded_and_cover_the_ranges():
+    a = uneven.random_cells(400, seed=11)
+    b = uneven.random_cells(400, seed=11)
+    assert [c["N"] for c in a] == [c["N"] for c in b]
+    assert [c["lambda0"] for c in a] == [c["lambda0"] for c in b]
+    Ns = {c["N"] for c in a}
+    assert Ns == {3, 4, 5, 6, 7}
+    nl = {len(c["dev"]["layers"]) for c in a}
+    assert nl == {1, 2, 3, 8}
+    P = np.array([c["dev"]["period"][0] for c in a])
+    L = np.array([c["lambda0"] for c in a])
+    assert P.min() >= 0.3 and P.max() <= 0.6 and L.min() >= 0.45 and L.max() <= 1.6
+    kinds = {len(c["dev"]["layers"][0]["shapes"]) for c in a}
+    assert 2 in kinds and 1 in kinds  # crosses (two rectangles) and single shapes
+
+
+def test_grouping_by_size_class():
+    cells = uneven.random_cells(200, seed=3)
+    g = uneven.group_cells(cells)
+    assert sum(len(v) for v in g.values()) == 200
+    for (N, nl, _inc, _sub), idx in g.items():
+        for i in idx:
+            assert cells[i]["N"] == N and len(cells[i]["dev"]["layers"]) == nl
+    ub = uneven.UnevenBatch(cells, Nsample=(64, 64))
+    assert ub.solves == 400
+    work = [ub._work(c["key"], len(c["cells"])) for c in ub.classes]
+    assert work == sorted(work, reverse=True)
+    for c in ub.classes:
+        pr = c["problem"]
+        assert pr.struct.nPages == len(c["cells"])
