@@ -147,6 +147,16 @@ int orf_plan_stage_times(const orf_plan* plan, int32_t max_stages, const char** 
                          double* total_ms, int32_t* launches, int32_t* n_stages);
 void orf_plan_enable_timing(orf_plan* plan, int32_t on);
 
+/* ------------------------------------------------------------------ memory (build extension)
+ * Admission by device footprint for request loops.  orf_plan_memory: bytes a
+ * created plan holds.  orf_plan_estimate: an upper bound before creation,
+ * for the full basis (reduced = 0) or the parity-sector basis (reduced = 1).
+ * Freed device blocks are cached by the library for reuse (hipFree would
+ * wait for the whole device); orf_release_cached_memory returns them. */
+int orf_plan_memory(const orf_plan* plan, int64_t* bytes);
+int orf_plan_estimate(const orf_problem* p, int32_t reduced, int64_t* bytes);
+void orf_release_cached_memory(void);
+
 #ifdef __cplusplus
 }
 #endif

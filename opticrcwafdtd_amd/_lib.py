@@ -65,7 +65,8 @@ PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_double, C.c_char_p, C.c_void_p)
 EXPORTS = [
     "orf_init", "orf_release", "orf_last_error", "orf_harmonic_count", "orf_rcwa2d",
     "orf_plan_create", "orf_plan_execute", "orf_plan_fetch", "orf_plan_destroy", "orf_plan_info",
-    "orf_plan_stage_times", "orf_plan_enable_timing",
+    "orf_plan_stage_times", "orf_plan_enable_timing", "orf_plan_memory", "orf_plan_estimate",
+    "orf_release_cached_memory",
     "orf_pagemtimes", "orf_pagemldivide", "orf_pagemrdivide", "orf_pageeig",
 ]
 
@@ -108,6 +109,12 @@ def load_library():
     lib.orf_plan_stage_times.restype = C.c_int
     lib.orf_plan_enable_timing.argtypes = [C.c_void_p, C.c_int32]
     lib.orf_plan_enable_timing.restype = None
+    lib.orf_plan_memory.argtypes = [C.c_void_p, P(C.c_int64)]
+    lib.orf_plan_memory.restype = C.c_int
+    lib.orf_plan_estimate.argtypes = [P(orf_problem), C.c_int32, P(C.c_int64)]
+    lib.orf_plan_estimate.restype = C.c_int
+    lib.orf_release_cached_memory.argtypes = []
+    lib.orf_release_cached_memory.restype = None
     lib.orf_pagemtimes.argtypes = [C.c_void_p, C.c_char, C.c_char, C.c_int32, C.c_int32, C.c_int32,
                                    P(C.c_double), C.c_int64, P(C.c_double), C.c_int64, P(C.c_double),
                                    C.c_int32]

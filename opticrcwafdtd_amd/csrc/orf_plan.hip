@@ -17,6 +17,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <unordered_map>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -132,17 +134,78 @@ struct StageTimer {
 };
 
 // ====================================================================== device memory
+// Caching allocator.  hipFree waits for the whole device, so a serving loop
+// that creates and destroys plans while other plans run on other streams
+// would serialise on it; freed blocks are kept here instead and handed to the
+// next request of a similar size (best fit, at most 25 % larger).  Every
+// owner synchronises its stream before it returns a block.
+struct DevicePool {
+  std::mutex mu;
+  std::multimap<size_t, void*> freeBlocks;
+  std::unordered_map<void*, size_t> sizeOf;
+  size_t cachedBytes = 0, liveBytes = 0;
+  static size_t round(size_t b) {
+    const size_t g = b >= (size_t(1) << 20) ? (size_t(2) << 20) : 512;
+    return (b + g - 1) / g * g;
+  }
+  void* get(size_t bytes) {
+    const size_t want = round(bytes);
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      auto it = freeBlocks.lower_bound(want);
+      if (it != freeBlocks.end() && it->first <= want + want / 4) {
+        void* p = it->second;
+        cachedBytes -= it->first;
+        liveBytes += it->first;
+        freeBlocks.erase(it);
+        return p;
+      }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      trim();
+      e = hipMalloc(&p, want);
+    }
+    if (e == hipErrorOutOfMemory) fail(ORF_OOM, "out of device memory (" + std::to_string(want >> 20) + " MiB)");
+    if (e != hipSuccess) fail(ORF_DEVICE_ERROR, std::string(hipGetErrorString(e)) + " in hipMalloc");
+    std::lock_guard<std::mutex> lk(mu);
+    sizeOf[p] = want;
+    liveBytes += want;
+    return p;
+  }
+  void put(void* p) {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = sizeOf.find(p);
+    if (it == sizeOf.end()) return;
+    freeBlocks.emplace(it->second, p);
+    cachedBytes += it->second;
+    liveBytes -= it->second;
+  }
+  void trim() {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& kv : freeBlocks) {
+      (void)hipFree(kv.second);
+      sizeOf.erase(kv.second);
+    }
+    freeBlocks.clear();
+    cachedBytes = 0;
+  }
+};
+static DevicePool g_pool;
+
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
   DBuf() {}
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
-  ~DBuf() { if (p) (void)hipFree(p); }
+  ~DBuf() { if (p) g_pool.put(p); }
   void alloc(size_t b) {
-    if (p) { (void)hipFree(p); p = nullptr; }
+    if (p) { g_pool.put(p); p = nullptr; }
     bytes = b;
-    if (b) HIPCHECK(hipMalloc(&p, b));
+    if (b) p = g_pool.get(b);
   }
   template <class T> T* as() const { return (T*)p; }
   template <class T> void upload(const std::vector<T>& v) {
@@ -322,6 +385,7 @@ struct orf_plan {
   bool ownStream = false;
   StageTimer timer;
   ~orf_plan() {
+    if (st) (void)hipStreamSynchronize(st);  // no block returns to the pool while in use
     for (auto* b : dPagesByLayer) delete b;
     for (auto* b : dSlotEps) delete b;
     for (auto* b : dSlotInv) delete b;
@@ -428,6 +492,18 @@ static Work make_work(orf_plan& P, int Cp) {
   w.colIdx = (int*)get(sizeof(int) * 2 * Cs);
   w.epsSolve = (cplx*)get(cs * Cs);
   return w;
+}
+
+// Device bytes held by a plan (resident tables, results and workspace).
+static size_t plan_bytes(const orf_plan& P) {
+  size_t b = 0;
+  for (const DBuf* d : {&P.dShapes, &P.dVerts, &P.dGrids, &P.dLabels, &P.dFx, &P.dFy, &P.dChiX, &P.dChiY, &P.dChi2,
+                        &P.dKxy, &P.dKz, &P.dKzi0, &P.dSectorData, &P.dSectors, &P.dSources, &P.dV0, &P.dSref,
+                        &P.dStrn, &P.dR, &P.dT, &P.dRtot, &P.dTtot, &P.dr, &P.dt})
+    b += d->bytes;
+  for (auto* v : {&P.dPagesByLayer, &P.dSlotEps, &P.dSlotInv, &P.secArrays, &P.ws})
+    for (const DBuf* d : *v) b += d->bytes;
+  return b;
 }
 
 // ---------------------------------------------------------------- dense S-matrix algebra
@@ -881,7 +957,8 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
           P.Nsx, P.Nsy, P.dLabels.as<uint8_t>(), flags.as<int>());
       KCHECK();
       std::vector<int> h(P.nGrids);
-      HIPCHECK(hipMemcpy(h.data(), flags.p, sizeof(int) * P.nGrids, hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpyAsync(h.data(), flags.p, sizeof(int) * P.nGrids, hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipStreamSynchronize(st));
       for (int v : h) sym &= v;
     }
     P.reduced = sym;
@@ -1007,7 +1084,7 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   size_t freeB = 0, totB = 0;
   HIPCHECK(hipMemGetInfo(&freeB, &totB));
   const size_t perMode = per_mode_bytes(P) + (size_t)P.nSec * per_solve_bytes(P);
-  size_t budget = (size_t)(0.6 * (double)freeB);
+  size_t budget = (size_t)(0.6 * (double)(freeB + g_pool.cachedBytes));
   int cp = (int)std::max<size_t>(1, budget / std::max<size_t>(perMode, 1));
   if (pr->batchSize > 0) cp = std::min(cp, pr->nPages > 0 ? pr->batchSize : pr->batchSize * P.L);
   cp = std::min(cp, 32768 / P.nSec);  // grid-dimension limit of the page-batched launches
@@ -1324,6 +1401,46 @@ int orf_plan_info(const orf_plan* plan, int32_t* solved_size, int32_t* symmetric
   if (n_solve_pages) *n_solve_pages = plan->nPages * plan->nSec;
   return ORF_OK;
 }
+
+int orf_plan_memory(const orf_plan* plan, int64_t* bytes) {
+  if (!plan || !bytes) return ORF_INVALID_ARG;
+  *bytes = (int64_t)plan_bytes(*plan);
+  return ORF_OK;
+}
+
+int orf_plan_estimate(const orf_problem* p, int32_t reduced, int64_t* bytes) {
+  ORF_TRY({
+    if (!p || !bytes) fail(ORF_INVALID_ARG, "problem or bytes is NULL");
+    orf_plan P;
+    P.Nx = p->Nx;
+    P.Ny = std::isfinite(p->Ly) ? p->Ny : 0;
+    P.Nsx = p->Nsx;
+    P.Nsy = std::isfinite(p->Ly) ? p->Nsy : 1;
+    P.M = (2 * P.Nx + 1) * (2 * P.Ny + 1);
+    P.nf = 2 * P.M;
+    P.factorization = p->factorization;
+    if (reduced) {
+      P.nr = (P.Nx + 1) * (P.Ny + 1) + P.Nx * P.Ny;
+      bool s = false, pp = false;
+      for (int i = 0; i < p->nPol; ++i) { s |= p->polKind[i] == ORF_POL_S; pp |= p->polKind[i] == ORF_POL_P; }
+      P.nSec = (int)s + (int)pp;
+      if (P.nSec == 0) P.nSec = 1;
+    } else {
+      P.nr = P.nf;
+      P.nSec = 1;
+    }
+    const size_t pages = p->nPages > 0 ? (size_t)p->nPages : (size_t)p->nStruct * p->nLambda;
+    const size_t n2 = (size_t)P.nr * P.nr;
+    size_t b = pages * (per_mode_bytes(P) + (size_t)P.nSec * per_solve_bytes(P));
+    b += pages * P.nSec * 9 * n2 * sizeof(cplx);          // half-space S-matrices and gap modes (upper bound)
+    b += (size_t)p->nStruct * p->nLayers * ((size_t)P.Nsx * P.Nsy +
+         sizeof(cplx) * ORF_MAX_SLOTS * ((size_t)(4 * P.Nx + 1) * (P.Nsy + 4 * P.Ny + 1) + (size_t)(4 * P.Ny + 1) * P.Nsx));
+    b += pages * p->nPol * (size_t)P.M * (2 * sizeof(double) + 6 * sizeof(cplx));
+    *bytes = (int64_t)b;
+  })
+}
+
+void orf_release_cached_memory(void) { g_pool.trim(); }
 
 void orf_plan_enable_timing(orf_plan* plan, int32_t on) {
   if (plan) plan->timer.on = on != 0;

@@ -188,6 +188,13 @@ class Problem:
         self.Nsample = (int(Nsx), int(Nsy))
 
 
+def estimate_memory(problem: Problem, reduced: bool) -> int:
+    """Upper bound of the device bytes a plan of ``problem`` will hold."""
+    b = C.c_int64()
+    _raise(_lib.load_library().orf_plan_estimate(C.byref(problem.struct), 1 if reduced else 0, C.byref(b)))
+    return b.value
+
+
 def _raise(rc: int, pending_exc=None):
     if rc == _lib.ORF_OK:
         return
@@ -225,6 +232,12 @@ class Plan:
         ns, sym, npg = C.c_int32(), C.c_int32(), C.c_int32()
         lib.orf_plan_info(h, C.byref(ns), C.byref(sym), C.byref(npg))
         self.solved_size, self.symmetric, self.solve_pages = ns.value, bool(sym.value), npg.value
+
+    def memory(self) -> int:
+        """Device bytes this plan holds (tables, results, workspace)."""
+        b = C.c_int64()
+        _raise(self._lib.orf_plan_memory(self.handle, C.byref(b)))
+        return b.value
 
     def execute(self, Progress=None):
         fn, state = _make_progress(Progress)

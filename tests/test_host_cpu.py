@@ -74,3 +74,18 @@ def test_grouping_by_size_class():
     for c in ub.classes:
         pr = c["problem"]
         assert pr.struct.nPages == len(c["cells"])
+
+
+def test_request_mix_is_seeded_with_the_stated_shares():
+    from opticrcwafdtd_amd import serving
+    a = serving.random_requests(2000, seed=1)
+    b = serving.random_requests(2000, seed=1)
+    assert a == b
+    kinds = [k for k, _ in a]
+    share = {k: kinds.count(k) / len(kinds) for k in ("lookup", "ladder", "sweep")}
+    assert abs(share["lookup"] - 0.7) < 0.04 and abs(share["ladder"] - 0.2) < 0.03 and abs(share["sweep"] - 0.1) < 0.03
+    sw = next(x for k, x in a if k == "sweep")
+    assert len(sw["angles"]) == 16 and sw["slices"] == 32
+    assert len(serving.sidewall_cell(85.0, slices=32)["layers"]) == 32
+    lad = next(x for k, x in a if k == "ladder")
+    assert lad["Ns"] == (3, 5, 7, 9)
