@@ -5,7 +5,13 @@ the SiO2 side), square period 0.4 um, 256 radii 40-180 nm x 8 wavelengths
 eps grid 256 x 256, TE and TM: 2048 (cell, wavelength) pages = 4096 solves.
 
 One step = one execution of the resident plan over the whole batch (inputs
-already in HBM).  Prints ONE JSON line.
+already in HBM).  Prints ONE JSON line.  Besides the headline (FP64, full
+basis) the line carries, unless ``--quick``: the same batch in the parity-
+sector basis (the reference's default Symmetry="auto"), in mixed precision
+with its per-efficiency error against FP64, the 4096-cell uneven batch, and
+the request loop's p50/p99 latency under Poisson arrivals at three loads.
+``--copy-probe`` only runs a device copy of known size (the calibration of
+the rocprofv3 FETCH_SIZE / WRITE_SIZE counters).
 """
 import argparse
 import json
@@ -90,17 +96,19 @@ def solve_flops(n, M):
     return 8.0 * M ** 3 + (8 + 100 + 8 + 90 + 2.0 / 3 + 2 * (85 + 1.0 / 3)) * n ** 3
 
 
-def run(args):
+def run(args, sym, precision="fp64", steps=None, warmup=None):
     import opticrcwafdtd_amd as orf
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     devs = make_devs(orf, RADII)
-    sym = args.symmetry
-    prob = orf.Problem(devs, list(LAMBDAS), 0.0, 0.0, ["s", "p"], Nx=N_HARM, Nsample=(256, 256), Symmetry=sym)
+    prob = orf.Problem(devs, list(LAMBDAS), 0.0, 0.0, ["s", "p"], Nx=N_HARM, Nsample=(256, 256), Symmetry=sym,
+                       Precision=precision)
     plan = orf.Plan(prob)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         plan.execute()
     plan.enable_timing(True)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         plan.execute()
     t1 = time.perf_counter()
     plan.enable_timing(False)
@@ -108,8 +116,87 @@ def run(args):
     solves = len(RADII) * len(LAMBDAS) * 2
     out = plan.fetch(amplitudes=False)
     plan.close()
-    return {"plan": plan, "elapsed": t1 - t0, "stages": stages, "solves": solves, "n": plan.solved_size,
-            "symmetric": plan.symmetric, "pages": plan.solve_pages, "Ttot": out["Ttot"]}
+    return {"elapsed": t1 - t0, "steps": steps, "stages": stages, "solves": solves, "n": plan.solved_size,
+            "symmetric": plan.symmetric, "pages": plan.solve_pages, "R": out["R"], "T": out["T"]}
+
+
+def extras(args, full):
+    """The other forms of the workload and the other configurations."""
+    from opticrcwafdtd_amd import serving, uneven
+    ex = {}
+    a = run(args, "auto", steps=max(args.steps, 3), warmup=1)
+    ex["symmetric_basis"] = {"solves_per_s": round(a["solves"] * a["steps"] / a["elapsed"], 1),
+                             "ms_per_step": round(1e3 * a["elapsed"] / a["steps"], 3), "solved_matrix": a["n"],
+                             "max_abs_dT_vs_full": float(np.max(np.abs(a["T"] - full["T"])))}
+    m = run(args, "none", precision="mixed", steps=args.steps, warmup=1)
+    dT, dR = np.abs(m["T"] - full["T"]), np.abs(m["R"] - full["R"])
+    ex["mixed_precision"] = {
+        "solves_per_s": round(m["solves"] * m["steps"] / m["elapsed"], 1),
+        "ms_per_step": round(1e3 * m["elapsed"] / m["steps"], 3),
+        "speedup_vs_fp64": round((full["elapsed"] / full["steps"]) / (m["elapsed"] / m["steps"]), 4),
+        "max_abs_dT": float(dT.max()), "mean_abs_dT": float(dT.mean()),
+        "max_abs_dR": float(dR.max()), "mean_abs_dR": float(dR.mean()),
+        "within_1e-6": bool(max(dT.max(), dR.max()) <= 1e-6),
+        "what": "FP32-MFMA convolution-matrix and per-layer star-product GEMMs; eigensolve, propagators, "
+                "every LU/solve and the last star product FP64"}
+    cells = uneven.random_cells(4096, seed=20261203)
+    ub = uneven.UnevenBatch(cells, pol=["s", "p"], Nsample=(256, 256), Symmetry="auto")
+    t0 = time.perf_counter()
+    ub.create()
+    t1 = time.perf_counter()
+    ub.execute()
+    t2 = time.perf_counter()
+    ub.execute()
+    t3 = time.perf_counter()
+    res = ub.results(amplitudes=False)
+    t4 = time.perf_counter()
+    ub.close()
+    ex["uneven_4096"] = {"cells": len(cells), "solves": ub.solves, "size_classes": len(ub.classes), "seed": 20261203,
+                         "solves_per_s_execute": round(ub.solves / (t3 - t2), 1),
+                         "solves_per_s_end_to_end": round(ub.solves / ((t1 - t0) + (t3 - t2) + (t4 - t3)), 1),
+                         "create_s": round(t1 - t0, 3), "execute_s": round(t3 - t2, 3),
+                         "max_energy": float(max(float(np.max(r["Rtot"] + r["Ttot"])) for r in res))}
+    with serving.Server() as srv:
+        serving.run_load(srv, serving.random_requests(40, seed=99), None)  # warm-up
+        sat = serving.run_load(srv, serving.random_requests(args.serve_requests, seed=1), None)
+        loads = []
+        for f, sd in ((0.5, 2), (0.8, 3), (0.95, 4)):
+            r = serving.run_load(srv, serving.random_requests(args.serve_requests, seed=sd),
+                                 f * sat["req_per_s"], seed=sd)
+            loads.append({"load": f, "offered_req_per_s": round(r["offered_req_per_s"], 2),
+                          "achieved_req_per_s": round(r["req_per_s"], 2), "solves_per_s": round(r["solves_per_s"], 1),
+                          "p50_ms": round(r["p50_ms"], 2), "p99_ms": round(r["p99_ms"], 2),
+                          "by_kind": r["by_kind"]})
+        ex["serving"] = {"mix": "70% lookups +-5, 20% ladders N=3,5,7,9, 10% sidewall sweeps 32 slices x 16 angles; "
+                                "TE+TM; Symmetry=auto; seeds 1-4",
+                         "saturated": {"req_per_s": round(sat["req_per_s"], 2),
+                                       "solves_per_s": round(sat["solves_per_s"], 1),
+                                       "p50_ms": round(sat["p50_ms"], 2), "p99_ms": round(sat["p99_ms"], 2)},
+                         "poisson": loads, "batches": srv.stats["batches"],
+                         "max_inflight_batches": srv.stats["max_inflight"]}
+    return ex
+
+
+def measured_traffic(stage, symmetry):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a
+    copy of known size), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "phase_library_pm5_full_schur_qr_traffic.json")
+    if symmetry != "none" or not os.path.exists(path):
+        return None
+    d = json.load(open(path))
+    return d["bytes_per_launch"] if d.get("stage") == stage else None
+
+
+def copy_probe():
+    """One device-to-device copy of 1 GiB, four times (counter calibration)."""
+    import torch
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(4):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": x.numel() * 8, "launches": 4}))
 
 
 def main():
@@ -119,15 +206,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--symmetry", default="none", choices=["none", "auto"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every unit cell is an independent solve, so N GPUs would each run "
               "their own share of the batch (see DESIGN.md); multi-GPU runs are not part of this build.",
               file=sys.stderr)
         sys.exit(2)
     cpu = None if args.no_cpu else cpu_baseline(args.symmetry, args.cpu_workers)
-    r = run(args)
+    r = run(args, args.symmetry)
     ms = 1e3 * r["elapsed"] / args.steps
     value = r["solves"] * args.steps / r["elapsed"]
     # dominant kernel over the timed region (HIP events on the plan's stream)
@@ -151,14 +244,16 @@ def main():
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 4) if achieved else None,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 5) if achieved else None,
-                     "traffic": None, "avg_launch_ms": round(tot_ms / launches, 3),
+                     "traffic": measured_traffic(dom, args.symmetry), "avg_launch_ms": round(tot_ms / launches, 3),
                      "flops_per_launch": flops},
         "whole_solve": {"canonical_flops_per_page": solve_flops(n, M if args.symmetry == "none" else 0),
                         "achieved_tflops": round(solve_flops(n, M if args.symmetry == "none" else 0) *
                                                  pages / (ms / 1e3) / 1e12, 4)},
-        "stages_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in r["stages"].items()},
+        "stages_ms_last_step": {k: round(v[0], 3) for k, v in r["stages"].items()},
         "cpu_baseline": cpu,
     }
+    if not args.quick:
+        line["extras"] = extras(args, r)
     print(json.dumps(line), flush=True)
 
 

@@ -23,6 +23,12 @@ extern "C" {
 int orf_pagemtimes(orf_ctx* ctx, char opA, char opB, int32_t m, int32_t n, int32_t k,
                    const double* A, int64_t strideA, const double* B, int64_t strideB, double* C,
                    int32_t batch);
+/* The same product with FP32 MFMA products and FP32 accumulation (build
+ * extension, the mixed-precision form's bulk GEMM): operands are rounded to
+ * complex64, results returned as complex128.  No counterpart in the reference. */
+int orf_pagemtimes_f32(orf_ctx* ctx, char opA, char opB, int32_t m, int32_t n, int32_t k,
+                       const double* A, int64_t strideA, const double* B, int64_t strideB, double* C,
+                       int32_t batch);
 /* X(:,:,p) = A(:,:,p) \ B(:,:,p), A n x n, B n x nrhs (LU, partial pivoting). */
 int orf_pagemldivide(orf_ctx* ctx, int32_t n, int32_t nrhs, const double* A, int64_t strideA,
                      const double* B, double* X, int32_t batch);

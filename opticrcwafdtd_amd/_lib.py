@@ -67,7 +67,7 @@ EXPORTS = [
     "orf_plan_create", "orf_plan_execute", "orf_plan_fetch", "orf_plan_destroy", "orf_plan_info",
     "orf_plan_stage_times", "orf_plan_enable_timing", "orf_plan_memory", "orf_plan_estimate",
     "orf_release_cached_memory",
-    "orf_pagemtimes", "orf_pagemldivide", "orf_pagemrdivide", "orf_pageeig",
+    "orf_pagemtimes", "orf_pagemtimes_f32", "orf_pagemldivide", "orf_pagemrdivide", "orf_pageeig",
 ]
 
 _lib = None
@@ -119,6 +119,8 @@ def load_library():
                                    P(C.c_double), C.c_int64, P(C.c_double), C.c_int64, P(C.c_double),
                                    C.c_int32]
     lib.orf_pagemtimes.restype = C.c_int
+    lib.orf_pagemtimes_f32.argtypes = lib.orf_pagemtimes.argtypes
+    lib.orf_pagemtimes_f32.restype = C.c_int
     lib.orf_pagemldivide.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_double), C.c_int64,
                                      P(C.c_double), P(C.c_double), C.c_int32]
     lib.orf_pagemldivide.restype = C.c_int

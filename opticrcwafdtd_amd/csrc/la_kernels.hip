@@ -124,7 +124,126 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(
         }
       }
 }
+
+// Same tiling with FP32 products and FP32 accumulation (v_mfma_f32_16x16x4f32):
+// operands are read as complex128 and rounded to complex64 while staged, the
+// 64x64 tile is accumulated in FP32, and alpha, beta and C stay FP64.  The
+// FP32 MFMA output layout differs from the FP64 one: lane l, register v holds
+// row 4 (l / 16) + v, column l % 16 of each 16x16 block.
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <int OPA, int OPB>
+__global__ __launch_bounds__(256) void zgemm32_mfma_kernel(
+    int M, int N, int K, cplx alpha, const cplx* __restrict__ A, int lda, long long sA,
+    const cplx* __restrict__ B, int ldb, long long sB, cplx beta, cplx* __restrict__ C, int ldc,
+    long long sC) {
+  constexpr int PAD = 4;
+  __shared__ float Asr[GT_K][GT_M + PAD], Asi[GT_K][GT_M + PAD];
+  __shared__ float Bsr[GT_K][GT_N + PAD], Bsi[GT_K][GT_N + PAD];
+  const int page = blockIdx.z;
+  A += (size_t)page * sA;
+  B += (size_t)page * sB;
+  C += (size_t)page * sC;
+  const int m0 = blockIdx.y * GT_M, n0 = blockIdx.x * GT_N;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+  v4f accR[2][2], accI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accR[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+      accI[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+    }
+  for (int k0 = 0; k0 < K; k0 += GT_K) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int r, k;
+      if (OPA == 0) { r = t & 63; k = (t >> 6) + 4 * q; }
+      else { k = t & 15; r = (t >> 4) + 16 * q; }
+      int gr = m0 + r, gk = k0 + k;
+      cplx v = (gr < M && gk < K) ? load_op<OPA>(A, lda, gr, gk) : cmk(0.0, 0.0);
+      Asr[k][r] = (float)v.x;
+      Asi[k][r] = (float)v.y;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int k, c;
+      if (OPB == 0) { k = t & 15; c = (t >> 4) + 16 * q; }
+      else { c = t & 63; k = (t >> 6) + 4 * q; }
+      int gk = k0 + k, gc = n0 + c;
+      cplx v = (gk < K && gc < N) ? load_op<OPB>(B, ldb, gk, gc) : cmk(0.0, 0.0);
+      Bsr[k][c] = (float)v.x;
+      Bsi[k][c] = (float)v.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GT_K; kk += 4) {
+      const int kr = kk + (lane >> 4), rr = lane & 15;
+      float ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = Asr[kr][wm + 16 * i + rr];
+        ai[i] = Asi[kr][wm + 16 * i + rr];
+        br[i] = Bsr[kr][wn + 16 * i + rr];
+        bi[i] = Bsi[kr][wn + 16 * i + rr];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], br[j], accR[i][j], 0, 0, 0);
+          accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[i], bi[j], accR[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], bi[j], accI[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], br[j], accI[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  const bool beta0 = (beta.x == 0.0 && beta.y == 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int gr = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
+        int gc = n0 + wn + 16 * j + (lane & 15);
+        if (gr < M && gc < N) {
+          cplx acc = cmk((double)accR[i][j][v], (double)accI[i][j][v]);
+          cplx out = cmul(alpha, acc);
+          size_t idx = (size_t)gr + (size_t)gc * ldc;
+          if (!beta0) out = cadd(out, cmul(beta, C[idx]));
+          C[idx] = out;
+        }
+      }
+}
 }  // namespace
+
+void orf_zgemm_f32(hipStream_t st, char opA, char opB, int M, int N, int K, cplx alpha, const cplx* A,
+                   int lda, long long sA, const cplx* B, int ldb, long long sB, cplx beta, cplx* C,
+                   int ldc, long long sC, int batch) {
+  if (M <= 0 || N <= 0 || batch <= 0) return;
+  while (batch > 65535) {  // grid z limit
+    orf_zgemm_f32(st, opA, opB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, 65535);
+    A += (size_t)sA * 65535;
+    B += (size_t)sB * 65535;
+    C += (size_t)sC * 65535;
+    batch -= 65535;
+  }
+  dim3 grid((N + GT_N - 1) / GT_N, (M + GT_M - 1) / GT_M, batch);
+  int oa = opA == 'N' ? 0 : (opA == 'T' ? 1 : 2);
+  int ob = opB == 'N' ? 0 : (opB == 'T' ? 1 : 2);
+#define ORF_GEMM_CASE(a, b)                                                                          \
+  if (oa == a && ob == b) {                                                                          \
+    zgemm32_mfma_kernel<a, b><<<grid, 256, 0, st>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, \
+                                                    ldc, sC);                                        \
+    return;                                                                                          \
+  }
+  ORF_GEMM_CASE(0, 0) ORF_GEMM_CASE(0, 1) ORF_GEMM_CASE(0, 2)
+  ORF_GEMM_CASE(1, 0) ORF_GEMM_CASE(1, 1) ORF_GEMM_CASE(1, 2)
+  ORF_GEMM_CASE(2, 0) ORF_GEMM_CASE(2, 1) ORF_GEMM_CASE(2, 2)
+#undef ORF_GEMM_CASE
+}
 
 void orf_zgemm(hipStream_t st, char opA, char opB, int M, int N, int K, cplx alpha, const cplx* A,
                int lda, long long sA, const cplx* B, int ldb, long long sB, cplx beta, cplx* C,
@@ -1469,7 +1588,10 @@ __global__ __launch_bounds__(256) void schur_qr_win_kernel(int n, cplx* __restri
 // rotation a fixed slot in a register strip of MB_SW columns.  The deferred
 // updates of the rows above the window, the columns right of it and Z then
 // serve all shifts of the pass with one load and one store per segment.
-constexpr int MB_SEG = 16;
+// Multishift configuration: bulge positions per segment and the waves per SIMD
+// the register budget is cut for (several QR workgroups per CU hide the
+// latency of the dependent rotation chain).
+constexpr int MBQ_SEG = 16, MBQ_OCC = 2;
 constexpr int MB_NBMAX = 8;
 
 // Eigenvalues of a small m x m (m <= 8) complex matrix held in LDS (ld 8), by
@@ -1529,8 +1651,8 @@ ORF_DEV void small_eigvals(int m, cplx* A, cplx* ev) {
   for (; i >= 0; --i) ev[i] = A[i + 8 * i];  // not converged: use the diagonal
 }
 
-template <int MB_NB>
-__global__ __launch_bounds__(256) void schur_qr_mb_kernel(int n, cplx* __restrict__ Hg, int ldh,
+template <int MB_NB, int MB_SEG, int OCC>
+__global__ __launch_bounds__(256, OCC) void schur_qr_mb_kernel(int n, cplx* __restrict__ Hg, int ldh,
                                                           long long sH, cplx* __restrict__ Zg,
                                                           long long sZ, cplx* __restrict__ wout,
                                                           int* __restrict__ info, int hInLds,
@@ -1760,9 +1882,13 @@ __global__ __launch_bounds__(256) void schur_qr_mb_kernel(int n, cplx* __restric
             const int g = a0 + q;
             v[q] = (g >= 0 && g < n) ? base[(ptrdiff_t)q * (ptrdiff_t)stride] : cmk(0.0, 0.0);
           }
+          // the compiler barrier after each bulge keeps one bulge's rotations live at a
+          // time (otherwise every rc/rs load is hoisted and the kernel runs at one wave
+          // per SIMD)
           if (rowop) {
 #pragma unroll
-            for (int b = 0; b < MB_NB; ++b)
+            for (int b = 0; b < MB_NB; ++b) {
+              asm volatile("" ::: "memory");
 #pragma unroll
               for (int j = 0; j < MB_SEG; ++j) {
                 const int q = 3 * (MB_NB - 1 - b) + 1 + j;
@@ -1772,9 +1898,11 @@ __global__ __launch_bounds__(256) void schur_qr_mb_kernel(int n, cplx* __restric
                 v[q] = cadd(cscale(h1, c), cmul(sn, h2));
                 v[q + 1] = csub(cscale(h2, c), cmul(cconj(sn), h1));
               }
+            }
           } else {
 #pragma unroll
-            for (int b = 0; b < MB_NB; ++b)
+            for (int b = 0; b < MB_NB; ++b) {
+              asm volatile("" ::: "memory");
 #pragma unroll
               for (int j = 0; j < MB_SEG; ++j) {
                 const int q = 3 * (MB_NB - 1 - b) + 1 + j;
@@ -1784,6 +1912,7 @@ __global__ __launch_bounds__(256) void schur_qr_mb_kernel(int n, cplx* __restric
                 v[q] = cadd(cscale(h1, c), cmul(cconj(sn), h2));
                 v[q + 1] = csub(cscale(h2, c), cmul(sn, h1));
               }
+            }
           }
 #pragma unroll
           for (int q = 0; q < MB_SW; ++q) {
@@ -1929,11 +2058,11 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
       const size_t lim2 = lim + 20 * 1024 - stat;
       const int hIn = bytes <= lim2, zIn = 2 * bytes <= lim2;
       if (nbul >= 8)
-        schur_qr_mb_kernel<8><<<batch, 256, (hIn + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w, info,
-                                                                        hIn, zIn);
+        schur_qr_mb_kernel<8, 16, 1><<<batch, 256, (hIn + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w,
+                                                                              info, hIn, zIn);
       else
-        schur_qr_mb_kernel<4><<<batch, 256, (hIn + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w, info,
-                                                                        hIn, zIn);
+        schur_qr_mb_kernel<4, MBQ_SEG, MBQ_OCC><<<batch, 256, (hIn + zIn) * bytes, st>>>(
+            n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn);
     }
   }
   te("eig_schur_qr");

@@ -509,6 +509,19 @@ static size_t plan_bytes(const orf_plan& P) {
 // ---------------------------------------------------------------- dense S-matrix algebra
 static const cplx ONE = {1.0, 0.0}, ZERO = {0.0, 0.0}, MONE = {-1.0, 0.0};
 
+// Bulk products that the mixed-precision form runs on FP32 MFMA: the
+// convolution-matrix products and the per-layer star products.  Everything
+// else (eigensolve, propagators, every LU and solve, the last star product
+// and the layer S-matrix) stays FP64.
+static void bulk_gemm(const orf_plan& P, char opA, char opB, int M, int N, int K, cplx alpha, const cplx* A,
+                      int lda, long long sA, const cplx* B, int ldb, long long sB, cplx beta, cplx* C, int ldc,
+                      long long sC, int batch) {
+  if (P.precision == ORF_PREC_MIXED)
+    orf_zgemm_f32(P.st, opA, opB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+  else
+    orf_zgemm(P.st, opA, opB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+}
+
 static void check_info(orf_plan& P, const int* dinfo, int count, const char* what) {
   std::vector<int> h(count);
   HIPCHECK(hipMemcpyAsync(h.data(), dinfo, sizeof(int) * count, hipMemcpyDeviceToHost, P.st));
@@ -593,28 +606,28 @@ static void smatrix_star(orf_plan& P, Work& w, int Cs, cplx* const SA[4], cplx* 
   STAGE("star");
   // D = SA12 / (I - SB11 SA22)
   eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.M1, n2);
-  orf_zgemm(st, 'N', 'N', n, n, n, MONE, SB[0], n, n2, SA[3], n, n2, ONE, w.M1, n, n2, Cs);
+  bulk_gemm(P, 'N', 'N', n, n, n, MONE, SB[0], n, n2, SA[3], n, n2, ONE, w.M1, n, n2, Cs);
   orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   HIPCHECK(hipMemcpyAsync(w.Dm, SA[1], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
   orf_zgetrs(st, 'R', n, n, w.M1, n, n2, w.ipiv, w.dinv, w.Dm, n, n2, Cs);
   // F = SB21 / (I - SA22 SB11)
   eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.M1, n2);
-  orf_zgemm(st, 'N', 'N', n, n, n, MONE, SA[3], n, n2, SB[0], n, n2, ONE, w.M1, n, n2, Cs);
+  bulk_gemm(P, 'N', 'N', n, n, n, MONE, SA[3], n, n2, SB[0], n, n2, ONE, w.M1, n, n2, Cs);
   orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   HIPCHECK(hipMemcpyAsync(w.Fm, SB[2], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
   orf_zgetrs(st, 'R', n, n, w.M1, n, n2, w.ipiv, w.dinv, w.Fm, n, n2, Cs);
   // S11 = SA11 + D SB11 SA21
-  orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Dm, n, n2, SB[0], n, n2, ZERO, w.Tm, n, n2, Cs);
+  bulk_gemm(P, 'N', 'N', n, n, n, ONE, w.Dm, n, n2, SB[0], n, n2, ZERO, w.Tm, n, n2, Cs);
   HIPCHECK(hipMemcpyAsync(out[0], SA[0], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-  orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Tm, n, n2, SA[2], n, n2, ONE, out[0], n, n2, Cs);
+  bulk_gemm(P, 'N', 'N', n, n, n, ONE, w.Tm, n, n2, SA[2], n, n2, ONE, out[0], n, n2, Cs);
   // S12 = D SB12
-  orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Dm, n, n2, SB[1], n, n2, ZERO, out[1], n, n2, Cs);
+  bulk_gemm(P, 'N', 'N', n, n, n, ONE, w.Dm, n, n2, SB[1], n, n2, ZERO, out[1], n, n2, Cs);
   // S21 = F SA21
-  orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Fm, n, n2, SA[2], n, n2, ZERO, out[2], n, n2, Cs);
+  bulk_gemm(P, 'N', 'N', n, n, n, ONE, w.Fm, n, n2, SA[2], n, n2, ZERO, out[2], n, n2, Cs);
   // S22 = SB22 + F SA22 SB12
-  orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Fm, n, n2, SA[3], n, n2, ZERO, w.Tm, n, n2, Cs);
+  bulk_gemm(P, 'N', 'N', n, n, n, ONE, w.Fm, n, n2, SA[3], n, n2, ZERO, w.Tm, n, n2, Cs);
   HIPCHECK(hipMemcpyAsync(out[3], SB[3], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-  orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.Tm, n, n2, SB[1], n, n2, ONE, out[3], n, n2, Cs);
+  bulk_gemm(P, 'N', 'N', n, n, n, ONE, w.Tm, n, n2, SB[1], n, n2, ONE, out[3], n, n2, Cs);
 }
 
 // Last star product S = SA * S_trn restricted to the source columns (rcwa2D
@@ -760,6 +773,7 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   P.phi = pr->phi;
   P.factorization = pr->factorization;
   P.precision = pr->precision;
+  if (P.precision != ORF_PREC_FP64 && P.precision != ORF_PREC_MIXED) fail(ORF_INVALID_ARG, "unknown precision");
   if (pr->nMaterials <= 0 || !pr->matEps) fail(ORF_INVALID_ARG, "material table missing");
   auto matEps = [&](int mat, int l) -> zc {
     if (mat < 0 || mat >= pr->nMaterials) fail(ORF_INVALID_ARG, "material index out of range");
@@ -1152,7 +1166,7 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
             orf_zgetrf(st, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, nb, nullptr);
             orf_zgetrs(st, 'L', Wx, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, inv, Wx, (long long)Wx * Wx, nb);
             // G = Fy * [A_j]^T : (Hy x Nsy) * (Nsy x Wx^2)
-            orf_zgemm(st, 'N', 'T', Hy, Wx * Wx, P.Nsy, ONE, P.dFy.as<cplx>(), Hy, 0, inv, Wx * Wx,
+            bulk_gemm(P, 'N', 'T', Hy, Wx * Wx, P.Nsy, ONE, P.dFy.as<cplx>(), Hy, 0, inv, Wx * Wx,
                       (long long)Wx * Wx * P.Nsy, ZERO, w.GH, Hy, (long long)Hy * Wx * Wx, Cp);
             // Eyy: inverse rule along y at each x-sample, Laurent along x
             const long long pery = (long long)Wy * Wy * P.Nsx;
@@ -1162,7 +1176,7 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
             eye_stack_kernel<<<g1((long long)Wy * Wy * nb2), 256, 0, st>>>(Wy, (long long)nb2, inv);
             orf_zgetrf(st, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, nb2, nullptr);
             orf_zgetrs(st, 'L', Wy, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, inv, Wy, (long long)Wy * Wy, nb2);
-            orf_zgemm(st, 'N', 'T', Hx, Wy * Wy, P.Nsx, ONE, P.dFx.as<cplx>(), Hx, 0, inv, Wy * Wy,
+            bulk_gemm(P, 'N', 'T', Hx, Wy * Wy, P.Nsx, ONE, P.dFx.as<cplx>(), Hx, 0, inv, Wy * Wy,
                       (long long)Wy * Wy * P.Nsx, ZERO, w.HH, Hx, (long long)Hx * Wy * Wy, Cp);
             li_gather_kernel<<<g1(M2, Cp), 256, 0, st>>>(P.Nx, P.Ny, M, w.GH, w.HH, w.Exx, w.Eyy, M2);
           } else {
@@ -1207,9 +1221,9 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
           }
           // T1 = Eyy W2, T2 = Exx W1 per sector
           for (int s = 0; s < nSec; ++s) {
-            orf_zgemm(st, 'N', 'N', M, nr, M, ONE, w.Eyy, M, M2, Wf + s * nfr + M, nf, nSec * nfr, ZERO,
+            bulk_gemm(P, 'N', 'N', M, nr, M, ONE, w.Eyy, M, M2, Wf + s * nfr + M, nf, nSec * nfr, ZERO,
                       w.T12 + s * nfr, nf, nSec * nfr, Cp);
-            orf_zgemm(st, 'N', 'N', M, nr, M, ONE, w.Exx, M, M2, Wf + s * nfr, nf, nSec * nfr, ZERO,
+            bulk_gemm(P, 'N', 'N', M, nr, M, ONE, w.Exx, M, M2, Wf + s * nfr, nf, nSec * nfr, ZERO,
                       w.T12 + s * nfr + M, nf, nSec * nfr, Cp);
           }
           qw_assemble_kernel<<<g1(nfr, Cs), 256, 0, st>>>(M, nr, w.solves, pages, P.dKxy.as<double>(), Wf, nfr, w.T12,
@@ -1488,6 +1502,27 @@ int orf_pagemtimes(orf_ctx* ctx, char opA, char opB, int32_t m, int32_t n, int32
     HIPCHECK(hipMemcpy(b.p, B, nb * 16, hipMemcpyHostToDevice));
     orf_zgemm(ctx->stream, opA, opB, m, n, k, ONE, a.as<cplx>(), lda, strideA ? (long long)m * k : 0, b.as<cplx>(), ldb,
               strideB ? (long long)k * n : 0, ZERO, c.as<cplx>(), m, (long long)m * n, batch);
+    KCHECK();
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(Cout, c.p, (size_t)batch * m * n * 16, hipMemcpyDeviceToHost));
+  })
+}
+
+int orf_pagemtimes_f32(orf_ctx* ctx, char opA, char opB, int32_t m, int32_t n, int32_t k, const double* A,
+                       int64_t strideA, const double* B, int64_t strideB, double* Cout, int32_t batch) {
+  ORF_TRY({
+    if (!ctx || m <= 0 || n <= 0 || k <= 0 || batch <= 0) fail(ORF_INVALID_ARG, "bad sizes");
+    HIPCHECK(hipSetDevice(ctx->device));
+    const int lda = (opA == 'N') ? m : k, ldb = (opB == 'N') ? k : n;
+    const size_t na = (size_t)(strideA ? batch : 1) * (size_t)m * k, nb = (size_t)(strideB ? batch : 1) * (size_t)k * n;
+    DBuf a, b, c;
+    a.alloc(na * 16);
+    b.alloc(nb * 16);
+    c.alloc((size_t)batch * m * n * 16);
+    HIPCHECK(hipMemcpy(a.p, A, na * 16, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(b.p, B, nb * 16, hipMemcpyHostToDevice));
+    orf_zgemm_f32(ctx->stream, opA, opB, m, n, k, ONE, a.as<cplx>(), lda, strideA ? (long long)m * k : 0, b.as<cplx>(),
+                  ldb, strideB ? (long long)k * n : 0, ZERO, c.as<cplx>(), m, (long long)m * n, batch);
     KCHECK();
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     HIPCHECK(hipMemcpy(Cout, c.p, (size_t)batch * m * n * 16, hipMemcpyDeviceToHost));

@@ -38,6 +38,28 @@ def test_pagemtimes(opA, opB, m, n, k, batch):
     assert np.max(np.abs(got - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("opA,opB,m,n,k,batch", [
+    ("N", "N", 17, 23, 9, 3), ("N", "N", 242, 242, 242, 2), ("T", "N", 64, 130, 70, 2), ("N", "C", 121, 61, 100, 2)])
+def test_pagemtimes_f32(opA, opB, m, n, k, batch):
+    """FP32 products with FP32 accumulation: exact up to the FP32 accumulation bound of the
+    complex64-rounded operands, and visibly different from the FP64 product."""
+    rng = np.random.default_rng(3)
+    A = _crand(rng, batch, m if opA == "N" else k, k if opA == "N" else m)
+    B = _crand(rng, batch, k if opB == "N" else n, n if opB == "N" else k)
+    op = {"N": lambda x: x, "T": lambda x: np.swapaxes(x, -1, -2), "C": lambda x: np.conj(np.swapaxes(x, -1, -2))}
+    A32, B32 = A.astype(np.complex64).astype(complex), B.astype(np.complex64).astype(complex)
+    ref32 = op[opA](A32) @ op[opB](B32)
+    bound = op[opA](np.abs(A32)) @ op[opB](np.abs(B32))
+    Cf = np.zeros((batch, n, m), complex)
+    lib = _lib.load_library()
+    rc = lib.orf_pagemtimes_f32(_lib.context(), opA.encode(), opB.encode(), m, n, k, _lib.dptr(_f(A)), A[0].size,
+                                _lib.dptr(_f(B)), B[0].size, _lib.dptr(Cf), batch)
+    assert rc == 0, _lib.last_error()
+    got = np.transpose(Cf, (0, 2, 1))
+    assert np.all(np.abs(got - ref32) <= 2 * (k + 2) * 2.0 ** -24 * bound + 1e-30)
+    assert np.max(np.abs(got - op[opA](A) @ op[opB](B))) > 1e-9  # really FP32
+
+
 def test_pagemtimes_broadcast():
     rng = np.random.default_rng(2)
     A = _crand(rng, 1, 40, 40)
