@@ -1108,6 +1108,226 @@ __global__ __launch_bounds__(256) void hessenberg_noq_kernel(int n, cplx* __rest
 // W = Q W with Q = H(0) H(1) ... H(n-3) from hessenberg_noq_kernel.  Columns
 // of W are independent: 16 columns per workgroup in LDS, 16 lanes per column,
 // reflectors applied last to first with no block barrier.
+// Blocked Householder reduction to Hessenberg form for matrices that do not
+// fit in LDS: the published panel algorithm behind LAPACK's zgehrd/zlahr2
+// (Quintana-Orti & van de Geijn).  A panel of NB reflectors is generated with
+// the trailing matrix left untouched except for one matrix-vector product
+// per reflector (Y = A V T accumulated in Yg); the trailing matrix is then
+// updated once per panel, from the right (A -= Y V^H) and from the left
+// (A -= V T^H V^H A).  Compared with one rank-1 update pair per column this
+// reads and writes the trailing matrix NB times less often.  The panel's
+// reflectors are staged in LDS (dynamic, n x NB) next to the current column.
+// Output layout as hessenberg_noq_kernel: H in A (zeros below the
+// subdiagonal), reflector k in column k of Vg (v[k+1] = 1), tau[k].
+template <int NB>
+__global__ __launch_bounds__(256) void hessenberg_blocked_kernel(int n, cplx* __restrict__ Ag, int lda,
+                                                                 long long sA, cplx* __restrict__ Vg,
+                                                                 cplx* __restrict__ taug,
+                                                                 cplx* __restrict__ Yg, long long sY) {
+  extern __shared__ cplx lds_dyn[];
+  cplx* Vs = lds_dyn;             // Vs[r + n j] = V(r, k0 + j), rows > k0 + j valid
+  cplx* bv = lds_dyn + (size_t)n * NB;  // current column / reflector, indexed by row
+  __shared__ cplx Ts[NB * NB];    // T(r, c) = Ts[r + NB c], upper triangular
+  __shared__ cplx sw[NB], sw2[NB];
+  __shared__ double red[4];
+  const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
+  const int lane = t & 63, w = t >> 6;
+  cplx* A = Ag + (size_t)page * sA;
+  cplx* V = Vg + (size_t)page * n * n;
+  cplx* tau = taug + (size_t)page * n;
+  cplx* Y = Yg + (size_t)page * sY;
+#define AA(r, c) A[(size_t)(r) + (size_t)(c) * lda]
+#define YY(r, j) Y[(size_t)(r) + (size_t)(j) * n]
+#define VS(r, j) Vs[(size_t)(r) + (size_t)(j) * n]
+  const int nref = n - 2;
+  for (int k0 = 0; k0 < nref; k0 += NB) {
+    const int ib = min(NB, nref - k0);
+    for (int i = 0; i < ib; ++i) {
+      const int c = k0 + i;
+      // (a) column c with the panel's earlier reflectors: right (A Q: b -= Y V(c,:)^H), then left (Q^H b)
+      if (t < i) sw[t] = cconj(VS(c, t));
+      __syncthreads();
+      for (int r = k0 + 1 + t; r < n; r += nt) {
+        cplx x = AA(r, c);
+        for (int j = 0; j < i; ++j) x = csub(x, cmul(YY(r, j), sw[j]));
+        bv[r] = x;
+      }
+      __syncthreads();
+      if (i > 0) {
+        for (int j = w; j < i; j += 4) {  // w_j = V(:, j)^H b, a wave per reflector
+          double sr = 0.0, si = 0.0;
+          for (int r = k0 + j + 1 + lane; r < n; r += 64) {
+            const cplx p = cmulc(VS(r, j), bv[r]);
+            sr += p.x;
+            si += p.y;
+          }
+          sr = wave_sum(sr);
+          si = wave_sum(si);
+          if (lane == 0) sw2[j] = cmk(sr, si);
+        }
+        __syncthreads();
+        if (t < i) {  // sw = T^H w
+          cplx s = cmk(0.0, 0.0);
+          for (int l = 0; l <= t; ++l) s = cadd(s, cmulc(Ts[l + NB * t], sw2[l]));
+          sw[t] = s;
+        }
+        __syncthreads();
+        for (int r = k0 + 1 + t; r < n; r += nt) {
+          cplx s = cmk(0.0, 0.0);
+          const int jmax = min(i, r - k0);  // V(r, j) = 0 for r <= k0 + j
+          for (int j = 0; j < jmax; ++j) s = cadd(s, cmul(VS(r, j), sw[j]));
+          bv[r] = csub(bv[r], s);
+        }
+        __syncthreads();
+      }
+      // (b) reflector annihilating rows c+2.. of column c
+      double s2 = 0.0;
+      for (int r = c + 2 + t; r < n; r += nt) s2 += cnorm2(bv[r]);
+      s2 = block_sum(s2, red);
+      const cplx alpha = bv[c + 1];
+      const bool triv = (s2 == 0.0 && alpha.y == 0.0);
+      double beta = 0.0;
+      cplx tk = cmk(0.0, 0.0), scal = cmk(0.0, 0.0);
+      if (!triv) {
+        beta = -copysign(sqrt(cnorm2(alpha) + s2), alpha.x);
+        tk = cmk((beta - alpha.x) / beta, -alpha.y / beta);
+        scal = crecip(csub(alpha, cmk(beta, 0.0)));
+      }
+      __syncthreads();  // every thread has read alpha before bv is overwritten
+      for (int r = c + 1 + t; r < n; r += nt) {
+        const cplx b = bv[r];
+        const cplx vr = (r == c + 1) ? cmk(1.0, 0.0) : (triv ? cmk(0.0, 0.0) : cmul(b, scal));
+        V[(size_t)c * n + r] = vr;
+        VS(r, i) = vr;
+        AA(r, c) = triv ? b : (r == c + 1 ? cmk(beta, 0.0) : cmk(0.0, 0.0));
+        bv[r] = vr;
+      }
+      for (int r = k0 + 1 + t; r <= c; r += nt) AA(r, c) = bv[r];
+      if (t == 0) tau[c] = tk;
+      __syncthreads();
+      // (c) Y(:, i) = tau (A v - Y (V^H v)) on rows k0+1..; T(:, i) = -tau T (V^H v), T(i, i) = tau
+      for (int j = w; j < i; j += 4) {
+        double sr = 0.0, si = 0.0;
+        for (int r = c + 1 + lane; r < n; r += 64) {
+          const cplx p = cmulc(VS(r, j), bv[r]);
+          sr += p.x;
+          si += p.y;
+        }
+        sr = wave_sum(sr);
+        si = wave_sum(si);
+        if (lane == 0) sw[j] = cmk(sr, si);
+      }
+      __syncthreads();
+      for (int r = k0 + 1 + t; r < n; r += nt) {
+        cplx y0 = cmk(0.0, 0.0), y1 = cmk(0.0, 0.0);
+        int q = c + 1;
+        for (; q + 1 < n; q += 2) {
+          y0 = cadd(y0, cmul(AA(r, q), bv[q]));
+          y1 = cadd(y1, cmul(AA(r, q + 1), bv[q + 1]));
+        }
+        if (q < n) y0 = cadd(y0, cmul(AA(r, q), bv[q]));
+        cplx y = cadd(y0, y1);
+        for (int j = 0; j < i; ++j) y = csub(y, cmul(YY(r, j), sw[j]));
+        YY(r, i) = cmul(tk, y);
+      }
+      if (t < i) {
+        cplx s = cmk(0.0, 0.0);
+        for (int l = t; l < i; ++l) s = cadd(s, cmul(Ts[t + NB * l], sw[l]));
+        Ts[t + NB * i] = cneg(cmul(tk, s));
+      }
+      if (t == 0) Ts[i + NB * i] = tk;
+      for (int l = i + 1 + t; l < NB; l += nt) Ts[l + NB * i] = cmk(0.0, 0.0);
+      __syncthreads();
+    }
+    const int q0 = k0 + ib;
+    // (e) rows 0..k0 of Y: P = A(0:k0, k0+1:) V in Y's columns NB.., then Y = P T
+    for (int e = t; e < (k0 + 1) * ib; e += nt) {
+      const int r = e % (k0 + 1), j = e / (k0 + 1);
+      cplx p0 = cmk(0.0, 0.0), p1 = cmk(0.0, 0.0);
+      int q = k0 + j + 1;
+      for (; q + 1 < n; q += 2) {
+        p0 = cadd(p0, cmul(AA(r, q), VS(q, j)));
+        p1 = cadd(p1, cmul(AA(r, q + 1), VS(q + 1, j)));
+      }
+      if (q < n) p0 = cadd(p0, cmul(AA(r, q), VS(q, j)));
+      YY(r, NB + j) = cadd(p0, p1);
+    }
+    __syncthreads();
+    for (int e = t; e < (k0 + 1) * ib; e += nt) {
+      const int r = e % (k0 + 1), j = e / (k0 + 1);
+      cplx s = cmk(0.0, 0.0);
+      for (int l = 0; l <= j; ++l) s = cadd(s, cmul(YY(r, NB + l), Ts[l + NB * j]));
+      YY(r, j) = s;
+    }
+    __syncthreads();
+    // (f) trailing columns from the right, every row: A(r, q) -= Y(r, :) V(q, :)^H
+    if (q0 < n) {
+      for (int r = t; r < n; r += nt) {
+        cplx yr[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) yr[j] = j < ib ? YY(r, j) : cmk(0.0, 0.0);
+        for (int q = q0; q < n; ++q) {
+          cplx s = cmk(0.0, 0.0);
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            if (j < ib) s = cadd(s, cmulc(VS(q, j), yr[j]));
+          AA(r, q) = csub(AA(r, q), s);
+        }
+      }
+    }
+    // (g) rows 0..k0 of the panel's columns k0+1..k0+ib-1
+    for (int e = t; e < (k0 + 1) * (ib - 1); e += nt) {
+      const int r = e % (k0 + 1), cc = k0 + 1 + e / (k0 + 1);
+      cplx s = cmk(0.0, 0.0);
+      for (int l = 0; l < cc - k0; ++l) s = cadd(s, cmulc(VS(cc, l), YY(r, l)));
+      AA(r, cc) = csub(AA(r, cc), s);
+    }
+    __syncthreads();
+    // (h) trailing columns from the left, rows k0+1..: A -= V T^H (V^H A), a wave per column
+    for (int q = q0 + w; q < n; q += 4) {
+      cplx x[TREVC_SLOTS_H];
+      cplx part[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) part[j] = cmk(0.0, 0.0);
+#pragma unroll
+      for (int sl = 0; sl < TREVC_SLOTS_H; ++sl) {
+        const int r = k0 + 1 + lane + 64 * sl;
+        x[sl] = (r < n) ? AA(r, q) : cmk(0.0, 0.0);
+        if (r < n) {
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            if (j < ib && r >= k0 + j + 1) part[j] = cadd(part[j], cmulc(VS(r, j), x[sl]));
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) part[j] = cmk(wave_sum(part[j].x), wave_sum(part[j].y));
+      // W := T^H W in place, last entry first (entry j needs W_0..W_j)
+#pragma unroll
+      for (int j = NB - 1; j >= 0; --j) {
+        cplx s = cmk(0.0, 0.0);
+#pragma unroll
+        for (int l = 0; l <= j; ++l) s = cadd(s, cmulc(Ts[l + NB * j], part[l]));
+        part[j] = j < ib ? s : cmk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int sl = 0; sl < TREVC_SLOTS_H; ++sl) {
+        const int r = k0 + 1 + lane + 64 * sl;
+        if (r < n) {
+          cplx s = cmk(0.0, 0.0);
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            if (j < ib && r >= k0 + j + 1) s = cadd(s, cmul(VS(r, j), part[j]));
+          AA(r, q) = csub(x[sl], s);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#undef AA
+#undef YY
+#undef VS
+}
+
 __global__ __launch_bounds__(256) void hess_q_apply_kernel(int n, const cplx* __restrict__ Vg,
                                                            const cplx* __restrict__ taug,
                                                            cplx* __restrict__ Wg, int ldw,
@@ -2041,7 +2261,17 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
   cplx* Vh = Hwork;
   cplx* tau = Hwork + (size_t)n * n * batch;
   tb("eig_hessenberg");
-  hessenberg_noq_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, Vh, tau, use_lds);
+  // in LDS: one rank-1 update pair per column; otherwise panels of reflectors with
+  // Level-3 trailing updates (panel reflectors and the current column in LDS, Y in Zwork)
+  const size_t pan16 = (size_t)n * 17 * sizeof(cplx), pan8 = (size_t)n * 9 * sizeof(cplx);
+  if (use_lds || n < 32)
+    hessenberg_noq_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, Vh, tau, use_lds);
+  else if (pan16 <= (size_t)g_lds_limit)
+    hessenberg_blocked_kernel<16><<<batch, 256, pan16, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
+  else if (pan8 <= (size_t)g_lds_limit)
+    hessenberg_blocked_kernel<8><<<batch, 256, pan8, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
+  else
+    hessenberg_noq_kernel<<<batch, 256, 0, st>>>(n, A, lda, sA, Vh, tau, 0);
   eye_batch(st, n, Zwork, sZ, batch);
   te("eig_hessenberg");
   tb("eig_schur_qr");
