@@ -12,12 +12,16 @@
 //  * zgetrf:      one workgroup per page, partial pivoting (max |re|+|im|, first
 //                 index on ties, as LAPACK izamax), matrix staged in LDS when it fits.
 //  * zgetrs:      left (A X = B) and right (X A = B) solves from the LU factors.
-//  * eig:         Householder Hessenberg reduction with the unitary factor
-//                 accumulated, complex single-shift QR (Wilkinson shift,
-//                 exceptional shifts, Ahues-Tisseur deflation) to Schur form,
-//                 then eigenvectors of the triangular factor by back
-//                 substitution (LAPACK ztrevc's small-pivot rule), back-
-//                 transformed by a ZGEMM and normalised to unit 2-norm.
+//  * eig:         Householder Hessenberg reduction (in LDS one column at a
+//                 time; otherwise blocked panels with Level-3 trailing
+//                 updates), its unitary factor formed as the starting Schur
+//                 basis, complex QR to Schur form (Wilkinson / exceptional
+//                 shifts, Ahues-Tisseur deflation; small matrices one
+//                 rotation at a time in LDS, larger ones with several bulges
+//                 chased in lockstep through an LDS window), eigenvectors of
+//                 the triangular factor by back substitution (LAPACK ztrevc's
+//                 small-pivot rule), back-transformed by one ZGEMM and
+//                 normalised to unit 2-norm.
 #include "orf_common.h"
 #include "orf_internal.h"
 
@@ -878,7 +882,8 @@ void orf_zgetrf_only(hipStream_t st, int n, cplx* A, int lda, long long sA, int*
     zgetrf_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, ipiv, info, use_lds);
     return;
   }
-  // panel + U rows in LDS: 2 * LB * n complex
+  // panel + U rows in LDS: 2 * LB * n complex (measured: a 16-wide panel at one
+  // workgroup per CU beats an 8-wide one at two)
   const size_t per = 2 * (size_t)n * sizeof(cplx);
   if (16 * per <= (size_t)g_lds_limit)
     zgetrf_blocked_kernel<16><<<batch, 256, 16 * per, st>>>(n, A, lda, sA, ipiv, info);
@@ -919,7 +924,7 @@ void orf_zgetrs(hipStream_t st, char side, int n, int nrhs, const cplx* LU, int 
     return;
   }
   if (cb > 32) cb = 32;  // 16-wide MFMA tiles; moderate LDS footprint keeps several blocks per CU
-  if (cb > 16 && nrhs <= 16) cb = 16;
+  if (cb > 16 && (nrhs <= 16 || (size_t)n * cb * sizeof(cplx) > (size_t)g_lds_limit / 2)) cb = 16;
   dim3 grid((nrhs + cb - 1) / cb, batch);
   const long long sD = orf_dinv_stride(n);
   if (side == 'L')
@@ -936,89 +941,10 @@ constexpr double ULP = 2.220446049250313e-16;   // LAPACK dlamch('P') = eps * ba
 constexpr double SAFMIN = 2.2250738585072014e-308;
 constexpr int TREVC_SLOTS_H = 12;  // columns up to 768 rows in registers
 
-// Householder reduction to upper Hessenberg form, Z = H(0) H(1) ... accumulated.
-// A (n x n, ld) is reduced in place; entries below the subdiagonal are zeroed.
-__global__ __launch_bounds__(256) void hessenberg_kernel(int n, cplx* __restrict__ Ag, int lda,
-                                                         long long sA, cplx* __restrict__ Zg,
-                                                         long long sZ, int use_lds) {
-  extern __shared__ cplx lds_mat[];
-  __shared__ cplx v[1024];
-  __shared__ double red[4];
-  const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
-  const int lane = t & 63, w = t >> 6;
-  Ag += (size_t)page * sA;
-  cplx* Z = Zg + (size_t)page * sZ;
-  cplx* A = Ag;
-  int ld = lda;
-  if (use_lds) {
-    A = lds_mat;
-    ld = n;
-    for (int e = t; e < n * n; e += nt) A[e] = Ag[(size_t)(e % n) + (size_t)(e / n) * lda];
-  }
-  for (int e = t; e < n * n; e += nt) Z[e] = cmk((e % n) == (e / n) ? 1.0 : 0.0, 0.0);
-  __syncthreads();
-  for (int k = 0; k < n - 2; ++k) {
-    const int m = n - k - 1;
-    cplx* colk = A + (size_t)k * ld;
-    double s2 = 0.0;
-    for (int i = k + 2 + t; i < n; i += nt) s2 += cnorm2(colk[i]);
-    s2 = block_sum(s2, red);
-    const cplx alpha = colk[k + 1];
-    if (s2 == 0.0 && alpha.y == 0.0) {
-      __syncthreads();
-      continue;  // H(k) = I
-    }
-    const double beta = -copysign(sqrt(cnorm2(alpha) + s2), alpha.x);
-    const cplx tau = cmk((beta - alpha.x) / beta, -alpha.y / beta);
-    const cplx scal = crecip(csub(alpha, cmk(beta, 0.0)));
-    for (int i = t; i < m; i += nt) v[i] = (i == 0) ? cmk(1.0, 0.0) : cmul(colk[k + 1 + i], scal);
-    __syncthreads();
-    for (int i = k + 1 + t; i < n; i += nt) colk[i] = (i == k + 1) ? cmk(beta, 0.0) : cmk(0.0, 0.0);
-    // right: rows 0..n-1 of A and Z, columns k+1..n-1:  X -= tau (X v) v^H
-    for (int r = t; r < n; r += nt) {
-      cplx wa = cmk(0.0, 0.0), wz = cmk(0.0, 0.0);
-      for (int j = 0; j < m; ++j) {
-        const cplx vj = v[j];
-        wa = cadd(wa, cmul(A[r + (size_t)(k + 1 + j) * ld], vj));
-        wz = cadd(wz, cmul(Z[r + (size_t)(k + 1 + j) * n], vj));
-      }
-      wa = cmul(tau, wa);
-      wz = cmul(tau, wz);
-      for (int j = 0; j < m; ++j) {
-        const cplx vjc = cconj(v[j]);
-        cplx* pa = A + r + (size_t)(k + 1 + j) * ld;
-        *pa = csub(*pa, cmul(wa, vjc));
-        cplx* pz = Z + r + (size_t)(k + 1 + j) * n;
-        *pz = csub(*pz, cmul(wz, vjc));
-      }
-    }
-    __syncthreads();
-    // left: rows k+1..n-1, columns k+1..n-1:  A -= conj(tau) v (v^H A); one wave per column
-    const cplx taub = cconj(tau);
-    for (int j = k + 1 + w; j < n; j += nt >> 6) {
-      cplx* col = A + (size_t)j * ld;
-      double zr = 0.0, zi = 0.0;
-      for (int i = lane; i < m; i += 64) {
-        cplx p = cmulc(v[i], col[k + 1 + i]);
-        zr += p.x;
-        zi += p.y;
-      }
-      zr = wave_sum(zr);
-      zi = wave_sum(zi);
-      const cplx f = cmul(taub, cmk(zr, zi));
-      for (int i = lane; i < m; i += 64) col[k + 1 + i] = csub(col[k + 1 + i], cmul(v[i], f));
-    }
-    __syncthreads();
-  }
-  if (use_lds) {
-    for (int e = t; e < n * n; e += nt) Ag[(size_t)(e % n) + (size_t)(e / n) * lda] = A[e];
-  }
-}
-
 // Householder reduction to Hessenberg form without forming Q: the reflectors
 // H(k) = I - tau_k v_k v_k^H (v_k[k+1] = 1) go to V (column k, rows k+1..n-1)
-// and tau; Q = H(0) H(1) ... is applied to the eigenvectors afterwards
-// (hess_q_apply_kernel).  The left update keeps each column in registers
+// and tau; Q = H(0) H(1) ... is formed afterwards as the QR's starting Schur
+// basis (hess_q_apply_kernel, form = 1).  The left update keeps each column in registers
 // (one read and one write per column and step).
 __global__ __launch_bounds__(256) void hessenberg_noq_kernel(int n, cplx* __restrict__ Ag, int lda,
                                                              long long sA, cplx* __restrict__ Vg,
@@ -1328,10 +1254,14 @@ __global__ __launch_bounds__(256) void hessenberg_blocked_kernel(int n, cplx* __
 #undef VS
 }
 
+// W := Q W with Q = H_0 H_1 ... H_{n-3} the Hessenberg reduction's reflectors,
+// 16 columns per workgroup held in LDS.  form = 1 writes Q itself (W = I on
+// entry is implied): column j of Q only needs H_0 .. H_{j-1}, a third of the
+// work of applying Q to a dense matrix.
 __global__ __launch_bounds__(256) void hess_q_apply_kernel(int n, const cplx* __restrict__ Vg,
                                                            const cplx* __restrict__ taug,
                                                            cplx* __restrict__ Wg, int ldw,
-                                                           long long sW) {
+                                                           long long sW, int form) {
   extern __shared__ cplx Wc[];  // n x 16
   const int page = blockIdx.y, c0 = blockIdx.x * 16, t = threadIdx.x;
   const cplx* V = Vg + (size_t)page * n * n;
@@ -1340,13 +1270,14 @@ __global__ __launch_bounds__(256) void hess_q_apply_kernel(int n, const cplx* __
   const int nc = min(16, n - c0);
   for (int e = t; e < n * nc; e += blockDim.x) {
     const int r = e % n, c = e / n;
-    Wc[r + c * n] = W[(size_t)r + (size_t)(c0 + c) * ldw];
+    Wc[r + c * n] = form ? cmk(r == c0 + c ? 1.0 : 0.0, 0.0) : W[(size_t)r + (size_t)(c0 + c) * ldw];
   }
   __syncthreads();
   const int cl = t >> 4, sub = t & 15;
   if (cl < nc) {
     cplx* col = Wc + cl * n;
-    for (int k = n - 3; k >= 0; --k) {
+    const int kstart = form ? min(n - 3, c0 + cl - 1) : n - 3;
+    for (int k = kstart; k >= 0; --k) {
       const cplx tk = tau[k];
       if (tk.x == 0.0 && tk.y == 0.0) continue;
       const int m = n - k - 1;
@@ -1557,247 +1488,10 @@ __global__ __launch_bounds__(256) void schur_qr_kernel(int n, cplx* __restrict__
 #undef HH
 }
 
-// Windowed variant of the same single-shift QR.  A sweep is cut into segments
-// of QR_SEG rotations.  Wave 0 chases the bulge through a small diagonal window
-// of H held in LDS (every entry of the window updated at once); the whole
-// workgroup then applies the segment's rotations, kept in registers, to the
-// rows above the window, the columns right of it and the Schur vectors Z.
-// Each H/Z element is read and written once per segment instead of once per
-// rotation, and the rotation chain needs no block barrier.
-constexpr int QR_SEG = 32;
-constexpr int QR_WIN = QR_SEG + 3;
-
+// LDS visibility and execution barrier within one wavefront.
 ORF_DEV void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-}
-
-__global__ __launch_bounds__(256) void schur_qr_win_kernel(int n, cplx* __restrict__ Hg, int ldh,
-                                                           long long sH, cplx* __restrict__ Zg,
-                                                           long long sZ, cplx* __restrict__ wout,
-                                                           int* __restrict__ info, int hInLds,
-                                                           int zInLds) {
-  extern __shared__ cplx lds_mat[];
-  __shared__ cplx win[QR_WIN * QR_WIN];
-  __shared__ double rc[QR_SEG];
-  __shared__ cplx rs[QR_SEG];
-  __shared__ int redk[4];
-  const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
-  const int lane = t & 63, wv = t >> 6, nw = nt >> 6;
-  Hg += (size_t)page * sH;
-  cplx* Zp = Zg + (size_t)page * sZ;
-  wout += (size_t)page * n;
-  cplx* H = Hg;
-  int ld = ldh;
-  cplx* Z = Zp;
-  cplx* ldsNext = lds_mat;
-  if (hInLds) {
-    H = ldsNext;
-    ld = n;
-    ldsNext += (size_t)n * n;
-    for (int e = t; e < n * n; e += nt) H[e] = Hg[(size_t)(e % n) + (size_t)(e / n) * ldh];
-  }
-  if (zInLds) {
-    Z = ldsNext;
-    for (int e = t; e < n * n; e += nt) Z[e] = Zp[e];
-  }
-  __syncthreads();
-#define HH(r, c) H[(size_t)(r) + (size_t)(c) * ld]
-#define WN(r, c) win[(r) + (c) * QR_WIN]
-  const double smlnum = SAFMIN * ((double)n / ULP);
-  const int itmax = 100;
-  int failed = 0;
-  int i = n - 1;
-  while (i >= 0) {
-    int l = 0;
-    int its;
-    for (its = 0; its <= itmax; ++its) {
-      int kfound = l;
-      for (int k = i - t; k > l; k -= nt) {
-        const cplx hkk1 = HH(k, k - 1);
-        bool small = false;
-        if (cabs1(hkk1) <= smlnum) small = true;
-        else {
-          double tst = cabs1(HH(k - 1, k - 1)) + cabs1(HH(k, k));
-          if (tst == 0.0) {
-            if (k - 2 >= 0) tst += cabs1(HH(k - 1, k - 2));
-            if (k + 1 <= n - 1) tst += cabs1(HH(k + 1, k));
-          }
-          if (cabs1(hkk1) <= ULP * tst) {
-            const double ab = fmax(cabs1(hkk1), cabs1(HH(k - 1, k)));
-            const double ba = fmin(cabs1(hkk1), cabs1(HH(k - 1, k)));
-            const cplx dd = csub(HH(k - 1, k - 1), HH(k, k));
-            const double aa = fmax(cabs1(HH(k, k)), cabs1(dd));
-            const double bb = fmin(cabs1(HH(k, k)), cabs1(dd));
-            const double s = aa + ab;
-            if (ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)))) small = true;
-          }
-        }
-        if (small) { kfound = k; break; }
-      }
-      int kv = kfound;
-      for (int o = 32; o > 0; o >>= 1) kv = max(kv, __shfl_xor(kv, o, 64));
-      __syncthreads();
-      if (lane == 0) redk[wv] = kv;
-      __syncthreads();
-      int kmax = redk[0];
-      for (int q = 1; q < nw; ++q) kmax = max(kmax, redk[q]);
-      l = kmax;
-      if (l > 0) {
-        __syncthreads();
-        if (t == 0) HH(l, l - 1) = cmk(0.0, 0.0);
-        __syncthreads();
-      }
-      if (l >= i) break;
-      cplx sig;
-      if (its % 20 == 10) {
-        sig = cadd(cmk(0.75 * cabs1(HH(l + 1, l)), 0.0), HH(l, l));
-      } else if (its > 0 && its % 20 == 0) {
-        sig = cadd(cmk(0.75 * cabs1(HH(i, i - 1)), 0.0), HH(i, i));
-      } else {
-        cplx tt = HH(i, i);
-        const cplx u = cmul(csqrt(HH(i - 1, i)), csqrt(HH(i, i - 1)));
-        double s = cabs1(u);
-        if (s != 0.0) {
-          const cplx x = cscale(csub(HH(i - 1, i - 1), tt), 0.5);
-          const double sx = cabs1(x);
-          s = fmax(s, cabs1(x));
-          const cplx xs = cscale(x, 1.0 / s), us = cscale(u, 1.0 / s);
-          cplx y = cscale(csqrt(cadd(cmul(xs, xs), cmul(us, us))), s);
-          if (sx > 0.0) {
-            const cplx xx = cscale(x, 1.0 / sx);
-            if (xx.x * y.x + xx.y * y.y < 0.0) y = cneg(y);
-          }
-          tt = csub(tt, cmul(u, cdiv(u, cadd(x, y))));
-        }
-        sig = tt;
-      }
-      if (!(isfinite(sig.x) && isfinite(sig.y))) {
-        its = itmax + 1;
-        break;
-      }
-      // ---- one sweep l..i in segments of QR_SEG rotations
-      for (int k0 = l; k0 < i; k0 += QR_SEG) {
-        const int k1 = min(k0 + QR_SEG, i);       // rotations k0 .. k1-1
-        const int a = (k0 == l) ? l : k0 - 1;     // window rows/cols a..e
-        const int e = min(k1 + 1, i);
-        const int W = e - a + 1;
-        __syncthreads();
-        for (int q = t; q < W * W; q += nt) {
-          const int r = q % W, c = q / W;
-          WN(r, c) = HH(a + r, a + c);
-        }
-        __syncthreads();
-        if (wv == 0) {
-          for (int k = k0; k < k1; ++k) {
-            const int kk = k - a;
-            cplx x, y;
-            if (k == l) { x = csub(WN(kk, kk), sig); y = WN(kk + 1, kk); }
-            else { x = WN(kk, kk - 1); y = WN(kk + 1, kk - 1); }
-            double c;
-            cplx s, r;
-            givens(x, y, c, s, r);
-            const cplx sc = cconj(s);
-            if (lane == 0) { rc[k - k0] = c; rs[k - k0] = s; }
-            // rows kk, kk+1 for window columns kk..W-1
-            const int j = kk + lane;
-            cplx h1, h2;
-            if (j < W) { h1 = WN(kk, j); h2 = WN(kk + 1, j); }
-            wave_sync_lds();
-            if (j < W) {
-              WN(kk, j) = cadd(cscale(h1, c), cmul(s, h2));
-              WN(kk + 1, j) = csub(cscale(h2, c), cmul(sc, h1));
-            }
-            if (k > l && lane == 0) {
-              WN(kk, kk - 1) = r;
-              WN(kk + 1, kk - 1) = cmk(0.0, 0.0);
-            }
-            wave_sync_lds();
-            // columns kk, kk+1 for window rows 0..min(kk+2, i-a)
-            const int rmax = min(kk + 2, i - a);
-            const int q = lane;
-            if (q <= rmax) { h1 = WN(q, kk); h2 = WN(q, kk + 1); }
-            wave_sync_lds();
-            if (q <= rmax) {
-              WN(q, kk) = cadd(cscale(h1, c), cmul(sc, h2));
-              WN(q, kk + 1) = csub(cscale(h2, c), cmul(s, h1));
-            }
-            wave_sync_lds();
-          }
-        }
-        __syncthreads();
-        // write the window back
-        for (int q = t; q < W * W; q += nt) {
-          const int r = q % W, c = q / W;
-          HH(a + r, a + c) = WN(r, c);
-        }
-        // deferred rotations: columns right of the window (row ops on rows k0..k1),
-        // rows above the window and all rows of Z (column ops on columns k0..k1)
-        const int nk = k1 - k0;                   // rotations in this segment
-        const int ncR = n - 1 - e;                // columns e+1..n-1
-        const int nrA = a;                        // rows 0..a-1
-        const int tasks = ncR + nrA + n;
-        for (int task = t; task < tasks; task += nt) {
-          cplx v[QR_SEG + 1];
-          if (task < ncR) {
-            cplx* col = H + (size_t)(e + 1 + task) * ld + k0;
-#pragma unroll
-            for (int q = 0; q <= QR_SEG; ++q) v[q] = (q <= nk) ? col[q] : cmk(0.0, 0.0);
-#pragma unroll
-            for (int q = 0; q < QR_SEG; ++q) {
-              if (q < nk) {
-                const double c = rc[q];
-                const cplx s = rs[q];
-                const cplx h1 = v[q], h2 = v[q + 1];
-                v[q] = cadd(cscale(h1, c), cmul(s, h2));
-                v[q + 1] = csub(cscale(h2, c), cmul(cconj(s), h1));
-              }
-            }
-#pragma unroll
-            for (int q = 0; q <= QR_SEG; ++q)
-              if (q <= nk) col[q] = v[q];
-          } else {
-            cplx* base;
-            size_t ldx;
-            int row;
-            if (task < ncR + nrA) { row = task - ncR; base = H; ldx = ld; }
-            else { row = task - ncR - nrA; base = Z; ldx = n; }
-            cplx* p0 = base + row + (size_t)k0 * ldx;
-#pragma unroll
-            for (int q = 0; q <= QR_SEG; ++q) v[q] = (q <= nk) ? p0[(size_t)q * ldx] : cmk(0.0, 0.0);
-#pragma unroll
-            for (int q = 0; q < QR_SEG; ++q) {
-              if (q < nk) {
-                const double c = rc[q];
-                const cplx s = rs[q];
-                const cplx h1 = v[q], h2 = v[q + 1];
-                v[q] = cadd(cscale(h1, c), cmul(cconj(s), h2));
-                v[q + 1] = csub(cscale(h2, c), cmul(s, h1));
-              }
-            }
-#pragma unroll
-            for (int q = 0; q <= QR_SEG; ++q)
-              if (q <= nk) p0[(size_t)q * ldx] = v[q];
-          }
-        }
-      }
-      __syncthreads();
-    }
-    if (its > itmax) {
-      if (!failed) failed = i + 1;
-      l = i;
-    }
-    if (t == 0) wout[i] = HH(i, i);
-    i = l - 1;
-    __syncthreads();
-  }
-  if (t == 0 && info) info[page] = failed;
-  if (hInLds)
-    for (int e = t; e < n * n; e += nt) Hg[(size_t)(e % n) + (size_t)(e / n) * ldh] = H[e];
-  if (zInLds)
-    for (int e = t; e < n * n; e += nt) Zp[e] = Z[e];
-#undef WN
-#undef HH
 }
 
 // Multi-shift variant: each pass chases up to MB_NB single-shift bulges at
@@ -2234,19 +1928,6 @@ void orf_zgeev(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, c
   orf_zgeev_timed(st, n, A, lda, sA, w, V, ldv, sV, Zwork, Ywork, Hwork, info, batch, OrfTimeHook());
 }
 
-static __global__ void eye_batch_kernel(int n, cplx* Z, long long sZ) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long long)n * n) return;
-  Z[(size_t)blockIdx.y * sZ + e] = cmk((e % n) == (e / n) ? 1.0 : 0.0, 0.0);
-}
-
-static void eye_batch(hipStream_t st, int n, cplx* Z, long long sZ, int batch) {
-  for (int b0 = 0; b0 < batch; b0 += 65535) {
-    const int bb = batch - b0 < 65535 ? batch - b0 : 65535;
-    eye_batch_kernel<<<dim3((unsigned)(((long long)n * n + 255) / 256), bb), 256, 0, st>>>(n, Z + (size_t)b0 * sZ, sZ);
-  }
-}
-
 void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V,
                      int ldv, long long sV, cplx* Zwork, cplx* Ywork, cplx* Hwork, int* info,
                      int batch, const OrfTimeHook& hook) {
@@ -2272,7 +1953,12 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
     hessenberg_blocked_kernel<8><<<batch, 256, pan8, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
   else
     hessenberg_noq_kernel<<<batch, 256, 0, st>>>(n, A, lda, sA, Vh, tau, 0);
-  eye_batch(st, n, Zwork, sZ, batch);
+  // Z = Q of the reduction, so that the QR's accumulated Z is the full Schur basis
+  for (int b0 = 0; b0 < batch; b0 += 65535) {
+    const int bb = batch - b0 < 65535 ? batch - b0 : 65535;
+    hess_q_apply_kernel<<<dim3((n + 15) / 16, bb), 256, (size_t)n * 16 * sizeof(cplx), st>>>(
+        n, Vh + (size_t)b0 * n * n, tau + (size_t)b0 * n, Zwork + (size_t)b0 * sZ, n, sZ, 1);
+  }
   te("eig_hessenberg");
   tb("eig_schur_qr");
   {
@@ -2301,11 +1987,6 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
   trevc_kernel<<<g2, 256, 0, st>>>(n, A, lda, sA, Ywork, n, sZ);
   orf_zgemm(st, 'N', 'N', n, n, n, cmk(1.0, 0.0), Zwork, n, sZ, Ywork, n, sZ, cmk(0.0, 0.0), V,
             ldv, sV, batch);
-  for (int b0 = 0; b0 < batch; b0 += 65535) {
-    const int bb = batch - b0 < 65535 ? batch - b0 : 65535;
-    hess_q_apply_kernel<<<dim3((n + 15) / 16, bb), 256, (size_t)n * 16 * sizeof(cplx), st>>>(
-        n, Vh + (size_t)b0 * n * n, tau + (size_t)b0 * n, V + (size_t)b0 * sV, ldv, sV);
-  }
   colnorm_kernel<<<g2, 256, 0, st>>>(n, V, ldv, sV);
   te("eig_vectors");
 }
