@@ -2,7 +2,7 @@
  * orf_rcwa.h -- C ABI of the MI355X (gfx950) 2D RCWA unit-cell solver.
  *
  * Drop-in boundary for the reference's RCWA path (MoonUsagi/OpticRCWAFDTD,
- * MATLAB).  A MATLAB MEX gateway (mex/orf_rcwa_mex.c) and the Python host
+ * MATLAB).  A MATLAB MEX gateway (opticrcwafdtd_amd/matlab/orf_rcwa_mex.c) and the Python host
  * package (opticrcwafdtd_amd) bind exactly these entry points.
  *
  * Conventions (docs/CONVENTIONS.md:9-19 of the reference): e^{-i w t},
