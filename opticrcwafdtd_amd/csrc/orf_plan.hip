@@ -59,6 +59,8 @@ __global__ void sum_diff_kernel(long long, cplx*, cplx*);
 __global__ void smat_rhs3_kernel(int, const cplx*, const double*, const cplx*, const cplx*, long long, cplx*,
                                  long long, cplx*);
 __global__ void rowscale_kernel(int, int, const cplx*, cplx*, long long);
+__global__ void rowscale_add_kernel(int, const cplx*, const cplx*, long long, double, const cplx*, long long, double,
+                                    cplx*, long long);
 __global__ void eye_axpby_kernel(int, cplx, cplx, const cplx*, long long, cplx*, long long);
 __global__ void gather_pages_kernel(long long, const int*, const cplx*, long long, cplx*, long long);
 __global__ void eye_stack_kernel(int, long long, cplx*);
@@ -555,9 +557,11 @@ static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx
     orf_zgetrs(st, 'L', n, n, Vm, n, n2, w.ipiv, w.dinv, w.X2, n, n2, Cs);  // V \ V0
   }
   sum_diff_kernel<<<g1(n2 * Cs), 256, 0, st>>>(n2 * Cs, w.X1, w.X2);  // X1 = A, X2 = B
-  smat_rhs3_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, lam, w.kL, w.X1, w.X2, n2, w.R3, 3 * n2, w.Xd);
-  HIPCHECK(hipMemcpyAsync(w.XB, w.X2, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-  rowscale_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, n, w.Xd, w.XB, n2);
+  smat_rhs3_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, lam, w.kL, w.X1, w.X2, n2, w.R3, 3 * n2, w.Xd);  // [XB XA B]
+  cplx* XBm = w.R3;
+  cplx* XAm = w.R3 + n2;
+  cplx* F = w.R3 + 2 * n2;
+  // F = B A^-1 (one right solve; X B A^-1 X A = X F X A and B A^-1 B = F B)
   HIPCHECK(hipMemcpyAsync(w.LUb, w.X1, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
   {
     STAGE("smat_lu");
@@ -565,21 +569,18 @@ static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx
   }
   {
     STAGE("smat_solve");
-    orf_zgetrs(st, 'L', n, 3 * n, w.LUb, n, n2, w.ipiv, w.dinv, w.R3, n, 3 * n2, Cs);  // [AiXB AiXA AiB]
+    orf_zgetrs(st, 'R', n, n, w.LUb, n, n2, w.ipiv, w.dinv, F, n, 3 * n2, Cs);
   }
-  cplx* AiXB = w.R3;
-  cplx* AiXA = w.R3 + n2;
-  cplx* AiB = w.R3 + 2 * n2;
-  // D = A - XB AiXB
-  HIPCHECK(hipMemcpyAsync(w.Dm, w.X1, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
   {
     STAGE("smat_gemm");
-    orf_zgemm(st, 'N', 'N', n, n, n, MONE, w.XB, n, n2, AiXB, n, 3 * n2, ONE, w.Dm, n, n2, Cs);
-    // RHS2 = [XB AiXA - B, X (A - B AiB)]
-    eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ZERO, MONE, w.X2, n2, w.RHS2, 2 * n2);
-    orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.XB, n, n2, AiXA, n, 3 * n2, ONE, w.RHS2, n, 2 * n2, Cs);
+    // D = A - X (F X B)
+    orf_zgemm(st, 'N', 'N', n, n, n, ONE, F, n, 3 * n2, XBm, n, 3 * n2, ZERO, w.XB, n, n2, Cs);
+    rowscale_add_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, w.Xd, w.XB, n2, -1.0, w.X1, n2, 1.0, w.Dm, n2);
+    // RHS2 = [X F X A - B, X (A - F B)]
+    orf_zgemm(st, 'N', 'N', n, n, n, ONE, F, n, 3 * n2, XAm, n, 3 * n2, ZERO, w.RHS2, n, 2 * n2, Cs);
+    rowscale_add_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, w.Xd, w.RHS2, 2 * n2, 1.0, w.X2, n2, -1.0, w.RHS2, 2 * n2);
     eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ZERO, ONE, w.X1, n2, w.RHS2 + n2, 2 * n2);
-    orf_zgemm(st, 'N', 'N', n, n, n, MONE, w.X2, n, n2, AiB, n, 3 * n2, ONE, w.RHS2 + n2, n, 2 * n2, Cs);
+    orf_zgemm(st, 'N', 'N', n, n, n, MONE, F, n, 3 * n2, w.X2, n, n2, ONE, w.RHS2 + n2, n, 2 * n2, Cs);
   }
   rowscale_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, n, w.Xd, w.RHS2 + n2, 2 * n2);
   {
@@ -599,23 +600,25 @@ static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx
 }
 
 // Redheffer star (smatrixStar.m:4-10): SA (4 page arrays) * SB -> out (4 page arrays).
+// One LU serves both inverses: with M = I - SB11 SA22, (I - SA22 SB11)^-1 =
+// I + SA22 M^-1 SB11, so D = SA12 M^-1 and F = SB21 + SB21 SA22 (M^-1 SB11).
 static void smatrix_star(orf_plan& P, Work& w, int Cs, cplx* const SA[4], cplx* const SB[4], cplx* const out[4]) {
   const int n = P.nr;
   const long long n2 = (long long)n * n;
   hipStream_t st = P.st;
   STAGE("star");
-  // D = SA12 / (I - SB11 SA22)
   eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.M1, n2);
   bulk_gemm(P, 'N', 'N', n, n, n, MONE, SB[0], n, n2, SA[3], n, n2, ONE, w.M1, n, n2, Cs);
   orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs, w.dinv);
+  // D = SA12 M^-1
   HIPCHECK(hipMemcpyAsync(w.Dm, SA[1], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
   orf_zgetrs(st, 'R', n, n, w.M1, n, n2, w.ipiv, w.dinv, w.Dm, n, n2, Cs);
-  // F = SB21 / (I - SA22 SB11)
-  eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.M1, n2);
-  bulk_gemm(P, 'N', 'N', n, n, n, MONE, SA[3], n, n2, SB[0], n, n2, ONE, w.M1, n, n2, Cs);
-  orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs, w.dinv);
+  // G = M^-1 SB11;  F = SB21 + (SB21 SA22) G
+  HIPCHECK(hipMemcpyAsync(w.Tm, SB[0], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+  orf_zgetrs(st, 'L', n, n, w.M1, n, n2, w.ipiv, w.dinv, w.Tm, n, n2, Cs);
+  bulk_gemm(P, 'N', 'N', n, n, n, ONE, SB[2], n, n2, SA[3], n, n2, ZERO, w.M1, n, n2, Cs);
   HIPCHECK(hipMemcpyAsync(w.Fm, SB[2], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-  orf_zgetrs(st, 'R', n, n, w.M1, n, n2, w.ipiv, w.dinv, w.Fm, n, n2, Cs);
+  bulk_gemm(P, 'N', 'N', n, n, n, ONE, w.M1, n, n2, w.Tm, n, n2, ONE, w.Fm, n, n2, Cs);
   // S11 = SA11 + D SB11 SA21
   bulk_gemm(P, 'N', 'N', n, n, n, ONE, w.Dm, n, n2, SB[0], n, n2, ZERO, w.Tm, n, n2, Cs);
   HIPCHECK(hipMemcpyAsync(out[0], SA[0], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
@@ -631,28 +634,26 @@ static void smatrix_star(orf_plan& P, Work& w, int Cs, cplx* const SA[4], cplx* 
 }
 
 // Last star product S = SA * S_trn restricted to the source columns (rcwa2D
-// needs r = S11 c and t = S21 c only, smatrixStar.m:5-10 applied to vectors):
-// R = SA11 C + SA12 (I - SB11 SA22)^-1 SB11 SA21 C,  T = SB21 (I - SA22 SB11)^-1 SA21 C.
+// needs r = S11 c and t = S21 c only, smatrixStar.m:5-10 applied to vectors).
+// With M = I - SB11 SA22, U = SA21 C and X = M^-1 SB11 U:
+// R = SA11 C + SA12 X,  T = SB21 (I - SA22 SB11)^-1 U = SB21 (U + SA22 X).
 static void smatrix_star_cols(orf_plan& P, Work& w, int Cs, cplx* const SA[4], cplx* const SB[4], int ncol,
                               const int* colIdx, cplx* R, cplx* T) {
   const int n = P.nr;
   const long long n2 = (long long)n * n, nc2 = (long long)n * ncol;
   hipStream_t st = P.st;
   STAGE("star_final");
-  cplx* U = w.Tm;          // SA21 C
-  cplx* Vv = w.Tm + nc2 * Cs;  // SB11 U, then X = M1^-1 V
+  cplx* U = w.Tm;   // SA21 C, then U + SA22 X
+  cplx* Xv = w.Fm;  // SB11 U, then X = M^-1 SB11 U
   select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, SA[2], n2, U, nc2);
   select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, SA[0], n2, R, nc2);
-  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, SB[0], n, n2, U, n, nc2, ZERO, Vv, n, nc2, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, SB[0], n, n2, U, n, nc2, ZERO, Xv, n, nc2, Cs);
   eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.M1, n2);
   orf_zgemm(st, 'N', 'N', n, n, n, MONE, SB[0], n, n2, SA[3], n, n2, ONE, w.M1, n, n2, Cs);
   orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs, w.dinv);
-  orf_zgetrs(st, 'L', n, ncol, w.M1, n, n2, w.ipiv, w.dinv, Vv, n, nc2, Cs);
-  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, SA[1], n, n2, Vv, n, nc2, ONE, R, n, nc2, Cs);
-  eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.M1, n2);
-  orf_zgemm(st, 'N', 'N', n, n, n, MONE, SA[3], n, n2, SB[0], n, n2, ONE, w.M1, n, n2, Cs);
-  orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, Cs, w.dinv);
-  orf_zgetrs(st, 'L', n, ncol, w.M1, n, n2, w.ipiv, w.dinv, U, n, nc2, Cs);
+  orf_zgetrs(st, 'L', n, ncol, w.M1, n, n2, w.ipiv, w.dinv, Xv, n, nc2, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, SA[1], n, n2, Xv, n, nc2, ONE, R, n, nc2, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, SA[3], n, n2, Xv, n, nc2, ONE, U, n, nc2, Cs);
   orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, SB[2], n, n2, U, n, nc2, ZERO, T, n, nc2, Cs);
   KCHECK();
 }

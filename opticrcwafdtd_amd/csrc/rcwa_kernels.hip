@@ -429,6 +429,17 @@ __global__ void smat_rhs3_kernel(int n, const cplx* __restrict__ lam, const doub
   if (e < n) Xd[(size_t)sp * n + e] = cexp(cscale(lam[(size_t)sp * n + e], -kL[sp]));
 }
 
+// out = s1 diag(X) P + s2 C  (n x n per page; out may alias P or C)
+__global__ void rowscale_add_kernel(int n, const cplx* __restrict__ Xd, const cplx* P, long long sP, double s1,
+                                    const cplx* C, long long sC, double s2, cplx* out, long long sO) {
+  const int sp = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n) return;
+  const int r = e % n;
+  const cplx xp = cmul(Xd[(size_t)sp * n + r], P[(size_t)sp * sP + e]);
+  out[(size_t)sp * sO + e] = cadd(cscale(xp, s1), cscale(C[(size_t)sp * sC + e], s2));
+}
+
 // Y = diag(X) * Y (rows scaled)
 __global__ void rowscale_kernel(int n, int ncol, const cplx* __restrict__ Xd, cplx* __restrict__ Y,
                                 long long sY) {
