@@ -97,6 +97,7 @@ typedef struct orf_problem {
   int32_t nPages;               /* explicit (structure, wavelength) pages; 0: all pairs */
   const int32_t* pageStruct;    /* [nPages] structure index                     */
   const int32_t* pageLambda;    /* [nPages] wavelength index                    */
+  int32_t keepModes;            /* 1: keep every layer's modes and S-matrix (one page only; orf_fields) */
 } orf_problem;
 
 /* Results, column-major like the reference's out struct (rcwa2D.m:31-34),
@@ -146,6 +147,17 @@ int orf_plan_info(const orf_plan* plan, int32_t* solved_size, int32_t* symmetric
 int orf_plan_stage_times(const orf_plan* plan, int32_t max_stages, const char** names,
                          double* total_ms, int32_t* launches, int32_t* n_stages);
 void orf_plan_enable_timing(orf_plan* plan, int32_t on);
+
+/* ------------------------------------------------------------------ fields
+ * orf.rcwaFields(dev, lambda0, theta, phi, pol, x, y, z) -- rcwaFields.m:1-21.
+ * One structure, one wavelength, one source (nStruct = nLambda = nPol = 1),
+ * solved in the full basis.  E and H~ = i eta0 H are complex
+ * [nx x ny x nz x 3] (x fastest, components x, y, z), flux [nz] is the power
+ * through the unit cell normalised to the incident flux; out receives the
+ * rcwa2D-style result (may have NULL members).  z = 0 is the top of the
+ * first layer. */
+int orf_fields(orf_ctx* ctx, const orf_problem* p, int32_t nx, const double* x, int32_t ny, const double* y,
+               int32_t nz, const double* z, double* E, double* H, double* flux, orf_result* out);
 
 /* ------------------------------------------------------------------ memory (build extension)
  * Admission by device footprint for request loops.  orf_plan_memory: bytes a

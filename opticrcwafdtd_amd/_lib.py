@@ -50,6 +50,7 @@ class orf_problem(C.Structure):
         ("precision", C.c_int32), ("batchSize", C.c_int32),
         ("structPeriod", C.POINTER(C.c_double)), ("nPages", C.c_int32),
         ("pageStruct", C.POINTER(C.c_int32)), ("pageLambda", C.POINTER(C.c_int32)),
+        ("keepModes", C.c_int32),
     ]
 
 
@@ -66,7 +67,7 @@ EXPORTS = [
     "orf_init", "orf_release", "orf_last_error", "orf_harmonic_count", "orf_rcwa2d",
     "orf_plan_create", "orf_plan_execute", "orf_plan_fetch", "orf_plan_destroy", "orf_plan_info",
     "orf_plan_stage_times", "orf_plan_enable_timing", "orf_plan_memory", "orf_plan_estimate",
-    "orf_release_cached_memory",
+    "orf_release_cached_memory", "orf_fields",
     "orf_pagemtimes", "orf_pagemtimes_f32", "orf_pagemldivide", "orf_pagemrdivide", "orf_pageeig",
 ]
 
@@ -113,6 +114,10 @@ def load_library():
     lib.orf_plan_memory.restype = C.c_int
     lib.orf_plan_estimate.argtypes = [P(orf_problem), C.c_int32, P(C.c_int64)]
     lib.orf_plan_estimate.restype = C.c_int
+    lib.orf_fields.argtypes = [C.c_void_p, P(orf_problem), C.c_int32, P(C.c_double), C.c_int32, P(C.c_double),
+                               C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double),
+                               P(orf_result)]
+    lib.orf_fields.restype = C.c_int
     lib.orf_release_cached_memory.argtypes = []
     lib.orf_release_cached_memory.restype = None
     lib.orf_pagemtimes.argtypes = [C.c_void_p, C.c_char, C.c_char, C.c_int32, C.c_int32, C.c_int32,

@@ -65,6 +65,14 @@ __global__ void eye_axpby_kernel(int, cplx, cplx, const cplx*, long long, cplx*,
 __global__ void gather_pages_kernel(long long, const int*, const cplx*, long long, cplx*, long long);
 __global__ void eye_stack_kernel(int, long long, cplx*);
 __global__ void select_cols_kernel(int, int, const int*, const cplx*, long long, cplx*, long long);
+__global__ void field_amp_kernel(int, int, const cplx*, const cplx*, const cplx*, double, double, const double*, cplx*,
+                                 cplx*);
+__global__ void curl_z_kernel(int, int, const double*, const double*, const cplx*, cplx, cplx*);
+__global__ void scatter_cols_kernel(int, int, const int*, const cplx*, int, cplx*, int);
+__global__ void phase_kernel(int, int, int, const double*, const double*, const double*, const double*, double, cplx*);
+__global__ void flux_kernel(int, const cplx*, const cplx*, double, double*);
+__global__ void copy_cols_kernel(long long, const cplx*, cplx*);
+__global__ void scale_kernel(long long, cplx, cplx*);
 __global__ void output_kernel(int, int, int, int, int, const DevSolve*, const DevPage*, const DevSector*,
                               const DevSource*, const double*, const double*, const double*, const cplx*,
                               const cplx*, long long, double*, double*, double*, double*, cplx*, cplx*, int);
@@ -385,6 +393,15 @@ struct orf_plan {
   std::vector<DBuf*> ws;
   Work work{};
   bool ownStream = false;
+  // modes and S-matrix of every layer of the single page, kept for field reconstruction
+  struct Kept {
+    DBuf W, V, lam, E, S11, S12;
+    bool patterned = false;
+  };
+  bool keepModes = false;
+  std::vector<Kept*> kept;
+  double kzi0 = 1.0;
+  zc epsInc0 = 1.0, epsSub0 = 1.0;  // half-space permittivities of context 0
   StageTimer timer;
   ~orf_plan() {
     if (st) (void)hipStreamSynchronize(st);  // no block returns to the pool while in use
@@ -393,6 +410,7 @@ struct orf_plan {
     for (auto* b : dSlotInv) delete b;
     for (auto* b : secArrays) delete b;
     for (auto* b : ws) delete b;
+    for (auto* k : kept) delete k;
     if (ownStream && st) (void)hipStreamDestroy(st);
   }
 };
@@ -857,6 +875,11 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
     if (std::fabs(ninc.imag()) > 1e-12) fail(ORF_LOSSY_INCIDENT, "Incident medium must be lossless.");
     const double nInc = ninc.real();
     kzi0[c] = nInc * ct;
+    if (c == 0) {
+      P.kzi0 = kzi0[0];
+      P.epsInc0 = eInc[0];
+      P.epsSub0 = eSub[0];
+    }
     double kmax = 0;
     for (int p = 0; p < P.M; ++p) {
       const double kx = nInc * khat[0] + P.mIdx[p] * lm / ctxLx[c];
@@ -962,7 +985,9 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   }
   // symmetry decision (rcwa2D.m:24-27): theta = 0, pol "s"/"p", mirror-symmetric grids
   P.reduced = 0;
-  if (pr->symmetry == ORF_SYM_AUTO && P.theta == 0.0 && allSP) {
+  P.keepModes = pr->keepModes != 0;
+  if (P.keepModes && P.nPages != 1) fail(ORF_INVALID_ARG, "keepModes needs exactly one page");
+  if (pr->symmetry == ORF_SYM_AUTO && P.theta == 0.0 && allSP && !P.keepModes) {
     int sym = 1;
     if (P.nGrids > 0) {
       DBuf flags;
@@ -1123,6 +1148,8 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
   const int Cmax = P.chunkPages;
   Work& w = P.work;
   if (P.timer.on) P.timer.reset();
+  if (P.keepModes && P.kept.empty())
+    for (int j = 0; j < P.nLayers; ++j) P.kept.push_back(new orf_plan::Kept());
   const int nChunks = (npages + Cmax - 1) / Cmax;
   const int M = P.M, nf = P.nf, nr = P.nr, nSec = P.nSec;
   const long long M2 = (long long)M * M, nf2 = (long long)nf * nf, n2 = (long long)nr * nr;
@@ -1154,6 +1181,11 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         {
           STAGE("conv");
           laurent_kernel<<<g1(M2, Cp), 256, 0, st>>>(P.Nx, P.Ny, M, pages, P.dChi2.as<cplx>(), se, P.nslotMax, w.E, M2);
+          if (P.keepModes) {
+            P.kept[j]->E.alloc(sizeof(cplx) * M2);
+            P.kept[j]->patterned = true;
+            HIPCHECK(hipMemcpyAsync(P.kept[j]->E.p, w.E, sizeof(cplx) * M2, hipMemcpyDeviceToDevice, st));
+          }
           if (P.factorization == ORF_FACT_LI) {
             // Exx: inverse rule along x at each y-sample, Laurent along y
             const long long perx = (long long)Wx * Wx * P.Nsy;
@@ -1255,7 +1287,23 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
       }
       HIPCHECK(hipMemcpyAsync(w.kL, kL.data(), sizeof(double) * Cs, hipMemcpyHostToDevice, st));
       gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dV0.as<cplx>(), n2, w.X2, n2);
+      if (P.keepModes) {
+        auto* K = P.kept[j];
+        K->W.alloc(sizeof(cplx) * n2);
+        K->V.alloc(sizeof(cplx) * n2);
+        K->lam.alloc(sizeof(cplx) * nr);
+        HIPCHECK(hipMemcpyAsync(K->W.p, w.Wm, sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
+        HIPCHECK(hipMemcpyAsync(K->V.p, w.Vm, sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
+        HIPCHECK(hipMemcpyAsync(K->lam.p, w.lam, sizeof(cplx) * nr, hipMemcpyDeviceToDevice, st));
+      }
       smatrix_layer(P, w, Cs, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1]);
+      if (P.keepModes) {
+        auto* K = P.kept[j];
+        K->S11.alloc(sizeof(cplx) * n2);
+        K->S12.alloc(sizeof(cplx) * n2);
+        HIPCHECK(hipMemcpyAsync(K->S11.p, w.Sb[0], sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
+        HIPCHECK(hipMemcpyAsync(K->S12.p, w.Sb[1], sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
+      }
       cplx* SB[4] = {w.Sb[0], w.Sb[1], w.Sb[1], w.Sb[0]};
       smatrix_star(P, w, Cs, w.Sa, SB, w.Sn);
       for (int k = 0; k < 4; ++k) std::swap(w.Sa[k], w.Sn[k]);
@@ -1315,6 +1363,233 @@ static void plan_fetch(orf_plan& P, orf_result* out) {
   if (out->Ttot) HIPCHECK(hipMemcpy(out->Ttot, P.dTtot.p, sizeof(double) * nout, hipMemcpyDeviceToHost));
   if (out->r) HIPCHECK(hipMemcpy(out->r, P.dr.p, sizeof(cplx) * 3 * P.M * nout, hipMemcpyDeviceToHost));
   if (out->t) HIPCHECK(hipMemcpy(out->t, P.dt.p, sizeof(cplx) * 3 * P.M * nout, hipMemcpyDeviceToHost));
+}
+
+// ---------------------------------------------------------------- fields (rcwaFields.m:1-21)
+// Every region is expanded in its own modes: s = [Ex; Ey] = W (fp a+ + fm a-),
+// u = [H~x; H~y] = V (-fp a+ + fm a-), fp = e^{-lam k0 z}, fm = e^{-lam k0 (d - z)},
+// z from the top of the region (the convention of smatrixLayer.m:6-18).  The
+// amplitudes come from the zero-thickness gaps between layers: with S_up the
+// stack above a gap and S_down the stack below, c+ = (I - S_up22 S_down11)^-1
+// S_up21 c_inc and c- = S_down11 c+; a layer's a+ follows from the gap above
+// and a- from the gap below, so no growing exponential is formed.
+// Ez = i [eps]^-1 (Kx H~y - Ky H~x) (pqMatrices.m:4), H~z = i (Kx Ey - Ky Ex);
+// the (x, y) synthesis is one ZGEMM with the phase matrix e^{i k0 (kx x + ky y)}.
+static void plan_fields(orf_plan& P, int nx, const double* x, int ny, const double* y, int nz, const double* z,
+                        double* Eout, double* Hout, double* fluxOut) {
+  if (!P.keepModes || !P.haveResult) fail(ORF_INVALID_ARG, "fields need an executed plan with keepModes");
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !x || !y || !z) fail(ORF_INVALID_ARG, "empty field grid");
+  const int M = P.M, n = P.nf, nL = P.nLayers;
+  const long long n2 = (long long)n * n;
+  hipStream_t st = P.st;
+  Work& w = P.work;
+  const double k0 = P.k0[0];
+  const double* kx = P.dKxy.as<double>();
+  const double* ky = kx + M;
+  // half-space modes (V, lam) of the incident and substrate media
+  DBuf dsv, dpg, deps, dWr, dVr, dLr;
+  dsv.upload(std::vector<DevSolve>{DevSolve{0, 0}, DevSolve{0, 0}});
+  dpg.upload(std::vector<DevPage>{DevPage{0, 0, -1, 0}});
+  deps.upload(std::vector<cplx>{C(P.epsInc0), C(P.epsSub0)});
+  dWr.alloc(sizeof(cplx) * n2 * 2);
+  dVr.alloc(sizeof(cplx) * n2 * 2);
+  dLr.alloc(sizeof(cplx) * n * 2);
+  homog_modes_kernel<<<g1(n2, 2), 256, 0, st>>>(M, n, 0, dsv.as<DevSolve>(), dpg.as<DevPage>(),
+                                                P.dSectors.as<DevSector>(), kx, deps.as<cplx>(), dWr.as<cplx>(),
+                                                dVr.as<cplx>(), n2, dLr.as<cplx>());
+  KCHECK();
+  // prefix (up) and suffix (down) stacks around every gap
+  DBuf chain;
+  chain.alloc(sizeof(cplx) * n2 * 8 * (nL + 1));
+  auto blk = [&](int which, int j, int k) { return chain.as<cplx>() + ((size_t)(which * (nL + 1) + j) * 4 + k) * n2; };
+  for (int k = 0; k < 4; ++k) {
+    HIPCHECK(hipMemcpyAsync(blk(0, 0, k), P.dSref.as<cplx>() + k * n2, sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(blk(1, nL, k), P.dStrn.as<cplx>() + k * n2, sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
+  }
+  for (int j = 0; j < nL; ++j) {
+    cplx* SA[4] = {blk(0, j, 0), blk(0, j, 1), blk(0, j, 2), blk(0, j, 3)};
+    cplx* SL[4] = {P.kept[j]->S11.as<cplx>(), P.kept[j]->S12.as<cplx>(), P.kept[j]->S12.as<cplx>(),
+                   P.kept[j]->S11.as<cplx>()};
+    cplx* O[4] = {blk(0, j + 1, 0), blk(0, j + 1, 1), blk(0, j + 1, 2), blk(0, j + 1, 3)};
+    smatrix_star(P, w, 1, SA, SL, O);
+  }
+  for (int j = nL - 1; j >= 0; --j) {
+    cplx* SL[4] = {P.kept[j]->S11.as<cplx>(), P.kept[j]->S12.as<cplx>(), P.kept[j]->S12.as<cplx>(),
+                   P.kept[j]->S11.as<cplx>()};
+    cplx* SB[4] = {blk(1, j + 1, 0), blk(1, j + 1, 1), blk(1, j + 1, 2), blk(1, j + 1, 3)};
+    cplx* O[4] = {blk(1, j, 0), blk(1, j, 1), blk(1, j, 2), blk(1, j, 3)};
+    smatrix_star(P, w, 1, SL, SB, O);
+  }
+  // gap amplitudes: c+ = (I - up22 down11)^-1 up21 c_inc, c- = down11 c+
+  std::vector<cplx> cinc(n, C(0));
+  cinc[P.i0] = P.sources[0].ex;
+  cinc[M + P.i0] = P.sources[0].ey;
+  DBuf dcinc, gap;
+  dcinc.upload(cinc);
+  gap.alloc(sizeof(cplx) * n * 2 * (nL + 1));  // [c+ | c-] per gap
+  for (int j = 0; j <= nL; ++j) {
+    cplx* cp = gap.as<cplx>() + (size_t)2 * j * n;
+    cplx* cm = cp + n;
+    eye_axpby_kernel<<<g1(n2, 1), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.M1, n2);
+    orf_zgemm(st, 'N', 'N', n, n, n, MONE, blk(0, j, 3), n, n2, blk(1, j, 0), n, n2, ONE, w.M1, n, n2, 1);
+    orf_zgetrf(st, n, w.M1, n, n2, w.ipiv, nullptr, 1, w.dinv);
+    orf_zgemm(st, 'N', 'N', n, 1, n, ONE, blk(0, j, 2), n, n2, dcinc.as<cplx>(), n, n, ZERO, cp, n, n, 1);
+    orf_zgetrs(st, 'L', n, 1, w.M1, n, n2, w.ipiv, w.dinv, cp, n, n, 1);
+    orf_zgemm(st, 'N', 'N', n, 1, n, ONE, blk(1, j, 0), n, n2, cp, n, n, ZERO, cm, n, n, 1);
+  }
+  // reflected and transmitted amplitudes: c_ref = Sref11 c_inc + Sref12 c-_0, c_trn = Strn21 c+_nL
+  DBuf cref, ctrn;
+  cref.alloc(sizeof(cplx) * n);
+  ctrn.alloc(sizeof(cplx) * n);
+  orf_zgemm(st, 'N', 'N', n, 1, n, ONE, P.dSref.as<cplx>(), n, n2, dcinc.as<cplx>(), n, n, ZERO, cref.as<cplx>(), n, n, 1);
+  orf_zgemm(st, 'N', 'N', n, 1, n, ONE, P.dSref.as<cplx>() + n2, n, n2, gap.as<cplx>() + n, n, n, ONE, cref.as<cplx>(), n,
+            n, 1);
+  orf_zgemm(st, 'N', 'N', n, 1, n, ONE, P.dStrn.as<cplx>() + 2 * n2, n, n2, gap.as<cplx>() + (size_t)2 * nL * n, n, n, ZERO,
+            ctrn.as<cplx>(), n, n, 1);
+  // layer amplitudes: a+ = (W^-1 (c+ + c-) + V^-1 V0 (c+ - c-)) / 2 from the gap above,
+  //                   a- = (W^-1 (d+ + d-) + V^-1 V0 (d- - d+)) / 2 from the gap below
+  std::vector<cplx> hgap((size_t)2 * n * (nL + 1));
+  HIPCHECK(hipMemcpyAsync(hgap.data(), gap.p, sizeof(cplx) * hgap.size(), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  auto Z = [](cplx a) { return zc(a.x, a.y); };
+  DBuf amps, rhs, rhs2;
+  amps.alloc(sizeof(cplx) * n * 2 * std::max(nL, 1));  // [a+ | a-] per layer
+  rhs.alloc(sizeof(cplx) * n * 2);
+  rhs2.alloc(sizeof(cplx) * n * 2);
+  for (int j = 0; j < nL; ++j) {
+    const cplx* cp = hgap.data() + (size_t)2 * j * n;
+    const cplx* cm = cp + n;
+    const cplx* dp = hgap.data() + (size_t)2 * (j + 1) * n;
+    const cplx* dm = dp + n;
+    std::vector<cplx> sum(2 * n), dif(2 * n);
+    for (int r = 0; r < n; ++r) {
+      sum[r] = C(Z(cp[r]) + Z(cm[r]));
+      sum[n + r] = C(Z(dp[r]) + Z(dm[r]));
+      dif[r] = C(Z(cp[r]) - Z(cm[r]));
+      dif[n + r] = C(Z(dm[r]) - Z(dp[r]));
+    }
+    HIPCHECK(hipMemcpyAsync(rhs.p, sum.data(), sizeof(cplx) * 2 * n, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(w.LUb, P.kept[j]->W.p, sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
+    orf_zgetrf(st, n, w.LUb, n, n2, w.ipiv, nullptr, 1, w.dinv);
+    orf_zgetrs(st, 'L', n, 2, w.LUb, n, n2, w.ipiv, w.dinv, rhs.as<cplx>(), n, 2 * n, 1);
+    HIPCHECK(hipMemcpyAsync(w.Dm, dif.data(), sizeof(cplx) * 2 * n, hipMemcpyHostToDevice, st));
+    orf_zgemm(st, 'N', 'N', n, 2, n, ONE, P.dV0.as<cplx>(), n, n2, w.Dm, n, 2 * n, ZERO, rhs2.as<cplx>(), n, 2 * n, 1);
+    HIPCHECK(hipMemcpyAsync(w.LUb, P.kept[j]->V.p, sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
+    orf_zgetrf(st, n, w.LUb, n, n2, w.ipiv, nullptr, 1, w.dinv);
+    orf_zgetrs(st, 'L', n, 2, w.LUb, n, n2, w.ipiv, w.dinv, rhs2.as<cplx>(), n, 2 * n, 1);
+    std::vector<cplx> h1(2 * n), h2(2 * n), a(2 * n);
+    HIPCHECK(hipMemcpyAsync(h1.data(), rhs.p, sizeof(cplx) * 2 * n, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(h2.data(), rhs2.p, sizeof(cplx) * 2 * n, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    for (int r = 0; r < 2 * n; ++r) a[r] = C(0.5 * (Z(h1[r]) + Z(h2[r])));
+    HIPCHECK(hipMemcpyAsync(amps.as<cplx>() + (size_t)2 * j * n, a.data(), sizeof(cplx) * 2 * n, hipMemcpyHostToDevice,
+                            st));
+    HIPCHECK(hipStreamSynchronize(st));
+  }
+  // regions of the field points: 0 incident, 1 + j layer j, nL + 1 substrate
+  std::vector<double> ztop(nL + 1, 0.0);
+  for (int j = 0; j < nL; ++j) ztop[j + 1] = ztop[j] + P.thick[(size_t)j * P.B + P.pageB[0]];
+  std::vector<std::vector<int>> idx(nL + 2);
+  std::vector<std::vector<double>> zl(nL + 2);
+  for (int k = 0; k < nz; ++k) {
+    int reg;
+    double zz;
+    if (z[k] < 0) { reg = 0; zz = z[k]; }
+    else if (z[k] >= ztop[nL]) { reg = nL + 1; zz = z[k] - ztop[nL]; }
+    else {
+      const int j = (int)(std::upper_bound(ztop.begin(), ztop.end(), z[k]) - ztop.begin()) - 1;
+      reg = 1 + j;
+      zz = z[k] - ztop[j];
+    }
+    idx[reg].push_back(k);
+    zl[reg].push_back(zz);
+  }
+  DBuf Sv, Uv, Ez, Pp, Pm, Tq, dz, di;
+  Sv.alloc(sizeof(cplx) * n * nz);
+  Uv.alloc(sizeof(cplx) * n * nz);
+  Ez.alloc(sizeof(cplx) * M * nz);
+  Pp.alloc(sizeof(cplx) * n * nz);
+  Pm.alloc(sizeof(cplx) * n * nz);
+  Tq.alloc(sizeof(cplx) * n * nz);
+  const cplx I1 = {0.0, 1.0};
+  for (int reg = 0; reg < nL + 2; ++reg) {
+    const int cnt = (int)idx[reg].size();
+    if (!cnt) continue;
+    dz.upload(zl[reg]);
+    di.upload(idx[reg]);
+    const cplx *lam, *ap, *am, *Wr = nullptr, *Vr;
+    double d = 0.0;
+    zc epsH = 0.0;
+    const cplx* Econv = nullptr;
+    if (reg == 0) {
+      lam = dLr.as<cplx>(); ap = dcinc.as<cplx>(); am = cref.as<cplx>(); Vr = dVr.as<cplx>(); epsH = P.epsInc0;
+    } else if (reg == nL + 1) {
+      lam = dLr.as<cplx>() + n; ap = ctrn.as<cplx>(); am = nullptr; Vr = dVr.as<cplx>() + n2; epsH = P.epsSub0;
+    } else {
+      const int j = reg - 1;
+      auto* K = P.kept[j];
+      lam = K->lam.as<cplx>(); ap = amps.as<cplx>() + (size_t)2 * j * n; am = ap + n;
+      Wr = K->W.as<cplx>(); Vr = K->V.as<cplx>();
+      d = ztop[j + 1] - ztop[j];
+      if (K->patterned) Econv = K->E.as<cplx>();
+      else epsH = P.layerBgEps[j][0];
+    }
+    field_amp_kernel<<<g1((long long)n * cnt), 256, 0, st>>>(n, cnt, lam, ap, am, k0, d, dz.as<double>(), Pp.as<cplx>(),
+                                                             Pm.as<cplx>());
+    // s = W Pp (W = I in the half-spaces), u = V Pm
+    const cplx* sv = Pp.as<cplx>();
+    if (Wr) {
+      orf_zgemm(st, 'N', 'N', n, cnt, n, ONE, Wr, n, 0, Pp.as<cplx>(), n, 0, ZERO, Tq.as<cplx>(), n, 0, 1);
+      sv = Tq.as<cplx>();
+    }
+    scatter_cols_kernel<<<g1((long long)n * cnt), 256, 0, st>>>(n, cnt, di.as<int>(), sv, n, Sv.as<cplx>(), n);
+    orf_zgemm(st, 'N', 'N', n, cnt, n, ONE, Vr, n, 0, Pm.as<cplx>(), n, 0, ZERO, Tq.as<cplx>(), n, 0, 1);
+    scatter_cols_kernel<<<g1((long long)n * cnt), 256, 0, st>>>(n, cnt, di.as<int>(), Tq.as<cplx>(), n, Uv.as<cplx>(), n);
+    // Ez = i [eps]^-1 (Kx H~y - Ky H~x)
+    curl_z_kernel<<<g1((long long)M * cnt), 256, 0, st>>>(M, cnt, kx, ky, Tq.as<cplx>(), I1, Pm.as<cplx>());
+    if (Econv) {
+      HIPCHECK(hipMemcpyAsync(w.LUb, Econv, sizeof(cplx) * M * M, hipMemcpyDeviceToDevice, st));
+      orf_zgetrf(st, M, w.LUb, M, (long long)M * M, w.ipiv, nullptr, 1, w.dinv);
+      orf_zgetrs(st, 'L', M, cnt, w.LUb, M, (long long)M * M, w.ipiv, w.dinv, Pm.as<cplx>(), M, (long long)M * cnt, 1);
+    } else {
+      scale_kernel<<<g1((long long)M * cnt), 256, 0, st>>>((long long)M * cnt, C(1.0 / epsH), Pm.as<cplx>());
+    }
+    scatter_cols_kernel<<<g1((long long)M * cnt), 256, 0, st>>>(M, cnt, di.as<int>(), Pm.as<cplx>(), M, Ez.as<cplx>(), M);
+    KCHECK();
+    HIPCHECK(hipStreamSynchronize(st));  // dz / di are reused by the next region
+  }
+  // coefficient matrix C (M x 6 nz): columns c * nz + k for Ex, Ey, Ez, H~x, H~y, H~z
+  DBuf Cf, Phi, F, dx, dy, seq;
+  Cf.alloc(sizeof(cplx) * M * 6 * nz);
+  std::vector<int> ks(nz);
+  for (int k = 0; k < nz; ++k) ks[k] = k;
+  seq.upload(ks);
+  const long long Mz = (long long)M * nz;
+  scatter_cols_kernel<<<g1(Mz), 256, 0, st>>>(M, nz, seq.as<int>(), Sv.as<cplx>(), n, Cf.as<cplx>() + 0 * Mz, M);
+  scatter_cols_kernel<<<g1(Mz), 256, 0, st>>>(M, nz, seq.as<int>(), Sv.as<cplx>() + M, n, Cf.as<cplx>() + 1 * Mz, M);
+  copy_cols_kernel<<<g1(Mz), 256, 0, st>>>(Mz, Ez.as<cplx>(), Cf.as<cplx>() + 2 * Mz);
+  scatter_cols_kernel<<<g1(Mz), 256, 0, st>>>(M, nz, seq.as<int>(), Uv.as<cplx>(), n, Cf.as<cplx>() + 3 * Mz, M);
+  scatter_cols_kernel<<<g1(Mz), 256, 0, st>>>(M, nz, seq.as<int>(), Uv.as<cplx>() + M, n, Cf.as<cplx>() + 4 * Mz, M);
+  curl_z_kernel<<<g1(Mz), 256, 0, st>>>(M, nz, kx, ky, Sv.as<cplx>(), I1, Cf.as<cplx>() + 5 * Mz);  // H~z
+  // synthesis on the grid
+  const long long nxy = (long long)nx * ny;
+  dx.upload(std::vector<double>(x, x + nx));
+  dy.upload(std::vector<double>(y, y + ny));
+  Phi.alloc(sizeof(cplx) * nxy * M);
+  phase_kernel<<<g1(nxy * M), 256, 0, st>>>(M, nx, ny, dx.as<double>(), dy.as<double>(), kx, ky, k0, Phi.as<cplx>());
+  F.alloc(sizeof(cplx) * nxy * 6 * nz);
+  orf_zgemm(st, 'N', 'N', (int)nxy, 6 * nz, M, ONE, Phi.as<cplx>(), (int)nxy, 0, Cf.as<cplx>(), M, 0, ZERO,
+            F.as<cplx>(), (int)nxy, 0, 1);
+  // flux through the cell at every z
+  DBuf fl;
+  fl.alloc(sizeof(double) * nz);
+  flux_kernel<<<nz, 256, 0, st>>>(M, Sv.as<cplx>(), Uv.as<cplx>(), 1.0 / P.kzi0, fl.as<double>());
+  KCHECK();
+  const size_t half = sizeof(cplx) * nxy * 3 * nz;
+  if (Eout) HIPCHECK(hipMemcpyAsync(Eout, F.p, half, hipMemcpyDeviceToHost, st));
+  if (Hout) HIPCHECK(hipMemcpyAsync(Hout, (char*)F.p + half, half, hipMemcpyDeviceToHost, st));
+  if (fluxOut) HIPCHECK(hipMemcpyAsync(fluxOut, fl.p, sizeof(double) * nz, hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
 }
 
 // ====================================================================== C ABI
@@ -1456,6 +1731,35 @@ int orf_plan_estimate(const orf_problem* p, int32_t reduced, int64_t* bytes) {
 }
 
 void orf_release_cached_memory(void) { g_pool.trim(); }
+
+int orf_fields(orf_ctx* ctx, const orf_problem* p, int32_t nx, const double* x, int32_t ny, const double* y,
+               int32_t nz, const double* z, double* E, double* H, double* flux, orf_result* out) {
+  orf_plan* P = nullptr;
+  try {
+    if (!ctx || !p) fail(ORF_INVALID_ARG, "ctx or problem is NULL");
+    if (p->nStruct != 1 || p->nLambda != 1 || p->nPol != 1 || p->nPages > 1)
+      fail(ORF_INVALID_ARG, "fields take one structure, one wavelength and one source");
+    HIPCHECK(hipSetDevice(ctx->device));
+    orf_problem q = *p;
+    q.keepModes = 1;
+    q.nPages = 0;
+    P = new orf_plan();
+    plan_create(ctx, &q, *P);
+    plan_execute(*P, nullptr, nullptr);
+    if (out) plan_fetch(*P, out);
+    plan_fields(*P, nx, x, ny, y, nz, z, E, H, flux);
+    delete P;
+    return ORF_OK;
+  } catch (const OrfError& e) {
+    delete P;
+    g_err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    delete P;
+    g_err = e.what();
+    return ORF_DEVICE_ERROR;
+  }
+}
 
 void orf_plan_enable_timing(orf_plan* plan, int32_t on) {
   if (plan) plan->timer.on = on != 0;

@@ -566,3 +566,84 @@ __global__ void select_cols_kernel(int n, int ncol, const int* __restrict__ colI
   const int r = e % n, q = e / n;
   out[(size_t)sp * sO + e] = S[(size_t)sp * sS + r + (size_t)n * colIdx[sp * ncol + q]];
 }
+
+// ====================================================================== fields (rcwaFields.m)
+// Mode amplitudes at depths zl[k] inside one region:
+//   Pp[:, k] = fp .* ap + fm .* am,  Pm[:, k] = -fp .* ap + fm .* am,
+//   fp = exp(-lam k0 zl), fm = exp(-lam k0 (d - zl))   (am may be null)
+__global__ void field_amp_kernel(int n, int nz, const cplx* __restrict__ lam, const cplx* __restrict__ ap,
+                                 const cplx* __restrict__ am, double k0, double d, const double* __restrict__ zl,
+                                 cplx* __restrict__ Pp, cplx* __restrict__ Pm) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * nz) return;
+  const int r = e % n, k = e / n;
+  const cplx l = lam[r];
+  const cplx fp = cexp(cscale(l, -k0 * zl[k]));
+  const cplx a = cmul(fp, ap[r]);
+  cplx b = cmk(0.0, 0.0);
+  if (am) b = cmul(cexp(cscale(l, -k0 * (d - zl[k]))), am[r]);
+  Pp[e] = cadd(a, b);
+  Pm[e] = csub(b, a);
+}
+
+// out[:, k] = fac (Kx v_y - Ky v_x) for the 2M-row columns v of Vv (z components of the curls)
+__global__ void curl_z_kernel(int M, int nz, const double* __restrict__ kx, const double* __restrict__ ky,
+                              const cplx* __restrict__ Vv, cplx fac, cplx* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M * nz) return;
+  const int p = e % M, k = e / M;
+  const cplx* v = Vv + (size_t)k * 2 * M;
+  out[e] = cmul(fac, csub(cscale(v[M + p], kx[p]), cscale(v[p], ky[p])));
+}
+
+// C(:, col[k]) = src(:, k), n rows
+__global__ void scatter_cols_kernel(int n, int ncol, const int* __restrict__ col, const cplx* __restrict__ src,
+                                    int lds, cplx* __restrict__ dst, int ldd) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * ncol) return;
+  const int r = e % n, k = e / n;
+  dst[(size_t)r + (size_t)col[k] * ldd] = src[(size_t)r + (size_t)k * lds];
+}
+
+// Phi[i + nx j, p] = exp(i k0 (kx_p x_i + ky_p y_j))
+__global__ void phase_kernel(int M, int nx, int ny, const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ kx, const double* __restrict__ ky, double k0,
+                             cplx* __restrict__ Phi) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nxy = (long long)nx * ny;
+  if (e >= nxy * M) return;
+  const int pt = (int)(e % nxy), p = (int)(e / nxy);
+  const int i = pt % nx, j = pt / nx;
+  double sn, cs;
+  sincos(k0 * (kx[p] * x[i] + ky[p] * y[j]), &sn, &cs);
+  Phi[e] = cmk(cs, sn);
+}
+
+// flux[k] = -Im sum_p (Ex_p conj(H~y_p) - Ey_p conj(H~x_p)) / kzi0, from the 2M-row columns of S and U
+__global__ void flux_kernel(int M, const cplx* __restrict__ S, const cplx* __restrict__ U, double inv_kzi0,
+                            double* __restrict__ flux) {
+  __shared__ double red[8];
+  const int k = blockIdx.x;
+  const cplx* s = S + (size_t)k * 2 * M;
+  const cplx* u = U + (size_t)k * 2 * M;
+  double acc = 0.0;
+  for (int p = threadIdx.x; p < M; p += blockDim.x) {
+    const cplx a = cmulc(u[M + p], s[p]);   // Ex conj(Hy)
+    const cplx b = cmulc(u[p], s[M + p]);   // Ey conj(Hx)
+    acc -= a.y - b.y;
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) flux[k] = acc * inv_kzi0;
+}
+
+// s = W c for W = I or a diagonal-free copy: out = c (used for the half-spaces, W = I)
+__global__ void copy_cols_kernel(long long count, const cplx* __restrict__ src, cplx* __restrict__ dst) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < count) dst[e] = src[e];
+}
+
+// columns divided by a scalar permittivity (homogeneous [eps]^-1)
+__global__ void scale_kernel(long long count, cplx f, cplx* __restrict__ v) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < count) v[e] = cmul(f, v[e]);
+}

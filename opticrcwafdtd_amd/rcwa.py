@@ -22,7 +22,7 @@ from typing import Callable, Optional, Sequence
 import numpy as np
 
 from . import _lib
-from .structure import OrfError, alignedSampleCount, default_sample_counts, material_index
+from .structure import OrfError, alignedSampleCount, default_sample_counts, layerEps, material_index
 
 _SHAPE_TYPES = {"circle": 0, "rect": 1, "polygon": 2}
 
@@ -329,6 +329,59 @@ def rcwa2D(devs, lambda0, theta, phi, pol, Nx=5, Ny=None, Nsample=0, Factorizati
         return plan.fetch()
     finally:
         plan.close()
+
+
+def rcwaFields(dev, lambda0, theta, phi, pol, x, y, z, Nx=5, Ny=None, Nsample=0, Factorization="li",
+               GapEps=float("nan")):
+    """orf.rcwaFields (rcwaFields.m:1-21) on the GPU.
+
+    Returns E, H (H~ = i eta0 H) as nx x ny x nz x 3 complex arrays, eps at
+    the points, the grid, zInterfaces, flux (normalised to the incident flux)
+    and ``out`` (the rcwa2D-style result).  z = 0 is the top of the first layer.
+    """
+    if isinstance(pol, (list, tuple)) and pol and isinstance(pol[0], str):
+        raise OrfError("rcwa:pol", "rcwaFields takes one polarisation")
+    x = np.ascontiguousarray(np.atleast_1d(np.asarray(x, float)))
+    y = np.ascontiguousarray(np.atleast_1d(np.asarray(y, float)))
+    z = np.ascontiguousarray(np.atleast_1d(np.asarray(z, float)))
+    prob = Problem([dev], float(np.asarray(lambda0).ravel()[0]), theta, phi, pol, Nx=Nx, Ny=Ny, Nsample=Nsample,
+                   Factorization=Factorization, GapEps=GapEps, Symmetry="none")
+    nx, ny, nz = x.size, y.size, z.size
+    E = np.zeros((3, nz, ny, nx), complex)
+    H = np.zeros((3, nz, ny, nx), complex)
+    flux = np.zeros(nz)
+    M = prob.M
+    m = np.zeros(M, np.int32)
+    n = np.zeros(M, np.int32)
+    R = np.zeros((1, 1, 1, M))
+    T = np.zeros((1, 1, 1, M))
+    Rt = np.zeros((1, 1, 1))
+    Tt = np.zeros((1, 1, 1))
+    r = np.zeros((1, 1, 1, M, 3), complex)
+    t = np.zeros((1, 1, 1, M, 3), complex)
+    Pp = C.POINTER
+    res = _lib.orf_result(m.ctypes.data_as(Pp(C.c_int32)), n.ctypes.data_as(Pp(C.c_int32)), _lib.dptr(R),
+                          _lib.dptr(T), _lib.dptr(Rt), _lib.dptr(Tt), _lib.dptr(r), _lib.dptr(t))
+    lib = _lib.load_library()
+    rc = lib.orf_fields(_lib.context(), C.byref(prob.struct), nx, _lib.dptr(x), ny, _lib.dptr(y), nz, _lib.dptr(z),
+                        _lib.dptr(E), _lib.dptr(H), _lib.dptr(flux), C.byref(res))
+    _raise(rc)
+    out = _pack(prob, m, n, R, T, Rt, Tt, r, t, False)
+    E = np.transpose(E, (3, 2, 1, 0))   # column-major nx x ny x nz x 3 -> numpy axes (x, y, z, c)
+    H = np.transpose(H, (3, 2, 1, 0))
+    Lx, Ly = dev["period"]
+    ztop = np.concatenate([[0.0], np.cumsum([L["thickness"] for L in dev["layers"]])])
+    lam = float(np.asarray(lambda0).ravel()[0])
+    eps = np.zeros((nx, ny, nz), complex)
+    for k, zz in enumerate(z):
+        if zz < 0:
+            eps[:, :, k] = material_index(dev["incident"], lam) ** 2
+        elif zz >= ztop[-1]:
+            eps[:, :, k] = material_index(dev["substrate"], lam) ** 2
+        else:
+            j = int(np.searchsorted(ztop, zz, side="right") - 1)
+            eps[:, :, k] = layerEps(dev["layers"][j], [Lx, Ly], x, y, lam)
+    return {"E": E, "H": H, "eps": eps, "x": x, "y": y, "z": z, "zInterfaces": ztop, "flux": flux, "out": out}
 
 
 def rcwa1D(dev, lambda0, theta, phi, pol, N=10, Nsample=0, Factorization="li", GapEps=float("nan")):

@@ -240,3 +240,49 @@ def default_sample_counts(devs: Sequence[dict], Nx: int, Ny: int) -> tuple:
         else:  # structures of different periods cannot all be edge-aligned by one count
             out.append(ns_min)
     return tuple(out)
+
+
+# ---------------------------------------------------------------- permittivity at points
+def _wrap(d, L):
+    """Displacement mapped into [-L/2, L/2) (periodic unit cell, layerEps.m:40-45)."""
+    if not math.isfinite(L):
+        return d
+    t = d + L / 2
+    return t - np.floor(t / L) * L - L / 2
+
+
+def _in_polygon(px, py, v):
+    """Edge-inclusive even-odd test of points against polygon vertices v (K x 2)."""
+    inside = np.zeros(px.shape, bool)
+    on = np.zeros(px.shape, bool)
+    k = v.shape[0]
+    for i in range(k):
+        x1, y1 = v[i]
+        x2, y2 = v[(i + 1) % k]
+        cross = (x2 - x1) * (py - y1) - (y2 - y1) * (px - x1)
+        box = (np.minimum(x1, x2) <= px) & (px <= np.maximum(x1, x2)) & \
+              (np.minimum(y1, y2) <= py) & (py <= np.maximum(y1, y2))
+        on |= (np.abs(cross) <= 1e-12 * max(1.0, abs(x2 - x1) + abs(y2 - y1))) & box
+        crosses = (y1 > py) != (y2 > py)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            xs = x1 + (py - y1) * (x2 - x1) / (y2 - y1)
+        inside ^= crosses & (px < xs)
+    return inside | on
+
+
+def layerEps(layer: dict, period: Sequence[float], x, y, lambda0: float) -> np.ndarray:
+    """eps = n^2 of a layer at the points x (nx) by y (ny): background, then the
+    shapes painted in order with periodic wrap (layerEps.m:19-45)."""
+    X, Y = np.meshgrid(np.atleast_1d(np.asarray(x, float)), np.atleast_1d(np.asarray(y, float)), indexing="ij")
+    eps = np.full(X.shape, material_index(layer["background"], lambda0) ** 2, complex)
+    for sh in layer["shapes"]:
+        dx = _wrap(X - sh["center"][0], period[0])
+        dy = _wrap(Y - sh["center"][1], period[1])
+        if sh["type"] == "circle":
+            mask = dx * dx + dy * dy < sh["radius"] ** 2
+        elif sh["type"] == "rect":
+            mask = (np.abs(dx) < sh["size"][0] / 2) & (np.abs(dy) < sh["size"][1] / 2)
+        else:
+            mask = _in_polygon(dx, dy, np.asarray(sh["vertices"], float))
+        eps[mask] = material_index(sh["mat"], lambda0) ** 2
+    return eps

@@ -601,3 +601,155 @@ def converge_n(fun: Callable, Tol=1e-3, Ns=range(3, 26, 2), Progress=None):
             break
     res["history"] = hist
     return res
+
+
+# ------------------------------------------------------------------ fields
+def _star2(SA, SB):
+    """Star product of two unbatched S-matrix dicts (smatrixStar.m:4-10)."""
+    return {k: v[0] if v.ndim == 3 else v for k, v in smatrix_star(SA, SB).items()}
+
+
+def rcwa_fields(dev, lambda0, theta, phi, pol, x, y, z, Nx=5, Ny=None, Nsample=0, Factorization="li",
+                GapEps=float("nan")):
+    """Fields in and around a grating (rcwaFields.m:1-21; algorithm restated).
+
+    Every layer (and the half-spaces) is expanded in its own modes with
+    s = [Ex; Ey] = W (e^{-lam k0 z} a+ + e^{-lam k0 (L - z)} a-) and
+    u = [H~x; H~y] = V (-e^{-lam k0 z} a+ + e^{-lam k0 (L - z)} a-), z measured
+    from the top of the layer (the convention of smatrixLayer.m:6-18).  The
+    amplitudes come from the zero-thickness gaps between layers: with S_up the
+    stack above a gap and S_down the stack below it, c+ = (I - S_up22
+    S_down11)^-1 S_up21 c_inc and c- = S_down11 c+; a layer's a+ follows from
+    the gap above it and a- from the gap below it, so no growing exponential
+    is ever formed.  Ez = i [eps]^-1 (Kx H~y - Ky H~x) (pqMatrices.m:4),
+    H~z = i (Kx Ey - Ky Ex).  Returns E, H (nx x ny x nz x 3), eps, flux
+    (normalised to the incident flux) and the rcwa2D-style result.
+    """
+    x = np.atleast_1d(np.asarray(x, float))
+    y = np.atleast_1d(np.asarray(y, float))
+    z = np.atleast_1d(np.asarray(z, float))
+    Lx, Ly = dev["period"]
+    if Ny is None:
+        Ny = Nx
+    if not math.isfinite(Ly):
+        Ny = 0
+    m, n = harmonics2d(Nx, Ny)
+    M = m.size
+    k0 = 2 * np.pi / lambda0
+    n_inc = nk(dev["incident"], lambda0)
+    if abs(n_inc.imag) > 1e-12:
+        raise OracleError("rcwa:lossyIncident", "Incident medium must be lossless.")
+    n_inc = n_inc.real
+    eps_inc = n_inc ** 2
+    eps_sub = nk(dev["substrate"], lambda0) ** 2
+    st, ct = math.sin(theta), math.cos(theta)
+    khat = np.array([st * math.cos(phi), st * math.sin(phi), ct])
+    E_inc = polarization_vector(khat, pol)
+    kx = (n_inc * khat[0] + m * lambda0 / Lx).astype(float)
+    ky = (n_inc * khat[1] + (n * lambda0 / Ly if math.isfinite(Ly) else 0.0 * n)).astype(float)
+    if isinstance(Nsample, (int, float)) and Nsample == 0:
+        Nsx, Nsy = default_sample_counts([dev], Nx, Ny)
+    else:
+        Nsx, Nsy = int(Nsample[0]), int(Nsample[1])
+    xg, yg = cell_grid([Lx, Ly], Nsx, Nsy)
+    eps_gap = GapEps if not math.isnan(GapEps) else 1.0 + float(np.max(kx ** 2 + ky ** 2))
+    W0, V0, _, _ = modes_homogeneous(kx, ky, eps_gap)
+    layers = []
+    for L in dev["layers"]:
+        if L["shapes"]:
+            e = layer_eps(L, [Lx, Ly], xg, yg, lambda0)
+            E = conv_matrix_2d(e, Nx, Ny)
+            Exx, Eyy = conv_matrices_li_2d(e, Nx, Ny) if Factorization == "li" else (E, E)
+            P, Q, Ei = pq_matrices(E, Exx, Eyy, kx, ky)
+            W, V, lam = modes_layer(P, Q)
+        else:
+            eps = nk(L["background"], lambda0) ** 2
+            W, V, lam, _ = modes_homogeneous(kx, ky, eps)
+            Ei = np.eye(M) / eps
+        S = smatrix_layer(W[None], V[None], lam[None], W0, V0, np.array([k0 * L["thickness"]]))
+        layers.append({"W": W, "V": V, "lam": lam, "Ei": Ei, "d": L["thickness"],
+                       "S": {k: v[0] for k, v in S.items()}})
+    Wr, Vr, lamr, _ = modes_homogeneous(kx, ky, eps_inc)
+    Ws, Vs, lams, _ = modes_homogeneous(kx, ky, eps_sub)
+    S_ref = smatrix_region(Wr, Vr, W0, V0, "ref")
+    S_trn = smatrix_region(Ws, Vs, W0, V0, "trn")
+    nL = len(layers)
+    # prefix stacks S_ref * S_1 * ... * S_{j}, suffix stacks S_{j+1} * ... * S_trn
+    up = [S_ref]
+    for j in range(nL):
+        up.append(_star2(up[-1], layers[j]["S"]))
+    down = [None] * (nL + 1)
+    down[nL] = S_trn
+    for j in range(nL - 1, -1, -1):
+        down[j] = _star2(layers[j]["S"], down[j + 1])
+    glob = _star2(up[nL], S_trn)
+    i0 = int(np.flatnonzero((m == 0) & (n == 0))[0])
+    c_inc = np.zeros(2 * M, complex)
+    c_inc[i0], c_inc[M + i0] = E_inc[0], E_inc[1]
+    c_ref = glob["S11"] @ c_inc
+    c_trn = glob["S21"] @ c_inc
+    I2 = np.eye(2 * M)
+    gaps = []   # gap above layer j (j = 0..nL): (c+, c-)
+    for j in range(nL + 1):
+        cp = np.linalg.solve(I2 - up[j]["S22"] @ down[j]["S11"], up[j]["S21"] @ c_inc)
+        gaps.append((cp, down[j]["S11"] @ cp))
+    ztop = np.concatenate([[0.0], np.cumsum([Lj["d"] for Lj in layers])])
+    nz = z.size
+    s_all = np.zeros((nz, 2 * M), complex)
+    u_all = np.zeros((nz, 2 * M), complex)
+    ez_all = np.zeros((nz, M), complex)
+    layer_of = np.full(nz, -1)
+    for iz, zz in enumerate(z):
+        if zz < 0:
+            fp = np.exp(-lamr * k0 * zz)
+            fm = np.exp(lamr * k0 * zz)
+            s = c_inc * fp + c_ref * fm
+            u = Vr @ (-c_inc * fp + c_ref * fm)
+            Ei = np.eye(M) / eps_inc
+        elif zz >= ztop[-1]:
+            fp = np.exp(-lams * k0 * (zz - ztop[-1]))
+            s = c_trn * fp
+            u = Vs @ (-c_trn * fp)
+            Ei = np.eye(M) / eps_sub
+            layer_of[iz] = nL
+        else:
+            j = int(np.searchsorted(ztop, zz, side="right") - 1)
+            Lj = layers[j]
+            layer_of[iz] = j
+            cp, cm = gaps[j]
+            dp, dm = gaps[j + 1]
+            WiW0 = np.linalg.solve(Lj["W"], W0)
+            ViV0 = np.linalg.solve(Lj["V"], V0)
+            ap = 0.5 * (WiW0 @ (cp + cm) + ViV0 @ (cp - cm))
+            am = 0.5 * (WiW0 @ (dp + dm) + ViV0 @ (dm - dp))
+            zl = zz - ztop[j]
+            fp = np.exp(-Lj["lam"] * k0 * zl)
+            fm = np.exp(-Lj["lam"] * k0 * (Lj["d"] - zl))
+            s = Lj["W"] @ (fp * ap + fm * am)
+            u = Lj["V"] @ (-fp * ap + fm * am)
+            Ei = Lj["Ei"]
+        s_all[iz], u_all[iz] = s, u
+        ez_all[iz] = 1j * Ei @ (kx * u[M:] - ky * u[:M])
+    hz_all = 1j * (kx * s_all[:, M:] - ky * s_all[:, :M])
+    # synthesis on the (x, y) grid: phase exp(i k0 (kx x + ky y))
+    ph = np.exp(1j * k0 * (kx[None, None, :] * x[:, None, None] + ky[None, None, :] * y[None, :, None]))
+    def synth(c):   # c: nz x M -> nx x ny x nz
+        return np.einsum("abp,zp->abz", ph, c)
+    E = np.stack([synth(s_all[:, :M]), synth(s_all[:, M:]), synth(ez_all)], axis=-1)
+    H = np.stack([synth(u_all[:, :M]), synth(u_all[:, M:]), synth(hz_all)], axis=-1)
+    flux = -np.imag(np.sum(s_all[:, :M] * np.conj(u_all[:, M:]) - s_all[:, M:] * np.conj(u_all[:, :M]),
+                           axis=1)) / (n_inc * ct)
+    # permittivity at the field points
+    eps = np.zeros((x.size, y.size, nz), complex)
+    for iz in range(nz):
+        j = layer_of[iz]
+        if j < 0:
+            eps[:, :, iz] = eps_inc
+        elif j == nL:
+            eps[:, :, iz] = eps_sub
+        else:
+            eps[:, :, iz] = layer_eps(dev["layers"][j], [Lx, Ly], x, y, lambda0)
+    out = rcwa2d(dev, lambda0, theta, phi, pol, Nx=Nx, Ny=Ny, Nsample=(Nsx, Nsy),
+                 Factorization=Factorization, GapEps=GapEps, Symmetry="none")
+    return {"E": E, "H": H, "eps": eps, "x": x, "y": y, "z": z, "zInterfaces": ztop, "flux": flux,
+            "out": out, "coeffs": {"s": s_all, "u": u_all, "ez": ez_all, "hz": hz_all}}

@@ -391,3 +391,79 @@ def test_invalid_pol_raises():
     with pytest.raises(O.OracleError) as e:
         O.rcwa2d(two_layers(), 1.0, 0, 0, "x", Nx=1)
     assert e.value.identifier == "rcwa:pol"
+
+
+# ------------------------------------------------------------------ fields (tests/TestRCWAFields.m)
+def _field_pillars():
+    return S.makeStructure([0.4, 0.4], mc(1.45), mc(1),
+                           [S.makeLayer(0.5, mc(1), [S.shapeCircle(0.1, mc(3.6))]), S.makeLayer(0.2, mc(2))])
+
+
+def test_fields_slab_matches_analytic_field():
+    """tests/TestRCWAFields.m:11-31: E_y(z) of a slab built from the substrate upward."""
+    nidx = [1, 2, 1.45]
+    d, k = 0.3, 2 * np.pi
+    slab = S.makeStructure([0.37, 0.37], mc(nidx[0]), mc(nidx[2]), [S.makeLayer(d, mc(nidx[1]))])
+    z = np.linspace(-0.4, 0.7, 23)
+    f = O.rcwa_fields(slab, 1, 0, 0, "s", 0, 0, z, Nx=1)
+    Ey = f["E"][0, 0, :, 1]
+    t = Ey[-1] / np.exp(1j * k * nidx[2] * (z[-1] - d))
+    q = k * nidx[1]
+    in_slab = lambda zz: t * np.cos(q * (zz - d)) + 1j * k * nidx[2] * t / q * np.sin(q * (zz - d))
+    E0 = in_slab(0)
+    dE0 = -q * t * np.sin(-q * d) + 1j * k * nidx[2] * t * np.cos(-q * d)
+    a = (E0 + dE0 / (1j * k * nidx[0])) / 2
+    b = (E0 - dE0 / (1j * k * nidx[0])) / 2
+    ref = (z < 0) * (a * np.exp(1j * k * nidx[0] * z) + b * np.exp(-1j * k * nidx[0] * z)) + \
+        ((z >= 0) & (z < d)) * in_slab(z) + (z >= d) * t * np.exp(1j * k * nidx[2] * (z - d))
+    np.testing.assert_allclose(Ey, ref, atol=1e-12, rtol=0)
+    assert abs(a - 1) < 1e-12
+    assert abs(nidx[2] * abs(t) ** 2 - O.tmm(1, nidx, [d], 0, "s")["T"][0, 0]) < 1e-12
+
+
+def test_fields_flux_is_conserved_and_equals_T():
+    """tests/TestRCWAFields.m:33-44 (Nx = 3 here to keep the CPU suite short)."""
+    dev = _field_pillars()
+    z = np.array([-0.3, 0, 0.1, 0.25, 0.499, 0.5, 0.6, 0.69, 0.7, 1.5])
+    for pol in "sp":
+        o = O.rcwa2d(dev, 0.94, 0.3, 0.4, pol, Nx=3, Symmetry="none")
+        f = O.rcwa_fields(dev, 0.94, 0.3, 0.4, pol, 0, 0, z, Nx=3)
+        np.testing.assert_allclose(f["out"]["T"], o["T"], atol=1e-12)
+        np.testing.assert_allclose(f["out"]["R"], o["R"], atol=1e-12)
+        np.testing.assert_allclose(f["flux"], o["Ttot"][0] * np.ones(z.size), atol=1e-10)
+
+
+def test_fields_tangential_components_are_continuous():
+    """tests/TestRCWAFields.m:46-58."""
+    dev = _field_pillars()
+    x = np.arange(-0.2, 0.2001, 0.04)
+    y = np.arange(-0.2, 0.2001, 0.1)
+    dz = 1e-12
+    f = O.rcwa_fields(dev, 0.94, 0.3, 0.4, "p", x, y, [-dz, dz, 0.5 - dz, 0.5 + dz], Nx=3)
+    for c in range(2):
+        for a, b in ((1, 0), (3, 2)):
+            np.testing.assert_allclose(f["E"][:, :, a, c], f["E"][:, :, b, c], atol=1e-9)
+            np.testing.assert_allclose(f["H"][:, :, a, c], f["H"][:, :, b, c], atol=1e-9)
+
+
+def test_fields_lossy_layer_absorbs():
+    """tests/TestRCWAFields.m:60-70: flux drops only inside the metal; 1 - R above, T below."""
+    met = mc(0.2 + 3j)
+    g = S.makeStructure([0.5, math.inf], mc(1), mc(1.45), [S.makeLayer(0.05, mc(1), [S.shapeRect(0.25, math.inf, met)])])
+    f = O.rcwa_fields(g, 0.8, 0.2, 0, "p", 0, 0, [-0.1, 0, 0.025, 0.05, 0.3], Nx=15, Nsample=(1024, 1))
+    assert abs(f["flux"][0] - (1 - f["out"]["Rtot"][0])) < 1e-10
+    assert abs(f["flux"][-1] - f["out"]["Ttot"][0]) < 1e-10
+    assert np.all(np.diff(f["flux"][1:4]) < 0)
+
+
+def test_fields_plane_wave_transversality_and_impedance():
+    """In a uniform medium the synthesised field is a plane wave: k . E = 0 and H~ = i k x E."""
+    dev = S.makeStructure([0.5, 0.5], mc(1.3), mc(1.3), [S.makeLayer(0.2, mc(1.3))])
+    th, ph = 0.4, 0.3
+    f = O.rcwa_fields(dev, 0.8, th, ph, [0.6, 0.8j], 0.1, -0.05, [-0.2, 0.1, 0.5], Nx=1)
+    kv = 1.3 * np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+    E = f["E"][0, 0]
+    H = f["H"][0, 0]
+    np.testing.assert_allclose(E @ kv, 0, atol=1e-12)
+    np.testing.assert_allclose(H, 1j * np.cross(kv, E), atol=1e-12)
+    np.testing.assert_allclose(np.linalg.norm(E, axis=1), 1.0, atol=1e-12)
