@@ -1166,6 +1166,11 @@ __global__ __launch_bounds__(256) void hessenberg_blocked_kernel(int n, cplx* __
       __syncthreads();
     }
     const int q0 = k0 + ib;
+    // T of this panel into the unused rows 0..c of V's columns (for forming Q by panels)
+    for (int e = t; e < ib * ib; e += nt) {
+      const int l = e % ib, j = e / ib;
+      if (l <= j) V[(size_t)(k0 + j) * n + l] = Ts[l + NB * j];
+    }
     // (e) rows 0..k0 of Y: P = A(0:k0, k0+1:) V in Y's columns NB.., then Y = P T
     for (int e = t; e < (k0 + 1) * ib; e += nt) {
       const int r = e % (k0 + 1), j = e / (k0 + 1);
@@ -1252,6 +1257,33 @@ __global__ __launch_bounds__(256) void hessenberg_blocked_kernel(int n, cplx* __
 #undef AA
 #undef YY
 #undef VS
+}
+
+// Z = I on every page.
+__global__ void eye_pages_kernel(int n, cplx* __restrict__ Z, long long sZ) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  Z[(size_t)blockIdx.y * sZ + e] = cmk((e % n) == (e / n) ? 1.0 : 0.0, 0.0);
+}
+
+// Clean copies of one panel of the blocked reduction for the GEMM formation of
+// Q: Vc (m x ib, rows k0+1.., unit lower trapezoid) and Tc (ib x ib upper).
+__global__ void panel_extract_kernel(int n, int k0, int ib, const cplx* __restrict__ Vg, cplx* __restrict__ Vc,
+                                     long long sVc, cplx* __restrict__ Tc, long long sTc) {
+  const int page = blockIdx.y;
+  const int m = n - k0 - 1;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const cplx* V = Vg + (size_t)page * n * n;
+  if (e < (long long)m * ib) {
+    const int r = (int)(e % m), j = (int)(e / m);
+    cplx v = cmk(0.0, 0.0);
+    if (r > j) v = V[(size_t)(k0 + j) * n + k0 + 1 + r];
+    else if (r == j) v = cmk(1.0, 0.0);
+    Vc[(size_t)page * sVc + e] = v;
+  } else if (e < (long long)m * ib + ib * ib) {
+    const int q = (int)(e - (long long)m * ib), l = q % ib, j = q / ib;
+    Tc[(size_t)page * sTc + q] = (l <= j) ? V[(size_t)(k0 + j) * n + l] : cmk(0.0, 0.0);
+  }
 }
 
 // W := Q W with Q = H_0 H_1 ... H_{n-3} the Hessenberg reduction's reflectors,
@@ -1954,10 +1986,39 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
   else
     hessenberg_noq_kernel<<<batch, 256, 0, st>>>(n, A, lda, sA, Vh, tau, 0);
   // Z = Q of the reduction, so that the QR's accumulated Z is the full Schur basis
-  for (int b0 = 0; b0 < batch; b0 += 65535) {
-    const int bb = batch - b0 < 65535 ? batch - b0 : 65535;
-    hess_q_apply_kernel<<<dim3((n + 15) / 16, bb), 256, (size_t)n * 16 * sizeof(cplx), st>>>(
-        n, Vh + (size_t)b0 * n * n, tau + (size_t)b0 * n, Zwork + (size_t)b0 * sZ, n, sZ, 1);
+  const int blockedNB = (use_lds || n < 32) ? 0 : (pan16 <= (size_t)g_lds_limit ? 16 : (pan8 <= (size_t)g_lds_limit ? 8 : 0));
+  if (blockedNB && sZ >= (long long)n * n && (long long)3 * n * blockedNB + blockedNB * blockedNB <= (long long)n * n) {
+    // Q = P_0 P_1 ... with P_p = I - V_p T_p V_p^H, applied to I from the last panel on;
+    // Q stays the identity outside its trailing (n-k0-1) block, so each panel is three
+    // MFMA GEMMs on that block (clean V_p, T_p and the products in Ywork)
+    const int NB = blockedNB, nref = n - 2;
+    const long long sY = sZ;
+    cplx* Vc = Ywork;
+    cplx* X = Ywork + (size_t)n * NB;
+    cplx* Yp = Ywork + (size_t)2 * n * NB;
+    cplx* Tc = Ywork + (size_t)3 * n * NB;
+    for (int b0 = 0; b0 < batch; b0 += 65535) {
+      const int bb = batch - b0 < 65535 ? batch - b0 : 65535;
+      eye_pages_kernel<<<dim3((unsigned)(((long long)n * n + 255) / 256), bb), 256, 0, st>>>(n, Zwork + (size_t)b0 * sZ, sZ);
+    }
+    for (int k0 = ((nref - 1) / NB) * NB; k0 >= 0; k0 -= NB) {
+      const int ib = min(NB, nref - k0), m = n - k0 - 1;
+      for (int b0 = 0; b0 < batch; b0 += 65535) {
+        const int bb = batch - b0 < 65535 ? batch - b0 : 65535;
+        panel_extract_kernel<<<dim3((unsigned)(((long long)m * ib + ib * ib + 255) / 256), bb), 256, 0, st>>>(
+            n, k0, ib, Vh + (size_t)b0 * n * n, Vc + (size_t)b0 * sY, sY, Tc + (size_t)b0 * sY, sY);
+      }
+      cplx* Q22 = Zwork + (size_t)(k0 + 1) * n + (k0 + 1);
+      orf_zgemm(st, 'C', 'N', ib, m, m, cmk(1.0, 0.0), Vc, m, sY, Q22, n, sZ, cmk(0.0, 0.0), X, ib, sY, batch);
+      orf_zgemm(st, 'N', 'N', ib, m, ib, cmk(1.0, 0.0), Tc, ib, sY, X, ib, sY, cmk(0.0, 0.0), Yp, ib, sY, batch);
+      orf_zgemm(st, 'N', 'N', m, m, ib, cmk(-1.0, 0.0), Vc, m, sY, Yp, ib, sY, cmk(1.0, 0.0), Q22, n, sZ, batch);
+    }
+  } else {
+    for (int b0 = 0; b0 < batch; b0 += 65535) {
+      const int bb = batch - b0 < 65535 ? batch - b0 : 65535;
+      hess_q_apply_kernel<<<dim3((n + 15) / 16, bb), 256, (size_t)n * 16 * sizeof(cplx), st>>>(
+          n, Vh + (size_t)b0 * n * n, tau + (size_t)b0 * n, Zwork + (size_t)b0 * sZ, n, sZ, 1);
+    }
   }
   te("eig_hessenberg");
   tb("eig_schur_qr");
