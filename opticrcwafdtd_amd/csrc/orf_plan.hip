@@ -352,6 +352,10 @@ struct Work {
   int* groupOf;
   int* colIdx;
   cplx* epsSolve;
+  // layer-parallel execution: mode tables of the chunk and page maps
+  DevPage *pagesA, *pagesH;
+  cplx *seA, *siA;
+  int *idxA, *idxB;
 };
 
 // ====================================================================== plan
@@ -400,6 +404,12 @@ struct orf_plan {
   };
   bool keepModes = false;
   std::vector<Kept*> kept;
+  // layer-parallel execution (several layers): every layer's modes and S-matrix in one
+  // batch, then the star products as a tree; host copies of the per-layer tables
+  bool layerParallel = false;
+  int workModes = 0;
+  std::vector<std::vector<DevPage>> hPages;
+  std::vector<std::vector<cplx>> hSe, hSi;
   double kzi0 = 1.0;
   zc epsInc0 = 1.0, epsSub0 = 1.0;  // half-space permittivities of context 0
   StageTimer timer;
@@ -511,6 +521,12 @@ static Work make_work(orf_plan& P, int Cp) {
   w.groupOf = (int*)get(sizeof(int) * Cs);
   w.colIdx = (int*)get(sizeof(int) * 2 * Cs);
   w.epsSolve = (cplx*)get(cs * Cs);
+  w.pagesA = (DevPage*)get(sizeof(DevPage) * Cp);
+  w.pagesH = (DevPage*)get(sizeof(DevPage) * Cp);
+  w.seA = (cplx*)get(cs * (size_t)Cp * P.nslotMax);
+  w.siA = (cplx*)get(cs * (size_t)Cp * P.nslotMax);
+  w.idxA = (int*)get(sizeof(int) * Cs);
+  w.idxB = (int*)get(sizeof(int) * Cs);
   return w;
 }
 
@@ -1104,6 +1120,9 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
     DBuf* dp = new DBuf();
     dp->upload(pages);
     P.dPagesByLayer.push_back(dp);
+    P.hPages.push_back(pages);
+    P.hSe.push_back(se);
+    P.hSi.push_back(si);
     DBuf* de = new DBuf();
     de->upload(se);
     P.dSlotEps.push_back(de);
@@ -1126,10 +1145,18 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   const size_t perMode = per_mode_bytes(P) + (size_t)P.nSec * per_solve_bytes(P);
   size_t budget = (size_t)(0.6 * (double)(freeB + g_pool.cachedBytes));
   int cp = (int)std::max<size_t>(1, budget / std::max<size_t>(perMode, 1));
-  if (pr->batchSize > 0) cp = std::min(cp, pr->nPages > 0 ? pr->batchSize : pr->batchSize * P.L);
   cp = std::min(cp, 32768 / P.nSec);  // grid-dimension limit of the page-batched launches
-  P.chunkPages = std::min(cp, npages);
-  P.work = make_work(P, P.chunkPages);
+  P.layerParallel = P.nLayers >= 2 && !P.keepModes;
+  if (P.layerParallel) {
+    P.chunkPages = std::max(1, std::min(cp / P.nLayers, npages));
+    if (pr->batchSize > 0) P.chunkPages = std::min(P.chunkPages, pr->nPages > 0 ? pr->batchSize : pr->batchSize * P.L);
+    P.workModes = P.chunkPages * P.nLayers;
+  } else {
+    if (pr->batchSize > 0) cp = std::min(cp, pr->nPages > 0 ? pr->batchSize : pr->batchSize * P.L);
+    P.chunkPages = std::min(cp, npages);
+    P.workModes = P.chunkPages;
+  }
+  P.work = make_work(P, P.workModes);
   // results
   const size_t nout = (size_t)P.nPages * P.nPol;
   P.dR.alloc(sizeof(double) * P.M * nout);
@@ -1138,6 +1165,275 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   P.dTtot.alloc(sizeof(double) * nout);
   P.dr.alloc(sizeof(cplx) * 3 * P.M * nout);
   P.dt.alloc(sizeof(cplx) * 3 * P.M * nout);
+  HIPCHECK(hipStreamSynchronize(st));
+}
+
+// Modes of Cp patterned mode pages (pages[m], slot permittivities se/si at
+// m * nslotMax), solve pages w.solves[0 .. Cs): convolution matrices
+// (Laurent / Li), Omega^2 = P Q, eigen-decomposition, W, V = Q W lam^-1 and
+// lam into w.Wm, w.Vm, w.lam.  keep (single-page field solves) receives [eps].
+static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cplx* se, const cplx* si, int Cp, int Cs,
+                            orf_plan::Kept* keep) {
+  hipStream_t st = P.st;
+  const int M = P.M, nf = P.nf, nr = P.nr, nSec = P.nSec;
+  const long long M2 = (long long)M * M, nf2 = (long long)nf * nf, n2 = (long long)nr * nr;
+  const long long nfr = (long long)nf * nr;
+  const int Wx = 2 * P.Nx + 1, Wy = 2 * P.Ny + 1, Hx = 4 * P.Nx + 1, Hy = 4 * P.Ny + 1;
+  {
+    STAGE("conv");
+    laurent_kernel<<<g1(M2, Cp), 256, 0, st>>>(P.Nx, P.Ny, M, pages, P.dChi2.as<cplx>(), se, P.nslotMax, w.E, M2);
+    if (keep) {
+      keep->E.alloc(sizeof(cplx) * M2);
+      keep->patterned = true;
+      HIPCHECK(hipMemcpyAsync(keep->E.p, w.E, sizeof(cplx) * M2, hipMemcpyDeviceToDevice, st));
+    }
+    if (P.factorization == ORF_FACT_LI) {
+      // Exx: inverse rule along x at each y-sample, Laurent along y
+      const long long perx = (long long)Wx * Wx * P.Nsy;
+      li_toeplitz_kernel<<<g1(perx, Cp), 256, 0, st>>>(0, P.Nx, P.Nsy, Hx, pages, P.dChiX.as<cplx>(),
+                                                        (long long)P.nslotMax * Hx * P.Nsy, si, P.nslotMax, w.Toe);
+      KCHECK();
+      // invert every Toeplitz block: LU then solve against the identity (in Yw scratch)
+      const int nb = Cp * P.Nsy;
+      cplx* inv = w.Inv;
+      eye_stack_kernel<<<g1((long long)Wx * Wx * nb), 256, 0, st>>>(Wx, (long long)nb, inv);
+      orf_zgetrf(st, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, nb, nullptr);
+      orf_zgetrs(st, 'L', Wx, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, inv, Wx, (long long)Wx * Wx, nb);
+      // G = Fy * [A_j]^T : (Hy x Nsy) * (Nsy x Wx^2)
+      bulk_gemm(P, 'N', 'T', Hy, Wx * Wx, P.Nsy, ONE, P.dFy.as<cplx>(), Hy, 0, inv, Wx * Wx,
+                (long long)Wx * Wx * P.Nsy, ZERO, w.GH, Hy, (long long)Hy * Wx * Wx, Cp);
+      // Eyy: inverse rule along y at each x-sample, Laurent along x
+      const long long pery = (long long)Wy * Wy * P.Nsx;
+      li_toeplitz_kernel<<<g1(pery, Cp), 256, 0, st>>>(1, P.Ny, P.Nsx, Hy, pages, P.dChiY.as<cplx>(),
+                                                        (long long)P.nslotMax * Hy * P.Nsx, si, P.nslotMax, w.Toe);
+      const int nb2 = Cp * P.Nsx;
+      eye_stack_kernel<<<g1((long long)Wy * Wy * nb2), 256, 0, st>>>(Wy, (long long)nb2, inv);
+      orf_zgetrf(st, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, nb2, nullptr);
+      orf_zgetrs(st, 'L', Wy, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, inv, Wy, (long long)Wy * Wy, nb2);
+      bulk_gemm(P, 'N', 'T', Hx, Wy * Wy, P.Nsx, ONE, P.dFx.as<cplx>(), Hx, 0, inv, Wy * Wy,
+                (long long)Wy * Wy * P.Nsx, ZERO, w.HH, Hx, (long long)Hx * Wy * Wy, Cp);
+      li_gather_kernel<<<g1(M2, Cp), 256, 0, st>>>(P.Nx, P.Ny, M, w.GH, w.HH, w.Exx, w.Eyy, M2);
+    } else {
+      HIPCHECK(hipMemcpyAsync(w.Exx, w.E, sizeof(cplx) * M2 * Cp, hipMemcpyDeviceToDevice, st));
+      HIPCHECK(hipMemcpyAsync(w.Eyy, w.E, sizeof(cplx) * M2 * Cp, hipMemcpyDeviceToDevice, st));
+    }
+    KCHECK();
+  }
+  {
+    STAGE("omega");
+    omega_rhs_kernel<<<g1(2 * M2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.Exx, w.Eyy, M2, w.G, 2 * M2);
+    orf_zgetrf(st, M, w.E, M, M2, w.ipiv, nullptr, Cp, w.dinv);
+    orf_zgetrs(st, 'L', M, 2 * M, w.E, M, M2, w.ipiv, w.dinv, w.G, M, 2 * M2, Cp);
+    omega_assemble_kernel<<<g1(nf2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.G, 2 * M2, w.Exx, w.Eyy,
+                                                       M2, w.Omega, nf2);
+    KCHECK();
+    if (P.reduced) {
+      sector_project_kernel<<<g1(n2, Cs), 256, 0, st>>>(nf, nr, w.solves, P.dSectors.as<DevSector>(), 0, 0,
+                                                         w.Omega, nf2, 0, w.A0, n2);
+    } else {
+      HIPCHECK(hipMemcpyAsync(w.A0, w.Omega, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+    }
+    KCHECK();
+  }
+  {
+    STAGE("eig");
+    orf_zgeev_timed(st, nr, w.A0, nr, n2, w.mu, w.Wm, nr, n2, w.Zw, w.Yw, w.Hv, w.info, Cs,
+                    [&](const char* nm, int phase) {
+                      if (phase == 0) P.timer.begin(nm, st); else P.timer.end(st);
+                    });
+    KCHECK();
+  }
+  check_info(P, w.info, Cs, "eigensolver (QR iteration did not converge)");
+  {
+    STAGE("modes");
+    lambda_kernel<<<g1((long long)nr * Cs), 256, 0, st>>>(nr, Cs, w.mu, w.lam);
+    cplx* Wf = w.Wm;
+    if (P.reduced) {
+      sector_expand_kernel<<<g1(nfr, Cs), 256, 0, st>>>(nf, nr, w.solves, P.dSectors.as<DevSector>(), w.Wm, n2,
+                                                         w.Wf, nfr);
+      Wf = w.Wf;
+    }
+    // T1 = Eyy W2, T2 = Exx W1 per sector
+    for (int s = 0; s < nSec; ++s) {
+      bulk_gemm(P, 'N', 'N', M, nr, M, ONE, w.Eyy, M, M2, Wf + s * nfr + M, nf, nSec * nfr, ZERO,
+                w.T12 + s * nfr, nf, nSec * nfr, Cp);
+      bulk_gemm(P, 'N', 'N', M, nr, M, ONE, w.Exx, M, M2, Wf + s * nfr, nf, nSec * nfr, ZERO,
+                w.T12 + s * nfr + M, nf, nSec * nfr, Cp);
+    }
+    qw_assemble_kernel<<<g1(nfr, Cs), 256, 0, st>>>(M, nr, w.solves, pages, P.dKxy.as<double>(), Wf, nfr, w.T12,
+                                                    nfr, w.QW, nfr);
+    if (P.reduced)
+      sector_rows_kernel<<<g1(n2, Cs), 256, 0, st>>>(nf, nr, nr, w.solves, P.dSectors.as<DevSector>(), w.QW, nfr,
+                                                     w.Vm, n2);
+    else
+      HIPCHECK(hipMemcpyAsync(w.Vm, w.QW, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+    colscale_inv_kernel<<<g1(n2, Cs), 256, 0, st>>>(nr, nr, w.lam, w.Vm, n2);
+    KCHECK();
+  }
+}
+
+// Last stage of a chunk: S = S_acc * S_trn on the source columns, then the
+// efficiencies.  Expects w.solves / w.groupOf in the standard order
+// (solve page m * nSec + s for mode page m of the chunk).  SB[0], SB[2] receive
+// S_trn; R, T (n x ncol per solve page) are scratch distinct from SA and SB.
+static void finish_chunk(orf_plan& P, Work& w, int p0, int Cp, cplx* const SA[4], cplx* const SB[4], cplx* R, cplx* T) {
+  hipStream_t st = P.st;
+  const int nr = P.nr, nSec = P.nSec, Cs = Cp * nSec;
+  const long long n2 = (long long)nr * nr, G = (long long)P.nCtx * nSec;
+  for (int k = 0; k < 4; k += 2)
+    gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dStrn.as<cplx>() + k * n2 * G, n2, SB[k], n2);
+  const int ncol = P.reduced ? 1 : 2;
+  {
+    std::vector<int> cols((size_t)Cs * ncol);
+    for (int sp = 0; sp < Cs; ++sp) {
+      if (P.reduced) cols[sp] = P.sectorSrcCol[sp % nSec];
+      else { cols[2 * sp] = P.i0; cols[2 * sp + 1] = P.M + P.i0; }
+    }
+    HIPCHECK(hipMemcpyAsync(w.colIdx, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, st));
+  }
+  smatrix_star_cols(P, w, Cs, SA, SB, ncol, w.colIdx, R, T);
+  {
+    STAGE("output");
+    const DevPage* basePages = P.dPagesByLayer[P.nLayers]->as<DevPage>() + p0;
+    output_kernel<<<dim3(Cs, P.nPol), 256, 0, st>>>(
+        P.M, nr, P.reduced, p0, P.nPages, w.solves, basePages, P.dSectors.as<DevSector>(), P.dSources.as<DevSource>(),
+        P.dKxy.as<double>(), (const double*)P.dKz.p, P.dKzi0.as<double>(), R, T, (long long)nr * ncol,
+        P.dR.as<double>(), P.dT.as<double>(), P.dRtot.as<double>(), P.dTtot.as<double>(), P.dr.as<cplx>(),
+        P.dt.as<cplx>(), P.i0);
+    KCHECK();
+  }
+}
+
+// One chunk with every layer at once: the modes and layer S-matrices of all
+// (layer, page, sector) triples in one batch (patterned layers through the
+// eigen-solver, homogeneous ones analytically), then S_1 * ... * S_L as a
+// binary tree of batched star products (log2 L levels instead of L sequential
+// products), S_ref * that, and the last star product on the source columns.
+// Used when no progress callback is given (the per-layer path reports after
+// every layer).
+static void execute_chunk_layers(orf_plan& P, Work& w, int p0, int Cp) {
+  hipStream_t st = P.st;
+  const int L = P.nLayers, nSec = P.nSec, nr = P.nr, M = P.M, ns = P.nslotMax;
+  const long long n2 = (long long)nr * nr, G = (long long)P.nCtx * nSec;
+  const int CpS = Cp * nSec;
+  std::vector<int> patL, homL;
+  for (int j = 0; j < L; ++j) (P.layerAnyPat[j] ? patL : homL).push_back(j);
+  const int Lp = (int)patL.size(), Lh = (int)homL.size();
+  const int CpP = Lp * Cp, CsP = CpP * nSec, CsH = Lh * CpS, CsAll = CsP + CsH;
+  std::vector<DevPage> pgA(std::max(CpP, 1)), pgH(std::max(Lh * Cp, 1));
+  std::vector<cplx> seA((size_t)std::max(CpP, 1) * ns), siA((size_t)std::max(CpP, 1) * ns);
+  std::vector<DevSolve> sv(CsAll);
+  std::vector<int> grp(CsAll), unified((size_t)L * CpS);
+  std::vector<double> kL(CsAll);
+  std::vector<cplx> epsH(std::max(CsH, 1));
+  auto fill = [&](int j, int a, int base, bool pat) {
+    for (int m = 0; m < Cp; ++m) {
+      const int pg = p0 + m, mode = a * Cp + m;
+      if (pat) {
+        pgA[mode] = P.hPages[j][pg];
+        for (int q = 0; q < ns; ++q) {
+          seA[(size_t)mode * ns + q] = P.hSe[j][(size_t)pg * ns + q];
+          siA[(size_t)mode * ns + q] = P.hSi[j][(size_t)pg * ns + q];
+        }
+      } else {
+        pgH[mode] = P.hPages[j][pg];
+      }
+      for (int sc = 0; sc < nSec; ++sc) {
+        const int sp = base + mode * nSec + sc;
+        sv[sp] = DevSolve{mode, sc};
+        grp[sp] = P.pageCtx[pg] * nSec + sc;
+        kL[sp] = P.k0[P.pageCtx[pg]] * P.thick[(size_t)j * P.B + P.pageB[pg]];
+        if (!pat) epsH[sp - base] = C(P.layerBgEps[j][pg]);
+        unified[(size_t)j * CpS + m * nSec + sc] = sp;
+      }
+    }
+  };
+  for (int a = 0; a < Lp; ++a) fill(patL[a], a, 0, true);
+  for (int a = 0; a < Lh; ++a) fill(homL[a], a, CsP, false);
+  HIPCHECK(hipMemcpyAsync(w.solves, sv.data(), sizeof(DevSolve) * CsAll, hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(w.groupOf, grp.data(), sizeof(int) * CsAll, hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(w.kL, kL.data(), sizeof(double) * CsAll, hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(w.idxA, unified.data(), sizeof(int) * unified.size(), hipMemcpyHostToDevice, st));
+  if (CpP) {
+    HIPCHECK(hipMemcpyAsync(w.pagesA, pgA.data(), sizeof(DevPage) * CpP, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(w.seA, seA.data(), sizeof(cplx) * CpP * ns, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(w.siA, siA.data(), sizeof(cplx) * CpP * ns, hipMemcpyHostToDevice, st));
+    patterned_modes(P, w, w.pagesA, w.seA, w.siA, CpP, CsP, nullptr);
+  }
+  if (CsH) {
+    STAGE("modes");
+    HIPCHECK(hipMemcpyAsync(w.pagesH, pgH.data(), sizeof(DevPage) * Lh * Cp, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(w.epsSolve, epsH.data(), sizeof(cplx) * CsH, hipMemcpyHostToDevice, st));
+    homog_modes_kernel<<<g1(n2, CsH), 256, 0, st>>>(M, nr, P.reduced, w.solves + CsP, w.pagesH,
+                                                    P.dSectors.as<DevSector>(), P.dKxy.as<double>(), w.epsSolve,
+                                                    w.Wm + (size_t)CsP * n2, w.Vm + (size_t)CsP * n2, n2,
+                                                    w.lam + (size_t)CsP * nr);
+    KCHECK();
+  }
+  gather_pages_kernel<<<g1(n2, CsAll), 256, 0, st>>>(n2, w.groupOf, P.dV0.as<cplx>(), n2, w.X2, n2);
+  smatrix_layer(P, w, CsAll, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1]);
+  // layer order: Sa[0] = S11 (= S22), Sa[1] = S12 (= S21) of page (j, m, s) at j * CpS + m * nSec + s
+  gather_pages_kernel<<<g1(n2, CsAll), 256, 0, st>>>(n2, w.idxA, w.Sb[0], n2, w.Sa[0], n2);
+  gather_pages_kernel<<<g1(n2, CsAll), 256, 0, st>>>(n2, w.idxA, w.Sb[1], n2, w.Sa[1], n2);
+  KCHECK();
+  // binary tree of star products over the layers, in order
+  cplx* cur[4] = {w.Sa[0], w.Sa[1], w.Sa[1], w.Sa[0]};
+  bool inSa = true;
+  int Kc = L;
+  const size_t half = (size_t)(CsAll / 2) * n2;
+  while (Kc > 1) {
+    const int K = Kc / 2;
+    std::vector<int> ev((size_t)K * CpS), od((size_t)K * CpS);
+    for (int q = 0; q < K * CpS; ++q) {
+      const int k = q / CpS, r = q % CpS;
+      ev[q] = 2 * k * CpS + r;
+      od[q] = (2 * k + 1) * CpS + r;
+    }
+    HIPCHECK(hipMemcpyAsync(w.idxA, ev.data(), sizeof(int) * ev.size(), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(w.idxB, od.data(), sizeof(int) * od.size(), hipMemcpyHostToDevice, st));
+    cplx* A[4];
+    cplx* B[4];
+    cplx* out[4];
+    for (int k = 0; k < 4; ++k) {
+      A[k] = w.Sn[k];
+      B[k] = w.Sn[k] + half;
+      out[k] = inSa ? w.Sb[k] : w.Sa[k];
+      gather_pages_kernel<<<g1(n2, K * CpS), 256, 0, st>>>(n2, w.idxA, cur[k], n2, A[k], n2);
+      gather_pages_kernel<<<g1(n2, K * CpS), 256, 0, st>>>(n2, w.idxB, cur[k], n2, B[k], n2);
+    }
+    smatrix_star(P, w, K * CpS, A, B, out);
+    if (Kc & 1)
+      for (int k = 0; k < 4; ++k)
+        HIPCHECK(hipMemcpyAsync(out[k] + (size_t)K * CpS * n2, cur[k] + (size_t)(Kc - 1) * CpS * n2,
+                                sizeof(cplx) * CpS * n2, hipMemcpyDeviceToDevice, st));
+    for (int k = 0; k < 4; ++k) cur[k] = out[k];
+    inSa = !inSa;
+    Kc = K + (Kc & 1);
+    HIPCHECK(hipStreamSynchronize(st));  // ev / od are rebuilt on the host for the next level
+  }
+  // standard solve order for the chunk's last stages
+  std::vector<DevSolve> sv0(CpS);
+  std::vector<int> grp0(CpS);
+  for (int m = 0; m < Cp; ++m)
+    for (int sc = 0; sc < nSec; ++sc) {
+      sv0[m * nSec + sc] = DevSolve{m, sc};
+      grp0[m * nSec + sc] = P.pageCtx[p0 + m] * nSec + sc;
+    }
+  HIPCHECK(hipMemcpyAsync(w.solves, sv0.data(), sizeof(DevSolve) * CpS, hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(w.groupOf, grp0.data(), sizeof(int) * CpS, hipMemcpyHostToDevice, st));
+  // S_acc = S_ref * (S_1 * ... * S_L)
+  cplx* Ref[4];
+  cplx* Acc[4];
+  cplx* Trn[4];
+  for (int k = 0; k < 4; ++k) {
+    Ref[k] = w.Sn[k];
+    Acc[k] = inSa ? w.Sb[k] : w.Sa[k];
+    Trn[k] = inSa ? w.Sa[k] : w.Sb[k];
+    gather_pages_kernel<<<g1(n2, CpS), 256, 0, st>>>(n2, w.groupOf, P.dSref.as<cplx>() + k * n2 * G, n2, Ref[k], n2);
+  }
+  smatrix_star(P, w, CpS, Ref, cur, Acc);
+  // Trn aliases cur (the other set); cur is consumed by the star above
+  finish_chunk(P, w, p0, Cp, Acc, Trn, w.Sn[0], w.Sn[2]);
   HIPCHECK(hipStreamSynchronize(st));
 }
 
@@ -1151,12 +1447,14 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
   if (P.keepModes && P.kept.empty())
     for (int j = 0; j < P.nLayers; ++j) P.kept.push_back(new orf_plan::Kept());
   const int nChunks = (npages + Cmax - 1) / Cmax;
-  const int M = P.M, nf = P.nf, nr = P.nr, nSec = P.nSec;
-  const long long M2 = (long long)M * M, nf2 = (long long)nf * nf, n2 = (long long)nr * nr;
-  const long long nfr = (long long)nf * nr;
-  const int Wx = 2 * P.Nx + 1, Wy = 2 * P.Ny + 1, Hx = 4 * P.Nx + 1, Hy = 4 * P.Ny + 1;
+  const int M = P.M, nr = P.nr, nSec = P.nSec;
+  const long long n2 = (long long)nr * nr;
   for (int ch = 0; ch < nChunks; ++ch) {
     const int p0 = ch * Cmax, Cp = std::min(Cmax, npages - p0), Cs = Cp * nSec;
+    if (P.layerParallel && !progress) {
+      execute_chunk_layers(P, w, p0, Cp);
+      continue;
+    }
     // solve pages: sp = mode * nSec + sector
     std::vector<DevSolve> sv(Cs);
     std::vector<int> grp(Cs);
@@ -1172,103 +1470,12 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
     for (int k = 0; k < 4; ++k)
       gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dSref.as<cplx>() + k * n2 * G, n2, w.Sa[k], n2);
     KCHECK();
-    const DevPage* basePages = P.dPagesByLayer[P.nLayers]->as<DevPage>() + p0;
     for (int j = 0; j < P.nLayers; ++j) {
       const DevPage* pages = P.dPagesByLayer[j]->as<DevPage>() + p0;
       if (P.layerAnyPat[j]) {
         const cplx* se = P.dSlotEps[j]->as<cplx>() + (size_t)p0 * P.nslotMax;
         const cplx* si = P.dSlotInv[j]->as<cplx>() + (size_t)p0 * P.nslotMax;
-        {
-          STAGE("conv");
-          laurent_kernel<<<g1(M2, Cp), 256, 0, st>>>(P.Nx, P.Ny, M, pages, P.dChi2.as<cplx>(), se, P.nslotMax, w.E, M2);
-          if (P.keepModes) {
-            P.kept[j]->E.alloc(sizeof(cplx) * M2);
-            P.kept[j]->patterned = true;
-            HIPCHECK(hipMemcpyAsync(P.kept[j]->E.p, w.E, sizeof(cplx) * M2, hipMemcpyDeviceToDevice, st));
-          }
-          if (P.factorization == ORF_FACT_LI) {
-            // Exx: inverse rule along x at each y-sample, Laurent along y
-            const long long perx = (long long)Wx * Wx * P.Nsy;
-            li_toeplitz_kernel<<<g1(perx, Cp), 256, 0, st>>>(0, P.Nx, P.Nsy, Hx, pages, P.dChiX.as<cplx>(),
-                                                              (long long)P.nslotMax * Hx * P.Nsy, si, P.nslotMax, w.Toe);
-            KCHECK();
-            // invert every Toeplitz block: LU then solve against the identity (in Yw scratch)
-            const int nb = Cp * P.Nsy;
-            cplx* inv = w.Inv;
-            eye_stack_kernel<<<g1((long long)Wx * Wx * nb), 256, 0, st>>>(Wx, (long long)nb, inv);
-            orf_zgetrf(st, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, nb, nullptr);
-            orf_zgetrs(st, 'L', Wx, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, inv, Wx, (long long)Wx * Wx, nb);
-            // G = Fy * [A_j]^T : (Hy x Nsy) * (Nsy x Wx^2)
-            bulk_gemm(P, 'N', 'T', Hy, Wx * Wx, P.Nsy, ONE, P.dFy.as<cplx>(), Hy, 0, inv, Wx * Wx,
-                      (long long)Wx * Wx * P.Nsy, ZERO, w.GH, Hy, (long long)Hy * Wx * Wx, Cp);
-            // Eyy: inverse rule along y at each x-sample, Laurent along x
-            const long long pery = (long long)Wy * Wy * P.Nsx;
-            li_toeplitz_kernel<<<g1(pery, Cp), 256, 0, st>>>(1, P.Ny, P.Nsx, Hy, pages, P.dChiY.as<cplx>(),
-                                                              (long long)P.nslotMax * Hy * P.Nsx, si, P.nslotMax, w.Toe);
-            const int nb2 = Cp * P.Nsx;
-            eye_stack_kernel<<<g1((long long)Wy * Wy * nb2), 256, 0, st>>>(Wy, (long long)nb2, inv);
-            orf_zgetrf(st, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, nb2, nullptr);
-            orf_zgetrs(st, 'L', Wy, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, inv, Wy, (long long)Wy * Wy, nb2);
-            bulk_gemm(P, 'N', 'T', Hx, Wy * Wy, P.Nsx, ONE, P.dFx.as<cplx>(), Hx, 0, inv, Wy * Wy,
-                      (long long)Wy * Wy * P.Nsx, ZERO, w.HH, Hx, (long long)Hx * Wy * Wy, Cp);
-            li_gather_kernel<<<g1(M2, Cp), 256, 0, st>>>(P.Nx, P.Ny, M, w.GH, w.HH, w.Exx, w.Eyy, M2);
-          } else {
-            HIPCHECK(hipMemcpyAsync(w.Exx, w.E, sizeof(cplx) * M2 * Cp, hipMemcpyDeviceToDevice, st));
-            HIPCHECK(hipMemcpyAsync(w.Eyy, w.E, sizeof(cplx) * M2 * Cp, hipMemcpyDeviceToDevice, st));
-          }
-          KCHECK();
-        }
-        {
-          STAGE("omega");
-          omega_rhs_kernel<<<g1(2 * M2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.Exx, w.Eyy, M2, w.G, 2 * M2);
-          orf_zgetrf(st, M, w.E, M, M2, w.ipiv, nullptr, Cp, w.dinv);
-          orf_zgetrs(st, 'L', M, 2 * M, w.E, M, M2, w.ipiv, w.dinv, w.G, M, 2 * M2, Cp);
-          omega_assemble_kernel<<<g1(nf2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.G, 2 * M2, w.Exx, w.Eyy,
-                                                             M2, w.Omega, nf2);
-          KCHECK();
-          if (P.reduced) {
-            sector_project_kernel<<<g1(n2, Cs), 256, 0, st>>>(nf, nr, w.solves, P.dSectors.as<DevSector>(), 0, 0,
-                                                               w.Omega, nf2, 0, w.A0, n2);
-          } else {
-            HIPCHECK(hipMemcpyAsync(w.A0, w.Omega, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-          }
-          KCHECK();
-        }
-        {
-          STAGE("eig");
-          orf_zgeev_timed(st, nr, w.A0, nr, n2, w.mu, w.Wm, nr, n2, w.Zw, w.Yw, w.Hv, w.info, Cs,
-                          [&](const char* nm, int phase) {
-                            if (phase == 0) P.timer.begin(nm, st); else P.timer.end(st);
-                          });
-          KCHECK();
-        }
-        check_info(P, w.info, Cs, "eigensolver (QR iteration did not converge)");
-        {
-          STAGE("modes");
-          lambda_kernel<<<g1((long long)nr * Cs), 256, 0, st>>>(nr, Cs, w.mu, w.lam);
-          cplx* Wf = w.Wm;
-          if (P.reduced) {
-            sector_expand_kernel<<<g1(nfr, Cs), 256, 0, st>>>(nf, nr, w.solves, P.dSectors.as<DevSector>(), w.Wm, n2,
-                                                               w.Wf, nfr);
-            Wf = w.Wf;
-          }
-          // T1 = Eyy W2, T2 = Exx W1 per sector
-          for (int s = 0; s < nSec; ++s) {
-            bulk_gemm(P, 'N', 'N', M, nr, M, ONE, w.Eyy, M, M2, Wf + s * nfr + M, nf, nSec * nfr, ZERO,
-                      w.T12 + s * nfr, nf, nSec * nfr, Cp);
-            bulk_gemm(P, 'N', 'N', M, nr, M, ONE, w.Exx, M, M2, Wf + s * nfr, nf, nSec * nfr, ZERO,
-                      w.T12 + s * nfr + M, nf, nSec * nfr, Cp);
-          }
-          qw_assemble_kernel<<<g1(nfr, Cs), 256, 0, st>>>(M, nr, w.solves, pages, P.dKxy.as<double>(), Wf, nfr, w.T12,
-                                                          nfr, w.QW, nfr);
-          if (P.reduced)
-            sector_rows_kernel<<<g1(n2, Cs), 256, 0, st>>>(nf, nr, nr, w.solves, P.dSectors.as<DevSector>(), w.QW, nfr,
-                                                           w.Vm, n2);
-          else
-            HIPCHECK(hipMemcpyAsync(w.Vm, w.QW, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-          colscale_inv_kernel<<<g1(n2, Cs), 256, 0, st>>>(nr, nr, w.lam, w.Vm, n2);
-          KCHECK();
-        }
+        patterned_modes(P, w, pages, se, si, Cp, Cs, P.keepModes ? P.kept[j] : nullptr);
       } else {
         STAGE("modes");
         std::vector<cplx> eps(Cs);
@@ -1318,30 +1525,7 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         }
       }
     }
-    // S = S * S_trn, source columns only
-    for (int k = 0; k < 4; k += 2)
-      gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dStrn.as<cplx>() + k * n2 * G, n2, w.Sb[k], n2);
-    const int ncol = P.reduced ? 1 : 2;
-    {
-      std::vector<int> cols((size_t)Cs * ncol);
-      for (int sp = 0; sp < Cs; ++sp) {
-        if (P.reduced) cols[sp] = P.sectorSrcCol[sv[sp].sector];
-        else { cols[2 * sp] = P.i0; cols[2 * sp + 1] = P.M + P.i0; }
-      }
-      HIPCHECK(hipMemcpyAsync(w.colIdx, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, st));
-    }
-    cplx* Rsel = w.Sn[0];
-    cplx* Tsel = w.Sn[2];
-    smatrix_star_cols(P, w, Cs, w.Sa, w.Sb, ncol, w.colIdx, Rsel, Tsel);
-    {
-      STAGE("output");
-      output_kernel<<<dim3(Cs, P.nPol), 256, 0, st>>>(
-          M, nr, P.reduced, p0, npages, w.solves, basePages, P.dSectors.as<DevSector>(), P.dSources.as<DevSource>(),
-          P.dKxy.as<double>(), (const double*)P.dKz.p, P.dKzi0.as<double>(), Rsel, Tsel, (long long)nr * ncol,
-          P.dR.as<double>(),
-          P.dT.as<double>(), P.dRtot.as<double>(), P.dTtot.as<double>(), P.dr.as<cplx>(), P.dt.as<cplx>(), P.i0);
-      KCHECK();
-    }
+    finish_chunk(P, w, p0, Cp, w.Sa, w.Sb, w.Sn[0], w.Sn[2]);
     if (progress && P.nLayers == 0) {
       if (progress((double)(ch + 1) / nChunks, "done", user)) fail(ORF_CANCELLED, "cancelled by the progress callback");
     }

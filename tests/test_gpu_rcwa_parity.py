@@ -214,3 +214,24 @@ def test_mixed_precision_error_is_bounded():
         np.testing.assert_allclose(g["T"][:, :, li, 0].T, o["T"], atol=1e-4, rtol=0)
         np.testing.assert_allclose(g["R"][:, :, li, 0].T, o["R"], atol=1e-4, rtol=0)
     assert np.max(np.abs(g["T"] - f["T"])) > 1e-12  # the FP32 products are really used
+
+
+@pytest.mark.parametrize("nslices", [2, 5, 8])
+def test_layer_parallel_tree_equals_sequential_and_oracle(nslices):
+    """Several layers: the all-layers-at-once path (no progress callback, star products as a
+    tree) equals the per-layer path (progress callback) and the oracle; odd layer counts carry."""
+    devs = []
+    for k, ang in enumerate((84.0, 88.0, 90.0)):
+        layers = orf.slantedLayers(0.5, mc(1), orf.shapeCircle(0.12 + 0.01 * k, mc(3.6 + 0.01j)), ang, NSlices=nslices)
+        layers.insert(1, orf.makeLayer(0.05, mc(1.9)))          # a homogeneous layer inside the stack
+        devs.append(orf.makeStructure([0.4, 0.4], mc(1.45), mc(1), layers))
+    for sym in ("auto", "none"):
+        fast = orf.rcwa2D(devs, 0.94, 0, 0, ["s", "p"], Nx=2, Nsample=(64, 64), Symmetry=sym)
+        seq = orf.rcwa2D(devs, 0.94, 0, 0, ["s", "p"], Nx=2, Nsample=(64, 64), Symmetry=sym,
+                         Progress=lambda f, m: None)
+        np.testing.assert_allclose(fast["T"], seq["T"], atol=1e-12, rtol=0)
+        np.testing.assert_allclose(fast["t"], seq["t"], atol=1e-12, rtol=0)
+        for pi, pol in enumerate("sp"):
+            o = O.rcwa2d(devs, 0.94, 0, 0, pol, Nx=2, Nsample=(64, 64), Symmetry=sym)
+            np.testing.assert_allclose(fast["T"][..., pi].T, o["T"], atol=TOL, rtol=0)
+            np.testing.assert_allclose(np.transpose(fast["r"][..., pi], (2, 1, 0)), o["r"], atol=TOL, rtol=0)
