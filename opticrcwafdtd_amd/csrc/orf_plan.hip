@@ -777,6 +777,9 @@ static void build_regions(orf_plan& P, const std::vector<zc>& epsInc, const std:
   HIPCHECK(hipStreamSynchronize(st));
 }
 
+// Solve pages per chunk in the all-layers-at-once execution.
+static const int kLayerChunkSolves = 4096;
+
 // ---------------------------------------------------------------- plan creation
 static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   if (!pr) fail(ORF_INVALID_ARG, "problem is NULL");
@@ -1148,7 +1151,10 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   cp = std::min(cp, 32768 / P.nSec);  // grid-dimension limit of the page-batched launches
   P.layerParallel = P.nLayers >= 2 && !P.keepModes;
   if (P.layerParallel) {
-    P.chunkPages = std::max(1, std::min(cp / P.nLayers, npages));
+    // every layer of a page is in the chunk at once: cap the chunk at kLayerChunkSolves
+    // solve pages (enough to fill the device) so that its workspace stays moderate
+    const int capPages = std::max(1, kLayerChunkSolves / (P.nLayers * P.nSec));
+    P.chunkPages = std::max(1, std::min(std::min(cp / P.nLayers, npages), capPages));
     if (pr->batchSize > 0) P.chunkPages = std::min(P.chunkPages, pr->nPages > 0 ? pr->batchSize : pr->batchSize * P.L);
     P.workModes = P.chunkPages * P.nLayers;
   } else {
@@ -1905,7 +1911,14 @@ int orf_plan_estimate(const orf_problem* p, int32_t reduced, int64_t* bytes) {
     }
     const size_t pages = p->nPages > 0 ? (size_t)p->nPages : (size_t)p->nStruct * p->nLambda;
     const size_t n2 = (size_t)P.nr * P.nr;
-    size_t b = pages * (per_mode_bytes(P) + (size_t)P.nSec * per_solve_bytes(P));
+    // workspace: one chunk (all pages; with several layers, every layer of at most
+    // kLayerChunkSolves solve pages' worth of pages)
+    size_t work = pages;
+    if (p->nLayers >= 2 && !p->keepModes) {
+      const size_t cap = (size_t)std::max(1, kLayerChunkSolves / (p->nLayers * P.nSec));
+      work = std::min(pages, cap) * p->nLayers;
+    }
+    size_t b = work * (per_mode_bytes(P) + (size_t)P.nSec * per_solve_bytes(P));
     b += pages * P.nSec * 9 * n2 * sizeof(cplx);          // half-space S-matrices and gap modes (upper bound)
     b += (size_t)p->nStruct * p->nLayers * ((size_t)P.Nsx * P.Nsy +
          sizeof(cplx) * ORF_MAX_SLOTS * ((size_t)(4 * P.Nx + 1) * (P.Nsy + 4 * P.Ny + 1) + (size_t)(4 * P.Ny + 1) * P.Nsx));
