@@ -1537,7 +1537,7 @@ ORF_DEV void wave_sync_lds() {
 // Multishift configuration: bulge positions per segment and the waves per SIMD
 // the register budget is cut for (several QR workgroups per CU hide the
 // latency of the dependent rotation chain).
-constexpr int MBQ_SEG = 16, MBQ_OCC = 2;
+constexpr int MBQ_SEG = 16, MBQ_OCC = 2, MBQ_GRP = 16;
 constexpr int MB_NBMAX = 8;
 
 // Eigenvalues of a small m x m (m <= 8) complex matrix held in LDS (ld 8), by
@@ -1597,7 +1597,7 @@ ORF_DEV void small_eigvals(int m, cplx* A, cplx* ev) {
   for (; i >= 0; --i) ev[i] = A[i + 8 * i];  // not converged: use the diagonal
 }
 
-template <int MB_NB, int MB_SEG, int OCC>
+template <int MB_NB, int MB_SEG, int OCC, int GRP>
 __global__ __launch_bounds__(256, OCC) void schur_qr_mb_kernel(int n, cplx* __restrict__ Hg, int ldh,
                                                           long long sH, cplx* __restrict__ Zg,
                                                           long long sZ, cplx* __restrict__ wout,
@@ -1828,15 +1828,15 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_mb_kernel(int n, cplx* __re
             const int g = a0 + q;
             v[q] = (g >= 0 && g < n) ? base[(ptrdiff_t)q * (ptrdiff_t)stride] : cmk(0.0, 0.0);
           }
-          // the compiler barrier after each bulge keeps one bulge's rotations live at a
-          // time (otherwise every rc/rs load is hoisted and the kernel runs at one wave
+          // a compiler barrier every GRP rotations keeps only that many rotations live
+          // (otherwise every rc/rs load is hoisted and the kernel runs at one wave
           // per SIMD)
           if (rowop) {
 #pragma unroll
             for (int b = 0; b < MB_NB; ++b) {
-              asm volatile("" ::: "memory");
 #pragma unroll
               for (int j = 0; j < MB_SEG; ++j) {
+                if (j % GRP == 0) asm volatile("" ::: "memory");
                 const int q = 3 * (MB_NB - 1 - b) + 1 + j;
                 const double c = rc[b * MB_SEG + j];
                 const cplx sn = rs[b * MB_SEG + j];
@@ -1848,9 +1848,9 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_mb_kernel(int n, cplx* __re
           } else {
 #pragma unroll
             for (int b = 0; b < MB_NB; ++b) {
-              asm volatile("" ::: "memory");
 #pragma unroll
               for (int j = 0; j < MB_SEG; ++j) {
+                if (j % GRP == 0) asm volatile("" ::: "memory");
                 const int q = 3 * (MB_NB - 1 - b) + 1 + j;
                 const double c = rc[b * MB_SEG + j];
                 const cplx sn = rs[b * MB_SEG + j];
@@ -2035,10 +2035,10 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
       const size_t lim2 = lim + 20 * 1024 - stat;
       const int hIn = bytes <= lim2, zIn = 2 * bytes <= lim2;
       if (nbul >= 8)
-        schur_qr_mb_kernel<8, 16, 1><<<batch, 256, (hIn + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w,
+        schur_qr_mb_kernel<8, 16, 1, 16><<<batch, 256, (hIn + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w,
                                                                               info, hIn, zIn);
       else
-        schur_qr_mb_kernel<4, MBQ_SEG, MBQ_OCC><<<batch, 256, (hIn + zIn) * bytes, st>>>(
+        schur_qr_mb_kernel<4, MBQ_SEG, MBQ_OCC, MBQ_GRP><<<batch, 256, (hIn + zIn) * bytes, st>>>(
             n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn);
     }
   }
