@@ -18,7 +18,8 @@
 //                 basis, complex QR to Schur form (Wilkinson / exceptional
 //                 shifts, Ahues-Tisseur deflation; small matrices one
 //                 rotation at a time in LDS, larger ones with several bulges
-//                 chased in lockstep through an LDS window), eigenvectors of
+//                 chased in lockstep through an LDS window and aggressive
+//                 early deflation on a small trailing window), eigenvectors of
 //                 the triangular factor by back substitution (LAPACK ztrevc's
 //                 small-pivot rule), back-transformed by one ZGEMM and
 //                 normalised to unit 2-norm.
@@ -843,12 +844,17 @@ __global__ __launch_bounds__(256) void ztrsm_right_kernel(int n, int nrows, int 
 }  // namespace
 
 static int g_lds_limit = 65536;
-static int g_qr_bulges = 4;      // bulges per multishift pass (4 or 8); 1 = per-rotation kernel
+static int g_qr_bulges = 4;      // 1 = the per-rotation kernel at every size
 static int g_qr_small_n = 96;    // up to this size the per-rotation LDS kernel is used
+static int g_qr_aed = 2;         // early deflation: 0 off, 1 reordering, 2 bottom only, 3 whole blocks only
 
 void orf_set_qr_params(int bulges, int small_n) {
   if (bulges > 0) g_qr_bulges = bulges;
   if (small_n >= 0) g_qr_small_n = small_n;
+}
+
+void orf_set_qr_aed(int mode) {
+  if (mode >= 0 && mode <= 3) g_qr_aed = mode;
 }
 
 void orf_set_lds_limit(int bytes) { g_lds_limit = bytes; }
@@ -1526,19 +1532,15 @@ ORF_DEV void wave_sync_lds() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Multi-shift variant: each pass chases up to MB_NB single-shift bulges at
-// once, with shifts taken as the eigenvalues of the trailing MB_NB x MB_NB
-// block (the choice of LAPACK's small-bulge multishift QR).  Bulge b trails
-// bulge b-1 by three positions; within a segment the bulges are processed one
-// after another (a reordering of commuting rotations), which gives every
-// rotation a fixed slot in a register strip of MB_SW columns.  The deferred
-// updates of the rows above the window, the columns right of it and Z then
-// serve all shifts of the pass with one load and one store per segment.
-// Multishift configuration: bulge positions per segment and the waves per SIMD
-// the register budget is cut for (several QR workgroups per CU hide the
-// latency of the dependent rotation chain).
-constexpr int MBQ_SEG = 16, MBQ_OCC = 2, MBQ_GRP = 16;
-constexpr int MB_NBMAX = 8;
+// Multishift configuration: bulges per pass, bulge positions per segment, the
+// waves per SIMD the register budget is cut for (two QR workgroups per CU hide
+// the latency of the dependent rotation chain), rotations between compiler
+// barriers in the strip updates, and the early-deflation window.  Measured on
+// the +-5 phase-library batch (n = 242): 4 bulges x 16 positions at 2
+// workgroups/CU beat 32 positions (longer chase) and 3-4 workgroups/CU
+// (spills); 8 bulges spill; windows of 8-12 beat 16-24 (the window's Schur
+// form is a serial rotation chain).
+constexpr int MBQ_NB = 4, MBQ_SEG = 16, MBQ_OCC = 2, MBQ_GRP = 16, MBQ_NW = 12;
 
 // Eigenvalues of a small m x m (m <= 8) complex matrix held in LDS (ld 8), by
 // single-shift QR without vectors; run by one thread.
@@ -1597,21 +1599,365 @@ ORF_DEV void small_eigvals(int m, cplx* A, cplx* ev) {
   for (; i >= 0; --i) ev[i] = A[i + 8 * i];  // not converged: use the diagonal
 }
 
-template <int MB_NB, int MB_SEG, int OCC, int GRP>
-__global__ __launch_bounds__(256, OCC) void schur_qr_mb_kernel(int n, cplx* __restrict__ Hg, int ldh,
-                                                          long long sH, cplx* __restrict__ Zg,
-                                                          long long sZ, cplx* __restrict__ wout,
-                                                          int* __restrict__ info, int hInLds,
-                                                          int zInLds) {
+// ------------------------------------------------- aggressive early deflation
+// Helpers run by ONE wave on an m x m window (m <= 32) held in LDS with
+// leading dimension LDW.  Every lane computes the scalar quantities (shifts,
+// rotations, reflectors) from the same LDS words; element updates are spread
+// over the lanes, one phase per wave_sync_lds.
+
+// Wilkinson shift of the trailing 2x2 block (rows/cols i-1, i) of W.
+template <int LDW>
+ORF_DEV cplx win_wilkinson(const cplx* W, int i) {
+  cplx tt = W[i + LDW * i];
+  const cplx u = cmul(csqrt(W[(i - 1) + LDW * i]), csqrt(W[i + LDW * (i - 1)]));
+  double sm = cabs1(u);
+  if (sm != 0.0) {
+    const cplx x = cscale(csub(W[(i - 1) + LDW * (i - 1)], tt), 0.5);
+    const double sx = cabs1(x);
+    sm = fmax(sm, sx);
+    const cplx xs = cscale(x, 1.0 / sm), us = cscale(u, 1.0 / sm);
+    cplx y = cscale(csqrt(cadd(cmul(xs, xs), cmul(us, us))), sm);
+    if (sx > 0.0) {
+      const cplx xx = cscale(x, 1.0 / sx);
+      if (xx.x * y.x + xx.y * y.y < 0.0) y = cneg(y);
+    }
+    tt = csub(tt, cmul(u, cdiv(u, cadd(x, y))));
+  }
+  return tt;
+}
+
+// Complex Schur form of the m x m Hessenberg window T by single-shift QR,
+// rotations accumulated into U (U <- U G^H).  0 on success, 1 when an
+// eigenvalue needs more than 30 sweeps (T and U then hold a partial, still
+// unitary, reduction).
+template <int LDW>
+ORF_DEV int win_schur(int m, cplx* T, cplx* U, int lane) {
+  const double smlnum = SAFMIN * ((double)m / ULP);
+  int hi = m - 1, its = 0;
+  while (hi > 0) {
+    bool small = false;
+    const int k = lane + 1;
+    if (k <= hi) {
+      const cplx hk = T[k + LDW * (k - 1)];
+      if (cabs1(hk) <= smlnum) small = true;
+      else {
+        double tst = cabs1(T[(k - 1) + LDW * (k - 1)]) + cabs1(T[k + LDW * k]);
+        if (tst == 0.0) {
+          if (k - 2 >= 0) tst += cabs1(T[(k - 1) + LDW * (k - 2)]);
+          if (k + 1 <= m - 1) tst += cabs1(T[(k + 1) + LDW * k]);
+        }
+        if (cabs1(hk) <= ULP * tst) {
+          const double ab = fmax(cabs1(hk), cabs1(T[(k - 1) + LDW * k]));
+          const double ba = fmin(cabs1(hk), cabs1(T[(k - 1) + LDW * k]));
+          const cplx dd = csub(T[(k - 1) + LDW * (k - 1)], T[k + LDW * k]);
+          const double aa = fmax(cabs1(T[k + LDW * k]), cabs1(dd));
+          const double bb = fmin(cabs1(T[k + LDW * k]), cabs1(dd));
+          const double s = aa + ab;
+          if (ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)))) small = true;
+        }
+      }
+    }
+    const unsigned long long msk = __ballot(small);
+    const int lo = msk ? 64 - __clzll(msk) : 0;  // largest k with a negligible T[k, k-1]
+    if (lo > 0 && lane == 0) T[lo + LDW * (lo - 1)] = cmk(0.0, 0.0);
+    wave_sync_lds();
+    if (lo >= hi) {
+      hi -= 1;
+      its = 0;
+      continue;
+    }
+    if (++its > 30) return 1;
+    cplx sig;
+    if (its == 10) sig = cadd(cmk(0.75 * cabs1(T[(lo + 1) + LDW * lo]), 0.0), T[lo + LDW * lo]);
+    else if (its == 20) sig = cadd(cmk(0.75 * cabs1(T[hi + LDW * (hi - 1)]), 0.0), T[hi + LDW * hi]);
+    else sig = win_wilkinson<LDW>(T, hi);
+    if (!(isfinite(sig.x) && isfinite(sig.y))) return 1;
+    for (int kk = lo; kk < hi; ++kk) {
+      cplx a, b;
+      if (kk == lo) { a = csub(T[lo + LDW * lo], sig); b = T[(lo + 1) + LDW * lo]; }
+      else { a = T[kk + LDW * (kk - 1)]; b = T[(kk + 1) + LDW * (kk - 1)]; }
+      double c;
+      cplx sn, r;
+      givens(a, b, c, sn, r);
+      // rows kk, kk+1 (columns kk..m-1) on lanes 0..31; U columns kk, kk+1 on lanes 32..63
+      if (lane < 32) {
+        const int j = kk + lane;
+        if (j < m) {
+          const cplx h1 = T[kk + LDW * j], h2 = T[(kk + 1) + LDW * j];
+          T[kk + LDW * j] = cadd(cscale(h1, c), cmul(sn, h2));
+          T[(kk + 1) + LDW * j] = csub(cscale(h2, c), cmulc(sn, h1));
+        }
+        if (lane == 0 && kk > lo) {
+          T[kk + LDW * (kk - 1)] = r;
+          T[(kk + 1) + LDW * (kk - 1)] = cmk(0.0, 0.0);
+        }
+      } else if (lane - 32 < m) {
+        const int q = lane - 32;
+        const cplx u1 = U[q + LDW * kk], u2 = U[q + LDW * (kk + 1)];
+        U[q + LDW * kk] = cadd(cscale(u1, c), cmulc(sn, u2));
+        U[q + LDW * (kk + 1)] = csub(cscale(u2, c), cmul(sn, u1));
+      }
+      wave_sync_lds();
+      // columns kk, kk+1 for rows 0..min(kk+2, hi)
+      if (lane <= min(kk + 2, hi)) {
+        const int q = lane;
+        const cplx h1 = T[q + LDW * kk], h2 = T[q + LDW * (kk + 1)];
+        T[q + LDW * kk] = cadd(cscale(h1, c), cmulc(sn, h2));
+        T[q + LDW * (kk + 1)] = csub(cscale(h2, c), cmul(sn, h1));
+      }
+      wave_sync_lds();
+    }
+  }
+  return 0;
+}
+
+// Swap the diagonal entries q, q+1 of the upper-triangular window T by one
+// rotation (the complex step of LAPACK ztrexc), carrying U along.
+template <int LDW>
+ORF_DEV void win_swap(int m, cplx* T, cplx* U, int q, int lane) {
+  const cplx t11 = T[q + LDW * q], t22 = T[(q + 1) + LDW * (q + 1)];
+  double c;
+  cplx sn, r;
+  givens(T[q + LDW * (q + 1)], csub(t22, t11), c, sn, r);
+  if (lane < 32) {
+    const int j = q + 2 + lane;  // rows q, q+1 right of the 2x2 block
+    if (j < m) {
+      const cplx h1 = T[q + LDW * j], h2 = T[(q + 1) + LDW * j];
+      T[q + LDW * j] = cadd(cscale(h1, c), cmul(sn, h2));
+      T[(q + 1) + LDW * j] = csub(cscale(h2, c), cmulc(sn, h1));
+    }
+    if (lane < q) {  // columns q, q+1 above the 2x2 block
+      const cplx h1 = T[lane + LDW * q], h2 = T[lane + LDW * (q + 1)];
+      T[lane + LDW * q] = cadd(cscale(h1, c), cmulc(sn, h2));
+      T[lane + LDW * (q + 1)] = csub(cscale(h2, c), cmul(sn, h1));
+    }
+    if (lane == 0) {
+      T[q + LDW * q] = t22;
+      T[(q + 1) + LDW * (q + 1)] = t11;
+    }
+  } else if (lane - 32 < m) {
+    const int p = lane - 32;
+    const cplx u1 = U[p + LDW * q], u2 = U[p + LDW * (q + 1)];
+    U[p + LDW * q] = cadd(cscale(u1, c), cmulc(sn, u2));
+    U[p + LDW * (q + 1)] = csub(cscale(u2, c), cmul(sn, u1));
+  }
+  wave_sync_lds();
+}
+
+// Householder reflector (LAPACK zlarfg convention): for x = [alpha; rest],
+// (I - conj(tau) v v^H) x = beta e1 with v = [1; rest / (alpha - beta)].
+// Returns tau; beta and the scale of the tail through the references.
+ORF_DEV cplx win_reflector(cplx alpha, double xnorm2, double& beta, cplx& scl) {
+  if (xnorm2 == 0.0 && alpha.y == 0.0) {
+    beta = alpha.x;
+    scl = cmk(0.0, 0.0);
+    return cmk(0.0, 0.0);
+  }
+  const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
+  beta = alpha.x >= 0.0 ? -nrm : nrm;
+  scl = crecip(csub(alpha, cmk(beta, 0.0)));
+  return cmk((beta - alpha.x) / beta, -alpha.y / beta);
+}
+
+// After deflation with ns > 1 undeflated eigenvalues and a nonzero spike
+// s * conj(U[0, :]): reflect the spike onto e1 and bring T[0:ns, 0:ns] back to
+// Hessenberg form (LAPACK zlaqr3's restore), updating T and U.  vv is LDS
+// scratch of LDW entries.
+template <int LDW>
+ORF_DEV void win_restore(int m, int ns, cplx* T, cplx* U, cplx* vv, int lane) {
+  // spike reflector on w = conj(U[0, 0:ns])
+  {
+    double xn2 = 0.0;
+    for (int k = 1; k < ns; ++k) xn2 += cnorm2(U[LDW * k]);
+    double beta;
+    cplx scl;
+    const cplx tau = win_reflector(cconj(U[0]), xn2, beta, scl);
+    if (lane < ns) vv[lane] = lane == 0 ? cmk(1.0, 0.0) : cmul(cconj(U[LDW * lane]), scl);
+    wave_sync_lds();
+    if (tau.x != 0.0 || tau.y != 0.0) {
+      // T[0:ns, :] <- (I - conj(tau) v v^H) T[0:ns, :]
+      if (lane < m) {
+        const int cc = lane;
+        cplx y = cmk(0.0, 0.0);
+        for (int k = 0; k < ns; ++k) y = cadd(y, cmulc(vv[k], T[k + LDW * cc]));
+        const cplx f = cmulc(tau, y);
+        for (int k = 0; k < ns; ++k) T[k + LDW * cc] = csub(T[k + LDW * cc], cmul(vv[k], f));
+      }
+      wave_sync_lds();
+      // T[0:ns, 0:ns] <- T (I - tau v v^H); U[:, 0:ns] likewise
+      if (lane < 32) {
+        if (lane < ns) {
+          const int rr = lane;
+          cplx y = cmk(0.0, 0.0);
+          for (int k = 0; k < ns; ++k) y = cadd(y, cmul(T[rr + LDW * k], vv[k]));
+          const cplx f = cmul(tau, y);
+          for (int k = 0; k < ns; ++k) T[rr + LDW * k] = csub(T[rr + LDW * k], cmul(f, cconj(vv[k])));
+        }
+      } else if (lane - 32 < m) {
+        const int rr = lane - 32;
+        cplx y = cmk(0.0, 0.0);
+        for (int k = 0; k < ns; ++k) y = cadd(y, cmul(U[rr + LDW * k], vv[k]));
+        const cplx f = cmul(tau, y);
+        for (int k = 0; k < ns; ++k) U[rr + LDW * k] = csub(U[rr + LDW * k], cmul(f, cconj(vv[k])));
+      }
+      wave_sync_lds();
+    }
+  }
+  // Hessenberg reduction of T[0:ns, 0:ns]; the reflectors act on indices >= 1,
+  // so the spike (now on e1) is untouched
+  for (int j = 0; j + 2 < ns; ++j) {
+    double xn2 = 0.0;
+    for (int k = j + 2; k < ns; ++k) xn2 += cnorm2(T[k + LDW * j]);
+    double beta;
+    cplx scl;
+    const cplx alpha = T[(j + 1) + LDW * j];
+    const cplx tau = win_reflector(alpha, xn2, beta, scl);
+    if (lane > j && lane < ns) vv[lane] = lane == j + 1 ? cmk(1.0, 0.0) : cmul(T[lane + LDW * j], scl);
+    wave_sync_lds();
+    if (tau.x == 0.0 && tau.y == 0.0) continue;
+    // left: rows j+1..ns-1 of columns j+1..m-1; column j becomes beta e1
+    if (lane < 32) {
+      const int cc = j + 1 + lane;
+      if (cc < m) {
+        cplx y = cmk(0.0, 0.0);
+        for (int k = j + 1; k < ns; ++k) y = cadd(y, cmulc(vv[k], T[k + LDW * cc]));
+        const cplx f = cmulc(tau, y);
+        for (int k = j + 1; k < ns; ++k) T[k + LDW * cc] = csub(T[k + LDW * cc], cmul(vv[k], f));
+      }
+    } else if (lane == 32) {
+      T[(j + 1) + LDW * j] = cmk(beta, 0.0);
+      for (int k = j + 2; k < ns; ++k) T[k + LDW * j] = cmk(0.0, 0.0);
+    }
+    wave_sync_lds();
+    // right: columns j+1..ns-1 of T rows 0..ns-1 and of U rows 0..m-1
+    if (lane < 32) {
+      if (lane < ns) {
+        const int rr = lane;
+        cplx y = cmk(0.0, 0.0);
+        for (int k = j + 1; k < ns; ++k) y = cadd(y, cmul(T[rr + LDW * k], vv[k]));
+        const cplx f = cmul(tau, y);
+        for (int k = j + 1; k < ns; ++k) T[rr + LDW * k] = csub(T[rr + LDW * k], cmul(f, cconj(vv[k])));
+      }
+    } else if (lane - 32 < m) {
+      const int rr = lane - 32;
+      cplx y = cmk(0.0, 0.0);
+      for (int k = j + 1; k < ns; ++k) y = cadd(y, cmul(U[rr + LDW * k], vv[k]));
+      const cplx f = cmul(tau, y);
+      for (int k = j + 1; k < ns; ++k) U[rr + LDW * k] = csub(U[rr + LDW * k], cmul(f, cconj(vv[k])));
+    }
+    wave_sync_lds();
+  }
+}
+
+// Deferred rotations of one multishift segment on one strip (a row or column
+// of MB_SEG + 3*MB_NB positions starting at matrix index a0, element stride
+// `stride`).  Rotations go in step order (step j: every bulge's rotation j, on
+// disjoint position pairs); positions 1+j .. 3*MB_NB-1+j are live at step j,
+// so a ring of 3*MB_NB + LA registers streams the strip: position 1+j is
+// stored after step j and position 3*MB_NB+LA+j fetched LA steps before it is
+// needed.  Compiler barriers keep the loads and stores where they are and only
+// a few rotations live.  CHECK = true when positions may fall outside the
+// matrix: their loads read zero and their stores go to a sink, by selects, so
+// the loop stays one basic block (branches around each access make the
+// register allocator spill).  Row pairs rotate with s, column pairs with conj(s).
+// Write sink for strip positions outside the matrix: a select between the real
+// address and this one keeps the strip loop free of branches.
+__device__ cplx g_strip_sink[1];
+
+template <int MB_NB, int MB_SEG, int GRP, int LA, bool CHECK>
+ORF_DEV void strip_rotations(cplx* base, ptrdiff_t stride, int a0, int n, bool rowop,
+                             const double* rc, const cplx* rs) {
+  constexpr int RING = 3 * MB_NB + LA;
+  constexpr int PLAST = MB_SEG + 3 * MB_NB - 2;  // last position a rotation touches
+  cplx* const sink = g_strip_sink;
+  auto inside = [&](int p) { return !CHECK || (a0 + p >= 0 && a0 + p < n); };
+  cplx v[RING];
+#pragma unroll
+  for (int p = 1; p < 3 * MB_NB + LA && p <= PLAST; ++p) {
+    const cplx x = *(inside(p) ? base + p * stride : sink);
+    v[p % RING] = inside(p) ? x : cmk(0.0, 0.0);
+  }
+  cplx* pst = base + stride;
+  cplx* pld = base + (ptrdiff_t)(3 * MB_NB + LA) * stride;
+#pragma unroll
+  for (int j = 0; j < MB_SEG; ++j) {
+    if ((j * MB_NB) % GRP == 0) asm volatile("" : "+v"(pst), "+v"(pld) : : "memory");
+#pragma unroll
+    for (int b = 0; b < MB_NB; ++b) {
+      const int q = 3 * (MB_NB - 1 - b) + 1 + j;
+      const double c = rc[b * MB_SEG + j];
+      const cplx sr = rs[b * MB_SEG + j];
+      const cplx sn = rowop ? sr : cconj(sr);
+      const cplx h1 = v[q % RING], h2 = v[(q + 1) % RING];
+      v[q % RING] = cadd(cscale(h1, c), cmul(sn, h2));
+      v[(q + 1) % RING] = csub(cscale(h2, c), cmulc(sn, h1));
+    }
+    {
+      const int p = 1 + j;
+      *(inside(p) ? pst : sink) = v[p % RING];
+      pst += stride;
+    }
+    {
+      const int p = 3 * MB_NB + LA + j;
+      if (p <= PLAST) {
+        const cplx x = *(inside(p) ? pld : sink);
+        v[p % RING] = inside(p) ? x : cmk(0.0, 0.0);
+        pld += stride;
+      }
+    }
+  }
+#pragma unroll
+  for (int p = MB_SEG + 1; p <= PLAST; ++p) {
+    *(inside(p) ? pst : sink) = v[p % RING];
+    pst += stride;
+  }
+}
+
+// Multishift QR with aggressive early deflation (Braman, Byers & Mathias 2002,
+// the scheme of LAPACK's zlaqr0/zlaqr3).
+//  * Passes: MB_NB single-shift bulges chased in lockstep through an LDS
+//    window by one wave (bulge b trails bulge b-1 by three positions, so the
+//    rotations of one step act on disjoint pairs), MB_SEG positions per
+//    segment; the rows above the window, the columns right of it and Z take
+//    the segment's rotations afterwards, streamed through a register ring by
+//    all threads (strip_rotations): one load and one store per element per
+//    segment serve every shift of the pass.
+//  * Early deflation (aedMode 0 and 1): before each pass the trailing NW x NW
+//    window is brought to Schur form in LDS (win_schur); eigenvalues whose
+//    spike entry is negligible deflate at once, from the bottom (mode 1) or
+//    with the undeflatable ones moved up (mode 0, LAPACK's reordering), and
+//    the remaining ones are the next pass's shifts.  An active block of at most
+//    NW + 4 rows is finished in the window outright (all modes but 3; mode 2
+//    does only that, mode 3 no early deflation at all).  The window's unitary
+//    factor reaches the rows above, the columns right and Z in one strip
+//    update instead of one per pass: on the +-5 batch this cuts the QR's HBM
+//    traffic by about 45 %.
+template <int MB_NB, int MB_SEG, int OCC, int GRP, int NW>
+__global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __restrict__ Hg, int ldh,
+                                                           long long sH, cplx* __restrict__ Zg,
+                                                           long long sZ, cplx* __restrict__ wout,
+                                                           int* __restrict__ info, int hInLds,
+                                                           int zInLds, int aedMode) {
   constexpr int MB_SW = MB_SEG + 3 * (MB_NB - 1) + 3;
+  constexpr int NWX = NW + 4;       // largest window (a whole small block)
+  constexpr int LDW = NWX + 1;
+  constexpr int NIBBLE = (NW * 14) / 100;  // deflations that make another window worth it
+  static_assert(NWX <= 32, "window lanes");
+  constexpr int WINSZ = MB_SW * MB_SW > LDW * NWX ? MB_SW * MB_SW : LDW * NWX;
+  constexpr int LA = MB_NB >= 8 ? 4 : 8;         // strip fetch lookahead (steps)
+  constexpr int RING = 3 * MB_NB + LA;           // strip registers per task
+  constexpr int PLAST = MB_SEG + 3 * MB_NB - 2;  // last strip position a rotation touches
   extern __shared__ cplx lds_mat[];
-  __shared__ cplx win[MB_SW * MB_SW];
+  __shared__ cplx win[WINSZ];       // chase window, or the AED window T
+  __shared__ cplx Uw[LDW * NWX];
+  __shared__ cplx vv[LDW];
   __shared__ double rc[MB_NB * MB_SEG];
   __shared__ cplx rs[MB_NB * MB_SEG];
   __shared__ cplx sm8[64];
   __shared__ cplx shifts[MB_NB];
   __shared__ cplx rr8[MB_NB];
   __shared__ int redk[4];
+  __shared__ int aedres[2];
   const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
   const int lane = t & 63, wv = t >> 6, nw = nt >> 6;
   Hg += (size_t)page * sH;
@@ -1634,254 +1980,332 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_mb_kernel(int n, cplx* __re
   __syncthreads();
 #define HH(r, c) H[(size_t)(r) + (size_t)(c) * ld]
 #define WN(r, c) win[(r) + (c) * MB_SW]
+#define TW(r, c) win[(r) + (c) * LDW]
+#define UW(r, c) Uw[(r) + (c) * LDW]
   const double smlnum = SAFMIN * ((double)n / ULP);
   const int itmax = 60;
   int failed = 0;
   int i = n - 1;
+  int its = 0;
   while (i >= 0) {
-    int l = 0;
-    int its;
-    for (its = 0; its <= itmax; ++its) {
-      int kfound = l;
-      for (int k = i - t; k > l; k -= nt) {
-        const cplx hkk1 = HH(k, k - 1);
-        bool small = false;
-        if (cabs1(hkk1) <= smlnum) small = true;
-        else {
-          double tst = cabs1(HH(k - 1, k - 1)) + cabs1(HH(k, k));
-          if (tst == 0.0) {
-            if (k - 2 >= 0) tst += cabs1(HH(k - 1, k - 2));
-            if (k + 1 <= n - 1) tst += cabs1(HH(k + 1, k));
-          }
-          if (cabs1(hkk1) <= ULP * tst) {
-            const double ab = fmax(cabs1(hkk1), cabs1(HH(k - 1, k)));
-            const double ba = fmin(cabs1(hkk1), cabs1(HH(k - 1, k)));
-            const cplx dd = csub(HH(k - 1, k - 1), HH(k, k));
-            const double aa = fmax(cabs1(HH(k, k)), cabs1(dd));
-            const double bb = fmin(cabs1(HH(k, k)), cabs1(dd));
-            const double s = aa + ab;
-            if (ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)))) small = true;
-          }
+    // ---- deflation search: largest k in (0, i] with negligible H[k, k-1]
+    int kfound = 0;
+    for (int k = i - t; k > 0; k -= nt) {
+      const cplx hkk1 = HH(k, k - 1);
+      bool small = false;
+      if (cabs1(hkk1) <= smlnum) small = true;
+      else {
+        double tst = cabs1(HH(k - 1, k - 1)) + cabs1(HH(k, k));
+        if (tst == 0.0) {
+          if (k - 2 >= 0) tst += cabs1(HH(k - 1, k - 2));
+          if (k + 1 <= n - 1) tst += cabs1(HH(k + 1, k));
         }
-        if (small) { kfound = k; break; }
+        if (cabs1(hkk1) <= ULP * tst) {
+          const double ab = fmax(cabs1(hkk1), cabs1(HH(k - 1, k)));
+          const double ba = fmin(cabs1(hkk1), cabs1(HH(k - 1, k)));
+          const cplx dd = csub(HH(k - 1, k - 1), HH(k, k));
+          const double aa = fmax(cabs1(HH(k, k)), cabs1(dd));
+          const double bb = fmin(cabs1(HH(k, k)), cabs1(dd));
+          const double s = aa + ab;
+          if (ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)))) small = true;
+        }
       }
-      int kv = kfound;
-      for (int o = 32; o > 0; o >>= 1) kv = max(kv, __shfl_xor(kv, o, 64));
+      if (small) { kfound = k; break; }
+    }
+    int kv = kfound;
+    for (int o = 32; o > 0; o >>= 1) kv = max(kv, __shfl_xor(kv, o, 64));
+    __syncthreads();
+    if (lane == 0) redk[wv] = kv;
+    __syncthreads();
+    int l = redk[0];
+    for (int q = 1; q < nw; ++q) l = max(l, redk[q]);
+    if (l > 0) {
       __syncthreads();
-      if (lane == 0) redk[wv] = kv;
+      if (t == 0) HH(l, l - 1) = cmk(0.0, 0.0);
       __syncthreads();
-      int kmax = redk[0];
-      for (int q = 1; q < nw; ++q) kmax = max(kmax, redk[q]);
-      l = kmax;
-      if (l > 0) {
-        __syncthreads();
-        if (t == 0) HH(l, l - 1) = cmk(0.0, 0.0);
-        __syncthreads();
+    }
+    if (l >= i) {  // 1x1 block at the bottom
+      if (t == 0) wout[i] = HH(i, i);
+      i -= 1;
+      its = 0;
+      __syncthreads();
+      continue;
+    }
+    if (its > itmax) {  // give up on eigenvalue i; keep going so every wave exits
+      if (!failed) failed = i + 1;
+      if (t == 0) wout[i] = HH(i, i);
+      i -= 1;
+      its = 0;
+      __syncthreads();
+      continue;
+    }
+    int na = i - l + 1;
+    // ---- aggressive early deflation on the window [kw, i] (or the whole block)
+    int nsh = 0;  // AED shifts available in shifts[]
+    if (aedMode != 3 && (na <= NWX || aedMode != 2)) {
+      const bool whole = na <= NWX;
+      const int m = whole ? na : NW;
+      const int kw = i - m + 1;
+      const cplx s = whole ? cmk(0.0, 0.0) : HH(kw, kw - 1);
+      for (int q = t; q < LDW * NWX; q += nt) {
+        const int r = q % LDW, c = q / LDW;
+        win[q] = (r < m && c < m && r <= c + 1) ? HH(kw + r, kw + c) : cmk(0.0, 0.0);
+        Uw[q] = cmk(r == c ? 1.0 : 0.0, 0.0);
       }
-      if (l >= i) break;
-      // ---- shifts
-      const int na = i - l + 1;
-      const bool exc = (its > 0 && its % 10 == 0);
-      const int nbul = (exc || na < 3 * MB_NB + 4) ? 1 : MB_NB;
-      if (nbul == 1) {
-        cplx sig;
-        if (its % 20 == 10) {
-          sig = cadd(cmk(0.75 * cabs1(HH(l + 1, l)), 0.0), HH(l, l));
-        } else if (its > 0 && its % 20 == 0) {
-          sig = cadd(cmk(0.75 * cabs1(HH(i, i - 1)), 0.0), HH(i, i));
-        } else {
-          cplx tt = HH(i, i);
-          const cplx u = cmul(csqrt(HH(i - 1, i)), csqrt(HH(i, i - 1)));
-          double s = cabs1(u);
-          if (s != 0.0) {
-            const cplx x = cscale(csub(HH(i - 1, i - 1), tt), 0.5);
-            const double sx = cabs1(x);
-            s = fmax(s, cabs1(x));
-            const cplx xs = cscale(x, 1.0 / s), us = cscale(u, 1.0 / s);
-            cplx y = cscale(csqrt(cadd(cmul(xs, xs), cmul(us, us))), s);
-            if (sx > 0.0) {
-              const cplx xx = cscale(x, 1.0 / sx);
-              if (xx.x * y.x + xx.y * y.y < 0.0) y = cneg(y);
-            }
-            tt = csub(tt, cmul(u, cdiv(u, cadd(x, y))));
-          }
-          sig = tt;
-        }
-        __syncthreads();
-        if (t == 0) shifts[0] = sig;
-        __syncthreads();
-      } else {
-        __syncthreads();
-        if (t < MB_NB * MB_NB) {
-          const int r = t % MB_NB, c = t / MB_NB;
-          sm8[r + 8 * c] = HH(i - MB_NB + 1 + r, i - MB_NB + 1 + c);
-        }
-        __syncthreads();
-        if (t == 0) small_eigvals(MB_NB, sm8, shifts);
-        __syncthreads();
-      }
-      bool bad = false;
-      for (int b = 0; b < nbul; ++b) bad |= !(isfinite(shifts[b].x) && isfinite(shifts[b].y));
-      if (bad) {
-        its = itmax + 1;
-        break;
-      }
-      // ---- one pass: segments of MB_SEG positions of bulge 0
-      for (int p0 = l; p0 <= i - 1 + 3 * (nbul - 1); p0 += MB_SEG) {
-        const int a0 = p0 - 3 * (MB_NB - 1) - 1;  // window origin (may be < 0)
-        __syncthreads();
-        for (int q = t; q < MB_SW * MB_SW; q += nt) {
-          const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
-          win[q] = (r >= 0 && r < n && c >= 0 && c < n) ? HH(r, c) : cmk(0.0, 0.0);
-        }
-        __syncthreads();
-        if (wv == 0) {
-          // lockstep: at step j bulge b sits at position p0 - 3b + j; the rotations
-          // of one step act on disjoint row and column pairs
-          for (int j = 0; j < MB_SEG; ++j) {
-            // A: lane b computes bulge b's rotation
-            if (lane < MB_NB) {
-              const int b = lane, k = p0 - 3 * b + j;
-              const bool valid = b < nbul && k >= l && k <= i - 1;
-              double c = 1.0;
-              cplx sn = cmk(0.0, 0.0), r = cmk(0.0, 0.0);
-              if (valid) {
-                const int kk = k - a0;
-                cplx x, y;
-                if (k == l) { x = csub(WN(kk, kk), shifts[b]); y = WN(kk + 1, kk); }
-                else { x = WN(kk, kk - 1); y = WN(kk + 1, kk - 1); }
-                givens(x, y, c, sn, r);
-              }
-              rc[b * MB_SEG + j] = c;
-              rs[b * MB_SEG + j] = sn;
-              rr8[b] = r;
-            }
-            wave_sync_lds();
-            // B: row ops of every bulge on window columns kk..MB_SW-1
-            for (int e = lane; e < MB_NB * MB_SW; e += 64) {
-              const int b = e / MB_SW, jc = e % MB_SW;
-              const int k = p0 - 3 * b + j;
-              if (!(b < nbul && k >= l && k <= i - 1)) continue;
-              const int kk = k - a0;
-              if (jc < kk || a0 + jc >= n) continue;
-              const double c = rc[b * MB_SEG + j];
-              const cplx sn = rs[b * MB_SEG + j];
-              const cplx h1 = WN(kk, jc), h2 = WN(kk + 1, jc);
-              WN(kk, jc) = cadd(cscale(h1, c), cmul(sn, h2));
-              WN(kk + 1, jc) = csub(cscale(h2, c), cmul(cconj(sn), h1));
-            }
-            if (lane < MB_NB) {
-              const int b = lane, k = p0 - 3 * b + j;
-              if (b < nbul && k > l && k <= i - 1) {
-                const int kk = k - a0;
-                WN(kk, kk - 1) = rr8[b];
-                WN(kk + 1, kk - 1) = cmk(0.0, 0.0);
+      __syncthreads();
+      if (wv == 0) {
+        int ns = -1;
+        if (win_schur<LDW>(m, win, Uw, lane) == 0) {
+          ns = m;
+          if (s.x == 0.0 && s.y == 0.0) {
+            ns = 0;
+          } else {
+            // deflation with reordering: negligible spike entries deflate from the
+            // bottom; an undeflatable eigenvalue is moved to the top (LAPACK zlaqr3)
+            int ilst = 0;
+            const double as = cabs1(s);
+            while (ilst < ns) {
+              const int k = ns - 1;
+              double foo = cabs1(TW(k, k));
+              if (foo == 0.0) foo = as;
+              if (as * cabs1(UW(0, k)) <= fmax(smlnum, ULP * foo)) {
+                ns -= 1;
+              } else if (aedMode == 1) {
+                break;
+              } else {
+                for (int q = k - 1; q >= ilst; --q) win_swap<LDW>(m, win, Uw, q, lane);
+                ilst += 1;
               }
             }
-            wave_sync_lds();
-            // C: column ops of every bulge on window rows 0..min(kk+2, i-a0)
-            for (int e = lane; e < MB_NB * MB_SW; e += 64) {
-              const int b = e / MB_SW, q = e % MB_SW;
-              const int k = p0 - 3 * b + j;
-              if (!(b < nbul && k >= l && k <= i - 1)) continue;
-              const int kk = k - a0;
-              if (q > min(kk + 2, i - a0) || a0 + q < 0) continue;
-              const double c = rc[b * MB_SEG + j];
-              const cplx sn = rs[b * MB_SEG + j];
-              const cplx h1 = WN(q, kk), h2 = WN(q, kk + 1);
-              WN(q, kk) = cadd(cscale(h1, c), cmul(cconj(sn), h2));
-              WN(q, kk + 1) = csub(cscale(h2, c), cmul(sn, h1));
-            }
-            wave_sync_lds();
           }
+          if (ns >= MB_NB && lane < MB_NB) shifts[lane] = TW(ns - MB_NB + lane, ns - MB_NB + lane);
+          if (ns < m && ns > 1) win_restore<LDW>(m, ns, win, Uw, vv, lane);
         }
-        __syncthreads();
-        for (int q = t; q < MB_SW * MB_SW; q += nt) {
-          const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
-          if (r >= 0 && r < n && c >= 0 && c < n) HH(r, c) = win[q];
+        if (lane == 0) aedres[0] = ns;
+      }
+      __syncthreads();
+      const int ns = aedres[0];
+      if (ns >= 0 && ns < m) {
+        // write the window back, then the strip update with Uw
+        for (int q = t; q < m * m; q += nt) {
+          const int r = q % m, c = q / m;
+          HH(kw + r, kw + c) = r <= c + 1 ? TW(r, c) : cmk(0.0, 0.0);
         }
-        // deferred: columns right of the window, rows above it, rows of Z
-        const int e0 = a0 + MB_SW;                    // first column right of the window
-        const int ncR = e0 < n ? n - e0 : 0;
-        const int nrA = a0 > 0 ? a0 : 0;
+        if (t == 0 && !whole) HH(kw, kw - 1) = cmul(s, cconj(UW(0, 0)));
+        for (int q = ns + t; q < m; q += nt) wout[kw + q] = TW(q, q);
+        const int e0 = kw + m;  // first column right of the window
+        const int ncR = n - e0, nrA = kw;
         const int tasks = ncR + nrA + n;
         for (int task = t; task < tasks; task += nt) {
-          cplx v[MB_SW];
+          cplx v[NWX];
           cplx* base;
           size_t stride;
-          bool rowop;
+          bool left;
           if (task < ncR) {
-            base = H + (size_t)(e0 + task) * ld + a0;  // column e0+task, rows a0..
+            base = H + (size_t)(e0 + task) * ld + kw;  // column e0+task, rows kw..kw+m-1
             stride = 1;
-            rowop = true;
+            left = true;
           } else if (task < ncR + nrA) {
-            base = H + (task - ncR) + (size_t)a0 * ld;  // row task-ncR, columns a0..
+            base = H + (task - ncR) + (size_t)kw * ld;  // row, columns kw..kw+m-1
             stride = ld;
-            rowop = false;
+            left = false;
           } else {
-            base = Z + (task - ncR - nrA) + (ptrdiff_t)a0 * n;
+            base = Z + (task - ncR - nrA) + (size_t)kw * n;
             stride = n;
-            rowop = false;
+            left = false;
           }
 #pragma unroll
-          for (int q = 0; q < MB_SW; ++q) {
-            const int g = a0 + q;
-            v[q] = (g >= 0 && g < n) ? base[(ptrdiff_t)q * (ptrdiff_t)stride] : cmk(0.0, 0.0);
-          }
-          // a compiler barrier every GRP rotations keeps only that many rotations live
-          // (otherwise every rc/rs load is hoisted and the kernel runs at one wave
-          // per SIMD)
-          if (rowop) {
+          for (int q = 0; q < NWX; ++q) v[q] = q < m ? base[(size_t)q * stride] : cmk(0.0, 0.0);
+          if (left) {  // U^H x
+            for (int j = 0; j < m; ++j) {
+              cplx acc = cmk(0.0, 0.0);
 #pragma unroll
-            for (int b = 0; b < MB_NB; ++b) {
-#pragma unroll
-              for (int j = 0; j < MB_SEG; ++j) {
-                if (j % GRP == 0) asm volatile("" ::: "memory");
-                const int q = 3 * (MB_NB - 1 - b) + 1 + j;
-                const double c = rc[b * MB_SEG + j];
-                const cplx sn = rs[b * MB_SEG + j];
-                const cplx h1 = v[q], h2 = v[q + 1];
-                v[q] = cadd(cscale(h1, c), cmul(sn, h2));
-                v[q + 1] = csub(cscale(h2, c), cmul(cconj(sn), h1));
-              }
+              for (int q = 0; q < NWX; ++q) acc = cadd(acc, cmulc(UW(q, j), v[q]));
+              base[(size_t)j * stride] = acc;
             }
-          } else {
+          } else {  // x^T U
+            for (int j = 0; j < m; ++j) {
+              cplx acc = cmk(0.0, 0.0);
 #pragma unroll
-            for (int b = 0; b < MB_NB; ++b) {
-#pragma unroll
-              for (int j = 0; j < MB_SEG; ++j) {
-                if (j % GRP == 0) asm volatile("" ::: "memory");
-                const int q = 3 * (MB_NB - 1 - b) + 1 + j;
-                const double c = rc[b * MB_SEG + j];
-                const cplx sn = rs[b * MB_SEG + j];
-                const cplx h1 = v[q], h2 = v[q + 1];
-                v[q] = cadd(cscale(h1, c), cmul(cconj(sn), h2));
-                v[q + 1] = csub(cscale(h2, c), cmul(sn, h1));
-              }
+              for (int q = 0; q < NWX; ++q) acc = cadd(acc, cmul(v[q], UW(q, j)));
+              base[(size_t)j * stride] = acc;
             }
           }
-#pragma unroll
-          for (int q = 0; q < MB_SW; ++q) {
-            const int g = a0 + q;
-            if (g >= 0 && g < n) base[(ptrdiff_t)q * (ptrdiff_t)stride] = v[q];
+        }
+        __syncthreads();
+        its = 0;
+        if (ns == 0) {  // the window deflated completely
+          i = kw - 1;
+          continue;
+        }
+        i = kw + ns - 1;
+        na = i - l + 1;
+        if (m - ns > NIBBLE || ns < MB_NB || na <= NWX) continue;
+      }
+      if (ns >= MB_NB) nsh = MB_NB;
+    }
+    // ---- one multishift pass on [l, i]
+    its += 1;
+    const bool exc = its % 10 == 0;
+    const int nbul = (exc || na < 3 * MB_NB + 4) ? 1 : MB_NB;
+    if (nbul == 1) {
+      cplx sig;
+      if (its % 20 == 10) {
+        sig = cadd(cmk(0.75 * cabs1(HH(l + 1, l)), 0.0), HH(l, l));
+      } else if (its % 20 == 0) {
+        sig = cadd(cmk(0.75 * cabs1(HH(i, i - 1)), 0.0), HH(i, i));
+      } else {
+        cplx tt = HH(i, i);
+        const cplx u = cmul(csqrt(HH(i - 1, i)), csqrt(HH(i, i - 1)));
+        double s = cabs1(u);
+        if (s != 0.0) {
+          const cplx x = cscale(csub(HH(i - 1, i - 1), tt), 0.5);
+          const double sx = cabs1(x);
+          s = fmax(s, cabs1(x));
+          const cplx xs = cscale(x, 1.0 / s), us = cscale(u, 1.0 / s);
+          cplx y = cscale(csqrt(cadd(cmul(xs, xs), cmul(us, us))), s);
+          if (sx > 0.0) {
+            const cplx xx = cscale(x, 1.0 / sx);
+            if (xx.x * y.x + xx.y * y.y < 0.0) y = cneg(y);
           }
+          tt = csub(tt, cmul(u, cdiv(u, cadd(x, y))));
+        }
+        sig = tt;
+      }
+      __syncthreads();
+      if (t == 0) shifts[0] = sig;
+      __syncthreads();
+    } else if (nsh < MB_NB) {
+      __syncthreads();
+      if (t < MB_NB * MB_NB) {
+        const int r = t % MB_NB, c = t / MB_NB;
+        sm8[r + 8 * c] = HH(i - MB_NB + 1 + r, i - MB_NB + 1 + c);
+      }
+      __syncthreads();
+      if (t == 0) small_eigvals(MB_NB, sm8, shifts);
+      __syncthreads();
+    }
+    bool bad = false;
+    for (int b = 0; b < nbul; ++b) bad |= !(isfinite(shifts[b].x) && isfinite(shifts[b].y));
+    if (bad) {  // non-finite input: give up on this block
+      if (!failed) failed = i + 1;
+      for (int q = l + t; q <= i; q += nt) wout[q] = HH(q, q);
+      i = l - 1;
+      its = 0;
+      __syncthreads();
+      continue;
+    }
+    // ---- one pass: segments of MB_SEG positions of bulge 0
+    for (int p0 = l; p0 <= i - 1 + 3 * (nbul - 1); p0 += MB_SEG) {
+      const int a0 = p0 - 3 * (MB_NB - 1) - 1;  // window origin (may be < 0)
+      __syncthreads();
+      for (int q = t; q < MB_SW * MB_SW; q += nt) {
+        const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
+        win[q] = (r >= 0 && r < n && c >= 0 && c < n) ? HH(r, c) : cmk(0.0, 0.0);
+      }
+      __syncthreads();
+      if (wv == 0) {
+        // lockstep: at step j bulge b sits at position p0 - 3b + j; the rotations
+        // of one step act on disjoint row and column pairs
+        for (int j = 0; j < MB_SEG; ++j) {
+          // A: lane b computes bulge b's rotation
+          if (lane < MB_NB) {
+            const int b = lane, k = p0 - 3 * b + j;
+            const bool valid = b < nbul && k >= l && k <= i - 1;
+            double c = 1.0;
+            cplx sn = cmk(0.0, 0.0), r = cmk(0.0, 0.0);
+            if (valid) {
+              const int kk = k - a0;
+              cplx x, y;
+              if (k == l) { x = csub(WN(kk, kk), shifts[b]); y = WN(kk + 1, kk); }
+              else { x = WN(kk, kk - 1); y = WN(kk + 1, kk - 1); }
+              givens(x, y, c, sn, r);
+            }
+            rc[b * MB_SEG + j] = c;
+            rs[b * MB_SEG + j] = sn;
+            rr8[b] = r;
+          }
+          wave_sync_lds();
+          // B: row ops of every bulge on window columns kk..MB_SW-1
+          for (int e = lane; e < MB_NB * MB_SW; e += 64) {
+            const int b = e / MB_SW, jc = e % MB_SW;
+            const int k = p0 - 3 * b + j;
+            if (!(b < nbul && k >= l && k <= i - 1)) continue;
+            const int kk = k - a0;
+            if (jc < kk || a0 + jc >= n) continue;
+            const double c = rc[b * MB_SEG + j];
+            const cplx sn = rs[b * MB_SEG + j];
+            const cplx h1 = WN(kk, jc), h2 = WN(kk + 1, jc);
+            WN(kk, jc) = cadd(cscale(h1, c), cmul(sn, h2));
+            WN(kk + 1, jc) = csub(cscale(h2, c), cmul(cconj(sn), h1));
+          }
+          if (lane < MB_NB) {
+            const int b = lane, k = p0 - 3 * b + j;
+            if (b < nbul && k > l && k <= i - 1) {
+              const int kk = k - a0;
+              WN(kk, kk - 1) = rr8[b];
+              WN(kk + 1, kk - 1) = cmk(0.0, 0.0);
+            }
+          }
+          wave_sync_lds();
+          // C: column ops of every bulge on window rows 0..min(kk+2, i-a0)
+          for (int e = lane; e < MB_NB * MB_SW; e += 64) {
+            const int b = e / MB_SW, q = e % MB_SW;
+            const int k = p0 - 3 * b + j;
+            if (!(b < nbul && k >= l && k <= i - 1)) continue;
+            const int kk = k - a0;
+            if (q > min(kk + 2, i - a0) || a0 + q < 0) continue;
+            const double c = rc[b * MB_SEG + j];
+            const cplx sn = rs[b * MB_SEG + j];
+            const cplx h1 = WN(q, kk), h2 = WN(q, kk + 1);
+            WN(q, kk) = cadd(cscale(h1, c), cmul(cconj(sn), h2));
+            WN(q, kk + 1) = csub(cscale(h2, c), cmul(sn, h1));
+          }
+          wave_sync_lds();
         }
       }
       __syncthreads();
+      for (int q = t; q < MB_SW * MB_SW; q += nt) {
+        const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
+        if (r >= 0 && r < n && c >= 0 && c < n) HH(r, c) = win[q];
+      }
+      // deferred: columns right of the window, rows above it, rows of Z
+      const int e0 = a0 + MB_SW;                    // first column right of the window
+      const int ncR = e0 < n ? n - e0 : 0;
+      const int nrA = a0 > 0 ? a0 : 0;
+      const int tasks = ncR + nrA + n;
+      for (int task = t; task < tasks; task += nt) {
+        cplx* base;
+        size_t stride;
+        bool rowop;
+        if (task < ncR) {
+          base = H + (size_t)(e0 + task) * ld + a0;  // column e0+task, rows a0..
+          stride = 1;
+          rowop = true;
+        } else if (task < ncR + nrA) {
+          base = H + (task - ncR) + (size_t)a0 * ld;  // row task-ncR, columns a0..
+          stride = ld;
+          rowop = false;
+        } else {
+          base = Z + (task - ncR - nrA) + (ptrdiff_t)a0 * n;
+          stride = n;
+          rowop = false;
+        }
+        // one instance only: with a checked and an unchecked copy behind a branch the
+        // compiler hoists every rotation's LDS loads above it and spills
+        strip_rotations<MB_NB, MB_SEG, GRP, LA, true>(base, (ptrdiff_t)stride, a0, n, rowop, rc, rs);
+      }
     }
-    if (its > itmax) {
-      if (!failed) failed = i + 1;
-      l = i;
-    }
-    if (t == 0) wout[i] = HH(i, i);
-    i = l - 1;
     __syncthreads();
   }
+
   if (t == 0 && info) info[page] = failed;
   if (hInLds)
     for (int e = t; e < n * n; e += nt) Hg[(size_t)(e % n) + (size_t)(e / n) * ldh] = H[e];
   if (zInLds)
     for (int e = t; e < n * n; e += nt) Zp[e] = Z[e];
+#undef UW
+#undef TW
 #undef WN
 #undef HH
 }
@@ -2031,15 +2455,13 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
       const int zIn = 2 * bytes <= lim;
       schur_qr_kernel<<<batch, 256, (1 + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w, info, 1, zIn);
     } else {
-      const size_t stat = 32 * 1024;
+      const size_t stat = 64 * 1024;
       const size_t lim2 = lim + 20 * 1024 - stat;
       const int hIn = bytes <= lim2, zIn = 2 * bytes <= lim2;
-      if (nbul >= 8)
-        schur_qr_mb_kernel<8, 16, 1, 16><<<batch, 256, (hIn + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w,
-                                                                              info, hIn, zIn);
-      else
-        schur_qr_mb_kernel<4, MBQ_SEG, MBQ_OCC, MBQ_GRP><<<batch, 256, (hIn + zIn) * bytes, st>>>(
-            n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn);
+      // ORF_QR_AED: 0 off, 1 with reordering, 2 from the bottom (default), 3 whole blocks only
+      const int mode = g_qr_aed == 1 ? 0 : (g_qr_aed == 2 ? 1 : (g_qr_aed == 3 ? 2 : 3));
+      schur_qr_aed_kernel<MBQ_NB, MBQ_SEG, MBQ_OCC, MBQ_GRP, MBQ_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
+          n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn, mode);
     }
   }
   te("eig_schur_qr");

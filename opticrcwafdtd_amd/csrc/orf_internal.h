@@ -39,3 +39,4 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
 // Schur QR dispatch knobs: bulges per multishift pass and the size below which the
 // per-rotation LDS kernel is used.
 void orf_set_qr_params(int bulges, int small_n);
+void orf_set_qr_aed(int mode);

@@ -1821,6 +1821,7 @@ int orf_init(int device, orf_ctx** out) {
     orf_set_lds_limit(std::max(lim, 32 * 1024));
     if (const char* e = getenv("ORF_QR_BULGES")) orf_set_qr_params(atoi(e), -1);
     if (const char* e = getenv("ORF_QR_SMALL_N")) orf_set_qr_params(0, atoi(e));
+    if (const char* e = getenv("ORF_QR_AED")) orf_set_qr_aed(atoi(e));
     *out = c;
   })
 }
