@@ -1151,14 +1151,20 @@ __global__ __launch_bounds__(256) void hessenberg_blocked_kernel(int n, cplx* __
       }
       __syncthreads();
       for (int r = k0 + 1 + t; r < n; r += nt) {
-        cplx y0 = cmk(0.0, 0.0), y1 = cmk(0.0, 0.0);
+        // the panel's HBM stream: sixteen columns per step keep sixteen loads in flight
+        cplx y8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) y8[u] = cmk(0.0, 0.0);
         int q = c + 1;
-        for (; q + 1 < n; q += 2) {
-          y0 = cadd(y0, cmul(AA(r, q), bv[q]));
-          y1 = cadd(y1, cmul(AA(r, q + 1), bv[q + 1]));
+        for (; q + 15 < n; q += 16) {
+          cplx a8[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u) a8[u] = AA(r, q + u);
+#pragma unroll
+          for (int u = 0; u < 16; ++u) y8[u & 7] = cadd(y8[u & 7], cmul(a8[u], bv[q + u]));
         }
-        if (q < n) y0 = cadd(y0, cmul(AA(r, q), bv[q]));
-        cplx y = cadd(y0, y1);
+        for (; q < n; ++q) y8[0] = cadd(y8[0], cmul(AA(r, q), bv[q]));
+        cplx y = cadd(cadd(cadd(y8[0], y8[1]), cadd(y8[2], y8[3])), cadd(cadd(y8[4], y8[5]), cadd(y8[6], y8[7])));
         for (int j = 0; j < i; ++j) y = csub(y, cmul(YY(r, j), sw[j]));
         YY(r, i) = cmul(tk, y);
       }
@@ -1180,14 +1186,19 @@ __global__ __launch_bounds__(256) void hessenberg_blocked_kernel(int n, cplx* __
     // (e) rows 0..k0 of Y: P = A(0:k0, k0+1:) V in Y's columns NB.., then Y = P T
     for (int e = t; e < (k0 + 1) * ib; e += nt) {
       const int r = e % (k0 + 1), j = e / (k0 + 1);
-      cplx p0 = cmk(0.0, 0.0), p1 = cmk(0.0, 0.0);
+      cplx p4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) p4[u] = cmk(0.0, 0.0);
       int q = k0 + j + 1;
-      for (; q + 1 < n; q += 2) {
-        p0 = cadd(p0, cmul(AA(r, q), VS(q, j)));
-        p1 = cadd(p1, cmul(AA(r, q + 1), VS(q + 1, j)));
+      for (; q + 3 < n; q += 4) {
+        cplx a4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a4[u] = AA(r, q + u);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) p4[u] = cadd(p4[u], cmul(a4[u], VS(q + u, j)));
       }
-      if (q < n) p0 = cadd(p0, cmul(AA(r, q), VS(q, j)));
-      YY(r, NB + j) = cadd(p0, p1);
+      for (; q < n; ++q) p4[0] = cadd(p4[0], cmul(AA(r, q), VS(q, j)));
+      YY(r, NB + j) = cadd(cadd(p4[0], p4[1]), cadd(p4[2], p4[3]));
     }
     __syncthreads();
     for (int e = t; e < (k0 + 1) * ib; e += nt) {
@@ -1203,6 +1214,7 @@ __global__ __launch_bounds__(256) void hessenberg_blocked_kernel(int n, cplx* __
         cplx yr[NB];
 #pragma unroll
         for (int j = 0; j < NB; ++j) yr[j] = j < ib ? YY(r, j) : cmk(0.0, 0.0);
+#pragma unroll 4
         for (int q = q0; q < n; ++q) {
           cplx s = cmk(0.0, 0.0);
 #pragma unroll
