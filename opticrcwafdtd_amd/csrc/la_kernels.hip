@@ -63,30 +63,37 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(
       accR[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
       accI[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
     }
-  for (int k0 = 0; k0 < K; k0 += GT_K) {
-    // ---- stage op(A)[m0:m0+64, k0:k0+16]
+  // this thread's staging slots: op(A)[ra_r[q], ra_k[q]] and op(B)[rb_k[q], rb_c[q]]
+  int ra_r[4], ra_k[4], rb_k[4], rb_c[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (OPA == 0) { ra_r[q] = t & 63; ra_k[q] = (t >> 6) + 4 * q; }
+    else { ra_k[q] = t & 15; ra_r[q] = (t >> 4) + 16 * q; }
+    if (OPB == 0) { rb_k[q] = t & 15; rb_c[q] = (t >> 4) + 16 * q; }
+    else { rb_c[q] = t & 63; rb_k[q] = (t >> 6) + 4 * q; }
+  }
+  // the next stage is fetched into registers while the current one feeds the MFMAs
+  cplx pa[4], pb[4];
+  auto fetch = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      int r, k;
-      if (OPA == 0) { r = t & 63; k = (t >> 6) + 4 * q; }
-      else { k = t & 15; r = (t >> 4) + 16 * q; }
-      int gr = m0 + r, gk = k0 + k;
-      cplx v = (gr < M && gk < K) ? load_op<OPA>(A, lda, gr, gk) : cmk(0.0, 0.0);
-      Asr[k][r] = v.x;
-      Asi[k][r] = v.y;
+      const int gr = m0 + ra_r[q], gk = k0 + ra_k[q];
+      pa[q] = (gr < M && gk < K) ? load_op<OPA>(A, lda, gr, gk) : cmk(0.0, 0.0);
+      const int gk2 = k0 + rb_k[q], gc = n0 + rb_c[q];
+      pb[q] = (gk2 < K && gc < N) ? load_op<OPB>(B, ldb, gk2, gc) : cmk(0.0, 0.0);
     }
-    // ---- stage op(B)[k0:k0+16, n0:n0+64]
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < K; k0 += GT_K) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      int k, c;
-      if (OPB == 0) { k = t & 15; c = (t >> 4) + 16 * q; }
-      else { c = t & 63; k = (t >> 6) + 4 * q; }
-      int gk = k0 + k, gc = n0 + c;
-      cplx v = (gk < K && gc < N) ? load_op<OPB>(B, ldb, gk, gc) : cmk(0.0, 0.0);
-      Bsr[k][c] = v.x;
-      Bsi[k][c] = v.y;
+      Asr[ra_k[q]][ra_r[q]] = pa[q].x;
+      Asi[ra_k[q]][ra_r[q]] = pa[q].y;
+      Bsr[rb_k[q]][rb_c[q]] = pb[q].x;
+      Bsi[rb_k[q]][rb_c[q]] = pb[q].y;
     }
     __syncthreads();
+    if (k0 + GT_K < K) fetch(k0 + GT_K);
 #pragma unroll
     for (int kk = 0; kk < GT_K; kk += 4) {
       const int kr = kk + (lane >> 4), rr = lane & 15;
