@@ -28,6 +28,12 @@
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// LDS visibility and execution barrier within one wavefront.
+ORF_DEV void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // ============================================================== ZGEMM (MFMA)
 namespace {
 constexpr int GT_M = 64, GT_N = 64, GT_K = 16, GT_PAD = 2;
@@ -632,6 +638,75 @@ __global__ __launch_bounds__(256) void zgetrf_blocked_kernel(int n, cplx* __rest
   if (t == 0 && info) info[page] = sing;
 }
 
+// Inverses of many small matrices (n <= SMALLINV_MAX): one wave per matrix,
+// Gauss-Jordan with partial pivoting (largest |re|+|im|, first row on ties)
+// on the augmented [A | I] held in LDS, four matrices per workgroup.  For the
+// inverse rule's Toeplitz blocks (one per page and grid line) this replaces a
+// 256-thread workgroup per matrix and a separate solve.
+constexpr int SMALLINV_MAX = 22;
+__global__ __launch_bounds__(256) void small_inverse_kernel(int n, const cplx* __restrict__ A, long long sA,
+                                                            cplx* __restrict__ X, long long sX, long long count,
+                                                            int* __restrict__ sing) {
+  extern __shared__ cplx gjs[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long mat = (long long)blockIdx.x * 4 + wv;
+  const int n2 = 2 * n;
+  cplx* M = gjs + (size_t)wv * n * n2;  // row-major n x 2n
+  if (mat >= count) return;             // whole waves only; no block barrier below
+  A += (size_t)mat * sA;
+  X += (size_t)mat * sX;
+  for (int e = lane; e < n * n2; e += 64) {
+    const int r = e / n2, c = e % n2;
+    M[e] = c < n ? A[r + (size_t)c * n] : cmk(c - n == r ? 1.0 : 0.0, 0.0);
+  }
+  wave_sync_lds();
+  bool singular = false;
+  for (int k = 0; k < n; ++k) {
+    // pivot: lane r - k looks at row r
+    const int r = k + lane;
+    double v = (r < n) ? cabs1(M[r * n2 + k]) : -1.0;
+    int idx = r;
+    for (int o = 32; o > 0; o >>= 1) {
+      const double v2 = __shfl_xor(v, o, 64);
+      const int i2 = __shfl_xor(idx, o, 64);
+      if (v2 > v || (v2 == v && i2 < idx)) { v = v2; idx = i2; }
+    }
+    if (v == 0.0) singular = true;
+    if (idx != k)
+      for (int c = lane; c < n2; c += 64) {
+        const cplx x = M[k * n2 + c];
+        M[k * n2 + c] = M[idx * n2 + c];
+        M[idx * n2 + c] = x;
+      }
+    wave_sync_lds();
+    const cplx rp = singular ? cmk(0.0, 0.0) : crecip(M[k * n2 + k]);
+    wave_sync_lds();  // every lane has the pivot before row k changes
+    for (int c = lane; c < n2; c += 64) M[k * n2 + c] = cmul(M[k * n2 + c], rp);
+    wave_sync_lds();
+    // eliminate column k from every other row (factors read before any update)
+    cplx f[SMALLINV_MAX];
+#pragma unroll
+    for (int q = 0; q < SMALLINV_MAX; ++q) f[q] = (q < n && q != k) ? M[q * n2 + k] : cmk(0.0, 0.0);
+    wave_sync_lds();
+    for (int e = lane; e < n * n2; e += 64) {
+      const int rr = e / n2, c = e % n2;
+      if (rr != k) {
+        cplx fr = cmk(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < SMALLINV_MAX; ++q)
+          if (q == rr) fr = f[q];
+        M[e] = csub(M[e], cmul(fr, M[k * n2 + c]));
+      }
+    }
+    wave_sync_lds();
+  }
+  for (int e = lane; e < n * n; e += 64) {
+    const int r = e % n, c = e / n;
+    X[r + (size_t)c * n] = M[r * n2 + n + c];
+  }
+  if (sing && lane == 0) sing[mat] = singular ? 1 : 0;
+}
+
 // Inverses of the 16x16 diagonal blocks of the LU factors, once per page:
 // Linv (unit lower) and Uinv (upper), padded with the identity beyond n.
 // Layout per page: [Linv blk 0..nblk-1][Uinv blk 0..nblk-1], 256 complex each.
@@ -904,6 +979,20 @@ void orf_zgetrf_only(hipStream_t st, int n, cplx* A, int lda, long long sA, int*
     zgetrf_blocked_kernel<8><<<batch, 256, 8 * per, st>>>(n, A, lda, sA, ipiv, info);
   else
     zgetrf_blocked_kernel<4><<<batch, 256, 4 * per, st>>>(n, A, lda, sA, ipiv, info);
+}
+
+// X_i = A_i^-1 for count small n x n matrices (column-major, strides sA, sX);
+// false when n is too large for the one-wave kernel (nothing launched).
+bool orf_small_inverse(hipStream_t st, int n, const cplx* A, long long sA, cplx* X, long long sX,
+                       long long count) {
+  if (n <= 0 || n > SMALLINV_MAX || count <= 0) return false;
+  const size_t lds = (size_t)4 * n * 2 * n * sizeof(cplx);
+  for (long long b0 = 0; b0 < count; b0 += 4LL * 65535 * 4) {
+    const long long cnt = std::min<long long>(count - b0, 4LL * 65535 * 4);
+    small_inverse_kernel<<<(unsigned)((cnt + 3) / 4), 256, lds, st>>>(n, A + (size_t)b0 * sA, sA,
+                                                                      X + (size_t)b0 * sX, sX, cnt, nullptr);
+  }
+  return true;
 }
 
 static int rhs_block(int n) {
@@ -1545,11 +1634,6 @@ __global__ __launch_bounds__(256) void schur_qr_kernel(int n, cplx* __restrict__
 #undef HH
 }
 
-// LDS visibility and execution barrier within one wavefront.
-ORF_DEV void wave_sync_lds() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
 
 // Multishift configuration: bulges per pass, bulge positions per segment, the
 // waves per SIMD the register budget is cut for (two QR workgroups per CU hide

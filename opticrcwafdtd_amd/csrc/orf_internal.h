@@ -38,5 +38,7 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
                      int batch, const OrfTimeHook& hook);
 // Schur QR dispatch knobs: bulges per multishift pass and the size below which the
 // per-rotation LDS kernel is used.
+bool orf_small_inverse(hipStream_t st, int n, const cplx* A, long long sA, cplx* X, long long sX,
+                       long long count);
 void orf_set_qr_params(int bulges, int small_n);
 void orf_set_qr_aed(int mode);

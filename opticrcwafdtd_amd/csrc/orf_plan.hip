@@ -1202,9 +1202,11 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
       // invert every Toeplitz block: LU then solve against the identity (in Yw scratch)
       const int nb = Cp * P.Nsy;
       cplx* inv = w.Inv;
-      eye_stack_kernel<<<g1((long long)Wx * Wx * nb), 256, 0, st>>>(Wx, (long long)nb, inv);
-      orf_zgetrf(st, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, nb, nullptr);
-      orf_zgetrs(st, 'L', Wx, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, inv, Wx, (long long)Wx * Wx, nb);
+      if (!orf_small_inverse(st, Wx, w.Toe, (long long)Wx * Wx, inv, (long long)Wx * Wx, nb)) {
+        eye_stack_kernel<<<g1((long long)Wx * Wx * nb), 256, 0, st>>>(Wx, (long long)nb, inv);
+        orf_zgetrf(st, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, nb, nullptr);
+        orf_zgetrs(st, 'L', Wx, Wx, w.Toe, Wx, (long long)Wx * Wx, w.ipivT, nullptr, inv, Wx, (long long)Wx * Wx, nb);
+      }
       // G = Fy * [A_j]^T : (Hy x Nsy) * (Nsy x Wx^2)
       bulk_gemm(P, 'N', 'T', Hy, Wx * Wx, P.Nsy, ONE, P.dFy.as<cplx>(), Hy, 0, inv, Wx * Wx,
                 (long long)Wx * Wx * P.Nsy, ZERO, w.GH, Hy, (long long)Hy * Wx * Wx, Cp);
@@ -1213,9 +1215,11 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
       li_toeplitz_kernel<<<g1(pery, Cp), 256, 0, st>>>(1, P.Ny, P.Nsx, Hy, pages, P.dChiY.as<cplx>(),
                                                         (long long)P.nslotMax * Hy * P.Nsx, si, P.nslotMax, w.Toe);
       const int nb2 = Cp * P.Nsx;
-      eye_stack_kernel<<<g1((long long)Wy * Wy * nb2), 256, 0, st>>>(Wy, (long long)nb2, inv);
-      orf_zgetrf(st, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, nb2, nullptr);
-      orf_zgetrs(st, 'L', Wy, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, inv, Wy, (long long)Wy * Wy, nb2);
+      if (!orf_small_inverse(st, Wy, w.Toe, (long long)Wy * Wy, inv, (long long)Wy * Wy, nb2)) {
+        eye_stack_kernel<<<g1((long long)Wy * Wy * nb2), 256, 0, st>>>(Wy, (long long)nb2, inv);
+        orf_zgetrf(st, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, nb2, nullptr);
+        orf_zgetrs(st, 'L', Wy, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, inv, Wy, (long long)Wy * Wy, nb2);
+      }
       bulk_gemm(P, 'N', 'T', Hx, Wy * Wy, P.Nsx, ONE, P.dFx.as<cplx>(), Hx, 0, inv, Wy * Wy,
                 (long long)Wy * Wy * P.Nsx, ZERO, w.HH, Hx, (long long)Hx * Wy * Wy, Cp);
       li_gather_kernel<<<g1(M2, Cp), 256, 0, st>>>(P.Nx, P.Ny, M, w.GH, w.HH, w.Exx, w.Eyy, M2);
