@@ -2466,6 +2466,86 @@ __global__ __launch_bounds__(256) void trevc_kernel(int n, const cplx* __restric
   }
 }
 
+// Blocked form of trevc_kernel: TV_EB eigenvectors of one page per workgroup.
+// Thread (e, g) = (t / 16, t % 16) holds rows g, g + 16, ... of eigenvector
+// k0 + e in registers.  Back substitution runs column by column from the
+// block's last one down to 0; each T column is staged in LDS once (fetched a
+// step ahead, three buffers) and serves every eigenvector of the block, so T
+// is read TV_EB times less often, and one barrier per column suffices.  Per
+// eigenvector the arithmetic and its order are those of trevc_kernel.
+constexpr int TV_EB = 16;
+template <int RPT>
+__global__ __launch_bounds__(256) void trevc_blocked_kernel(int n, const cplx* __restrict__ T, int ldt,
+                                                            long long sT, cplx* __restrict__ Y, int ldy,
+                                                            long long sY) {
+  extern __shared__ cplx tvs[];
+  cplx* tcol = tvs;                    // 3 x n: staged columns
+  cplx* xjb = tvs + 3 * (size_t)n;     // 2 x TV_EB: x_j of every eigenvector of the block
+  const int page = blockIdx.y, k0 = blockIdx.x * TV_EB, t = threadIdx.x;
+  const int e = t >> 4, g = t & 15;
+  const int k = k0 + e;
+  const bool act = k < n;
+  T += (size_t)page * sT;
+  Y += (size_t)page * sY;
+  const double smlnum = SAFMIN * ((double)n / ULP);
+  const cplx tkk = act ? T[k + (size_t)k * ldt] : cmk(0.0, 0.0);
+  const double smin = fmax(ULP * cabs1(tkk), smlnum);
+  cplx x[RPT];
+#pragma unroll
+  for (int sl = 0; sl < RPT; ++sl) {
+    const int r = g + 16 * sl;
+    x[sl] = (act && r < k) ? cneg(T[r + (size_t)k * ldt]) : cmk(act && r == k ? 1.0 : 0.0, 0.0);
+  }
+  const int kmax = min(k0 + TV_EB, n) - 1;
+  const int jtop = kmax - 1;
+  if (jtop >= 0)
+    for (int r = t; r <= jtop; r += 256) tcol[(size_t)(jtop % 3) * n + r] = T[r + (size_t)jtop * ldt];
+  __syncthreads();
+  for (int j = jtop; j >= 0; --j) {
+    const cplx* tc = tcol + (size_t)(j % 3) * n;
+    if (g == (j & 15)) {  // owner of row j
+      const int sj = j >> 4;
+      cplx xv = cmk(0.0, 0.0);
+#pragma unroll
+      for (int sl = 0; sl < RPT; ++sl)
+        if (sl == sj) xv = x[sl];
+      if (act && k > j) {
+        cplx d = csub(tc[j], tkk);
+        if (cabs1(d) < smin) d = cmk(smin, 0.0);
+        xv = cdiv(xv, d);
+#pragma unroll
+        for (int sl = 0; sl < RPT; ++sl)
+          if (sl == sj) x[sl] = xv;
+      } else {
+        xv = cmk(0.0, 0.0);
+      }
+      xjb[(j & 1) * TV_EB + e] = xv;
+    }
+    if (j > 0)  // next column, rows 0..j-1 (its diagonal included)
+      for (int r = t; r < j; r += 256) tcol[(size_t)((j - 1) % 3) * n + r] = T[r + (size_t)(j - 1) * ldt];
+    __syncthreads();
+    const cplx xj = xjb[(j & 1) * TV_EB + e];
+#pragma unroll
+    for (int sl = 0; sl < RPT; ++sl) {
+      const int r = g + 16 * sl;
+      if (r < j) x[sl] = csub(x[sl], cmul(tc[r], xj));
+    }
+  }
+  double mx = 0.0;
+#pragma unroll
+  for (int sl = 0; sl < RPT; ++sl)
+    if (act && g + 16 * sl <= k) mx = fmax(mx, cabs1(x[sl]));
+  for (int o = 8; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  const double sc = mx > 0.0 ? 1.0 / mx : 1.0;
+  if (act) {
+#pragma unroll
+    for (int sl = 0; sl < RPT; ++sl) {
+      const int r = g + 16 * sl;
+      if (r < n) Y[r + (size_t)k * ldy] = (r <= k) ? cscale(x[sl], sc) : cmk(0.0, 0.0);
+    }
+  }
+}
+
 // Scale every column of W to unit 2-norm.
 __global__ __launch_bounds__(256) void colnorm_kernel(int n, cplx* __restrict__ W, int ld,
                                                       long long sW) {
@@ -2570,7 +2650,16 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
   te("eig_schur_qr");
   dim3 g2((n + 3) / 4, batch);
   tb("eig_vectors");
-  trevc_kernel<<<g2, 256, 0, st>>>(n, A, lda, sA, Ywork, n, sZ);
+  {
+    const size_t lds = (3 * (size_t)n + 2 * TV_EB) * sizeof(cplx);
+    const dim3 gb((n + TV_EB - 1) / TV_EB, batch);
+    if (n <= 256)
+      trevc_blocked_kernel<16><<<gb, 256, lds, st>>>(n, A, lda, sA, Ywork, n, sZ);
+    else if (n <= 512)
+      trevc_blocked_kernel<32><<<gb, 256, lds, st>>>(n, A, lda, sA, Ywork, n, sZ);
+    else
+      trevc_kernel<<<g2, 256, 0, st>>>(n, A, lda, sA, Ywork, n, sZ);
+  }
   orf_zgemm(st, 'N', 'N', n, n, n, cmk(1.0, 0.0), Zwork, n, sZ, Ywork, n, sZ, cmk(0.0, 0.0), V,
             ldv, sV, batch);
   colnorm_kernel<<<g2, 256, 0, st>>>(n, V, ldv, sV);
