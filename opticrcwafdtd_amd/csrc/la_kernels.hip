@@ -2054,8 +2054,8 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
   __shared__ cplx win[WINSZ];       // chase window, or the AED window T
   __shared__ cplx Uw[LDW * NWX];
   __shared__ cplx vv[LDW];
-  __shared__ double rc[MB_NB * MB_SEG];
-  __shared__ cplx rs[MB_NB * MB_SEG];
+  __shared__ double rc[2 * MB_NB * MB_SEG];  // rotations of two segments (pipelined)
+  __shared__ cplx rs[2 * MB_NB * MB_SEG];
   __shared__ cplx sm8[64];
   __shared__ cplx shifts[MB_NB];
   __shared__ cplx rr8[MB_NB];
@@ -2299,107 +2299,133 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
       continue;
     }
     // ---- one pass: segments of MB_SEG positions of bulge 0
-    for (int p0 = l; p0 <= i - 1 + 3 * (nbul - 1); p0 += MB_SEG) {
-      const int a0 = p0 - 3 * (MB_NB - 1) - 1;  // window origin (may be < 0)
-      __syncthreads();
-      for (int q = t; q < MB_SW * MB_SW; q += nt) {
-        const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
-        win[q] = (r >= 0 && r < n && c >= 0 && c < n) ? HH(r, c) : cmk(0.0, 0.0);
-      }
-      __syncthreads();
-      if (wv == 0) {
-        // lockstep: at step j bulge b sits at position p0 - 3b + j; the rotations
-        // of one step act on disjoint row and column pairs
-        for (int j = 0; j < MB_SEG; ++j) {
-          // A: lane b computes bulge b's rotation
-          if (lane < MB_NB) {
-            const int b = lane, k = p0 - 3 * b + j;
-            const bool valid = b < nbul && k >= l && k <= i - 1;
-            double c = 1.0;
-            cplx sn = cmk(0.0, 0.0), r = cmk(0.0, 0.0);
-            if (valid) {
-              const int kk = k - a0;
-              cplx x, y;
-              if (k == l) { x = csub(WN(kk, kk), shifts[b]); y = WN(kk + 1, kk); }
-              else { x = WN(kk, kk - 1); y = WN(kk + 1, kk - 1); }
-              givens(x, y, c, sn, r);
+    // Segments are pipelined: while wave 0 loads and chases segment `it`, the
+    // other waves apply segment it-1's rotations to everything outside the next
+    // window (phase 0); then all threads write window `it` back and update the
+    // MB_SEG columns the next window reads (phase 1).  Rotations are double
+    // buffered.  Every region a phase touches is disjoint from what the other
+    // side of that phase touches, and successive updates of one element are
+    // separated by a barrier.
+    const int nseg = (i - 1 + 3 * (nbul - 1) - l) / MB_SEG + 1;
+    for (int it = 0; it <= nseg; ++it) {
+      for (int ph = 0; ph < 2; ++ph) {
+        const int cs = ph == 0 ? it : -1;           // segment chased in this phase
+        const int ss = ph == 0 ? it - 1 : it;       // segment whose strips run in this phase
+        const int a0 = l + it * MB_SEG - 3 * (MB_NB - 1) - 1;  // window origin of segment it
+        if (ph == 0 && cs < nseg && wv == 0) {
+          double* rcb = rc + (cs & 1) * MB_NB * MB_SEG;
+          cplx* rsb = rs + (cs & 1) * MB_NB * MB_SEG;
+          const int p0 = l + cs * MB_SEG;
+          for (int q = lane; q < MB_SW * MB_SW; q += 64) {
+            const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
+            win[q] = (r >= 0 && r < n && c >= 0 && c < n) ? HH(r, c) : cmk(0.0, 0.0);
+          }
+          wave_sync_lds();
+          // lockstep: at step j bulge b sits at position p0 - 3b + j; the rotations
+          // of one step act on disjoint row and column pairs
+          for (int j = 0; j < MB_SEG; ++j) {
+            // A: lane b computes bulge b's rotation
+            if (lane < MB_NB) {
+              const int b = lane, k = p0 - 3 * b + j;
+              const bool valid = b < nbul && k >= l && k <= i - 1;
+              double c = 1.0;
+              cplx sn = cmk(0.0, 0.0), r = cmk(0.0, 0.0);
+              if (valid) {
+                const int kk = k - a0;
+                cplx x, y;
+                if (k == l) { x = csub(WN(kk, kk), shifts[b]); y = WN(kk + 1, kk); }
+                else { x = WN(kk, kk - 1); y = WN(kk + 1, kk - 1); }
+                givens(x, y, c, sn, r);
+              }
+              rcb[b * MB_SEG + j] = c;
+              rsb[b * MB_SEG + j] = sn;
+              rr8[b] = r;
             }
-            rc[b * MB_SEG + j] = c;
-            rs[b * MB_SEG + j] = sn;
-            rr8[b] = r;
-          }
-          wave_sync_lds();
-          // B: row ops of every bulge on window columns kk..MB_SW-1
-          for (int e = lane; e < MB_NB * MB_SW; e += 64) {
-            const int b = e / MB_SW, jc = e % MB_SW;
-            const int k = p0 - 3 * b + j;
-            if (!(b < nbul && k >= l && k <= i - 1)) continue;
-            const int kk = k - a0;
-            if (jc < kk || a0 + jc >= n) continue;
-            const double c = rc[b * MB_SEG + j];
-            const cplx sn = rs[b * MB_SEG + j];
-            const cplx h1 = WN(kk, jc), h2 = WN(kk + 1, jc);
-            WN(kk, jc) = cadd(cscale(h1, c), cmul(sn, h2));
-            WN(kk + 1, jc) = csub(cscale(h2, c), cmul(cconj(sn), h1));
-          }
-          if (lane < MB_NB) {
-            const int b = lane, k = p0 - 3 * b + j;
-            if (b < nbul && k > l && k <= i - 1) {
+            wave_sync_lds();
+            // B: row ops of every bulge on window columns kk..MB_SW-1
+            for (int e = lane; e < MB_NB * MB_SW; e += 64) {
+              const int b = e / MB_SW, jc = e % MB_SW;
+              const int k = p0 - 3 * b + j;
+              if (!(b < nbul && k >= l && k <= i - 1)) continue;
               const int kk = k - a0;
-              WN(kk, kk - 1) = rr8[b];
-              WN(kk + 1, kk - 1) = cmk(0.0, 0.0);
+              if (jc < kk || a0 + jc >= n) continue;
+              const double c = rcb[b * MB_SEG + j];
+              const cplx sn = rsb[b * MB_SEG + j];
+              const cplx h1 = WN(kk, jc), h2 = WN(kk + 1, jc);
+              WN(kk, jc) = cadd(cscale(h1, c), cmul(sn, h2));
+              WN(kk + 1, jc) = csub(cscale(h2, c), cmul(cconj(sn), h1));
+            }
+            if (lane < MB_NB) {
+              const int b = lane, k = p0 - 3 * b + j;
+              if (b < nbul && k > l && k <= i - 1) {
+                const int kk = k - a0;
+                WN(kk, kk - 1) = rr8[b];
+                WN(kk + 1, kk - 1) = cmk(0.0, 0.0);
+              }
+            }
+            wave_sync_lds();
+            // C: column ops of every bulge on window rows 0..min(kk+2, i-a0)
+            for (int e = lane; e < MB_NB * MB_SW; e += 64) {
+              const int b = e / MB_SW, q = e % MB_SW;
+              const int k = p0 - 3 * b + j;
+              if (!(b < nbul && k >= l && k <= i - 1)) continue;
+              const int kk = k - a0;
+              if (q > min(kk + 2, i - a0) || a0 + q < 0) continue;
+              const double c = rcb[b * MB_SEG + j];
+              const cplx sn = rsb[b * MB_SEG + j];
+              const cplx h1 = WN(q, kk), h2 = WN(q, kk + 1);
+              WN(q, kk) = cadd(cscale(h1, c), cmul(cconj(sn), h2));
+              WN(q, kk + 1) = csub(cscale(h2, c), cmul(sn, h1));
+            }
+            wave_sync_lds();
+          }
+        }
+        if (ph == 1 && it < nseg) {
+          for (int q = t; q < MB_SW * MB_SW; q += nt) {
+            const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
+            if (r >= 0 && r < n && c >= 0 && c < n) HH(r, c) = win[q];
+          }
+        }
+        if (ss >= 0 && ss < nseg) {
+          // strips of segment ss: columns right of its window, rows above it, rows of Z;
+          // phase 1 takes the first MB_SEG columns right (the next window's), phase 0 the rest
+          const int sa0 = l + ss * MB_SEG - 3 * (MB_NB - 1) - 1;
+          const int e0 = sa0 + MB_SW;
+          const int ncR = e0 < n ? n - e0 : 0;
+          const int nrA = sa0 > 0 ? sa0 : 0;
+          const int nfirst = min(MB_SEG, ncR);
+          const int tlo = ph == 1 ? 0 : nfirst, thi = ph == 1 ? nfirst : ncR + nrA + n;
+          const bool shared = ph == 0 && it < nseg;  // wave 0 is chasing
+          const int t0 = shared ? t - 64 : t, tstep = shared ? nt - 64 : nt;
+          const double* rcs = rc + (ss & 1) * MB_NB * MB_SEG;
+          const cplx* rss = rs + (ss & 1) * MB_NB * MB_SEG;
+          if (t0 >= 0) {
+            for (int task = tlo + t0; task < thi; task += tstep) {
+              cplx* base;
+              size_t stride;
+              bool rowop;
+              if (task < ncR) {
+                base = H + (size_t)(e0 + task) * ld + sa0;  // column e0+task, rows sa0..
+                stride = 1;
+                rowop = true;
+              } else if (task < ncR + nrA) {
+                base = H + (task - ncR) + (size_t)sa0 * ld;  // row task-ncR, columns sa0..
+                stride = ld;
+                rowop = false;
+              } else {
+                base = Z + (task - ncR - nrA) + (ptrdiff_t)sa0 * n;
+                stride = n;
+                rowop = false;
+              }
+              // one instance only: with a checked and an unchecked copy behind a branch
+              // the compiler hoists every rotation's LDS loads above it and spills
+              strip_rotations<MB_NB, MB_SEG, GRP, LA, true>(base, (ptrdiff_t)stride, sa0, n, rowop, rcs, rss);
             }
           }
-          wave_sync_lds();
-          // C: column ops of every bulge on window rows 0..min(kk+2, i-a0)
-          for (int e = lane; e < MB_NB * MB_SW; e += 64) {
-            const int b = e / MB_SW, q = e % MB_SW;
-            const int k = p0 - 3 * b + j;
-            if (!(b < nbul && k >= l && k <= i - 1)) continue;
-            const int kk = k - a0;
-            if (q > min(kk + 2, i - a0) || a0 + q < 0) continue;
-            const double c = rc[b * MB_SEG + j];
-            const cplx sn = rs[b * MB_SEG + j];
-            const cplx h1 = WN(q, kk), h2 = WN(q, kk + 1);
-            WN(q, kk) = cadd(cscale(h1, c), cmul(cconj(sn), h2));
-            WN(q, kk + 1) = csub(cscale(h2, c), cmul(sn, h1));
-          }
-          wave_sync_lds();
         }
-      }
-      __syncthreads();
-      for (int q = t; q < MB_SW * MB_SW; q += nt) {
-        const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
-        if (r >= 0 && r < n && c >= 0 && c < n) HH(r, c) = win[q];
-      }
-      // deferred: columns right of the window, rows above it, rows of Z
-      const int e0 = a0 + MB_SW;                    // first column right of the window
-      const int ncR = e0 < n ? n - e0 : 0;
-      const int nrA = a0 > 0 ? a0 : 0;
-      const int tasks = ncR + nrA + n;
-      for (int task = t; task < tasks; task += nt) {
-        cplx* base;
-        size_t stride;
-        bool rowop;
-        if (task < ncR) {
-          base = H + (size_t)(e0 + task) * ld + a0;  // column e0+task, rows a0..
-          stride = 1;
-          rowop = true;
-        } else if (task < ncR + nrA) {
-          base = H + (task - ncR) + (size_t)a0 * ld;  // row task-ncR, columns a0..
-          stride = ld;
-          rowop = false;
-        } else {
-          base = Z + (task - ncR - nrA) + (ptrdiff_t)a0 * n;
-          stride = n;
-          rowop = false;
-        }
-        // one instance only: with a checked and an unchecked copy behind a branch the
-        // compiler hoists every rotation's LDS loads above it and spills
-        strip_rotations<MB_NB, MB_SEG, GRP, LA, true>(base, (ptrdiff_t)stride, a0, n, rowop, rc, rs);
+        __syncthreads();
       }
     }
-    __syncthreads();
   }
 
   if (t == 0 && info) info[page] = failed;
