@@ -516,6 +516,17 @@ ORF_DEV void ztile_mfma(FA fa, FB fb, v4d& accR, v4d& accI) {
   }
 }
 
+// The MFMA chain of ztile_mfma<4> on fragments already in registers.
+ORF_DEV void ztile_mfma_frag(const cplx* av, const cplx* bv, v4d& accR, v4d& accI) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    accR = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].x, bv[s].x, accR, 0, 0, 0);
+    accR = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[s].y, bv[s].y, accR, 0, 0, 0);
+    accI = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].x, bv[s].y, accI, 0, 0, 0);
+    accI = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].y, bv[s].x, accI, 0, 0, 0);
+  }
+}
+
 // Right-looking blocked LU with partial pivoting, one workgroup per page.
 // The LB-wide panel and the LB rows of U12 live in LDS; the trailing update
 // A22 -= L21 U12 runs on MFMA tiles; row interchanges reach the other
@@ -807,23 +818,36 @@ __global__ __launch_bounds__(256) void ztrsm_left_kernel(int n, int nrhs, int CB
       const int rlo = pass == 0 ? kb + nb : 0, rhi = pass == 0 ? n : kb;
       const int rows = rhi - rlo;
       if (rows > 0) {
-        const int trn = (rows + 15) / 16;
-        for (int tile = wv; tile < trn * tcn; tile += nw) {
-          const int r0 = rlo + (tile % trn) * 16, cc0 = (tile / trn) * 16;
-          v4d aR = {0, 0, 0, 0}, aI = {0, 0, 0, 0};
-          ztile_mfma<4>(
-              [&](int r, int k) -> cplx {
-                return (r0 + r < rhi && k < nb) ? LU[(size_t)(r0 + r) + (size_t)(kb + k) * lda] : cmk(0.0, 0.0);
-              },
-              [&](int k, int c) -> cplx {
-                return (k < nb && cc0 + c < nc) ? X[kb + k + (cc0 + c) * n] : cmk(0.0, 0.0);
-              },
-              aR, aI);
-          const int cc = cc0 + (lane & 15);
+        // a wave's tiles four at a time, every LU fragment fetched before the first MFMA
+        const int trn = (rows + 15) / 16, ntile = trn * tcn;
+        const int fr = lane & 15, kq = lane >> 4;
+        for (int tb = wv; tb < ntile; tb += 4 * nw) {
+          cplx af[4][4], bf[4][4];
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int rr = r0 + (lane >> 4) + 4 * v;
-            if (rr < rhi && cc < nc) X[rr + cc * n] = csub(X[rr + cc * n], cmk(aR[v], aI[v]));
+          for (int q = 0; q < 4; ++q) {
+            const int tile = tb + q * nw;
+            const int r0 = rlo + (tile % trn) * 16, cc0 = (tile / trn) * 16;
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+              const int k = 4 * s4 + kq;
+              af[q][s4] = (tile < ntile && r0 + fr < rhi && k < nb) ? LU[(size_t)(r0 + fr) + (size_t)(kb + k) * lda]
+                                                                    : cmk(0.0, 0.0);
+              bf[q][s4] = (tile < ntile && k < nb && cc0 + fr < nc) ? X[kb + k + (cc0 + fr) * n] : cmk(0.0, 0.0);
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int tile = tb + q * nw;
+            if (tile >= ntile) break;
+            const int r0 = rlo + (tile % trn) * 16, cc0 = (tile / trn) * 16;
+            v4d aR = {0, 0, 0, 0}, aI = {0, 0, 0, 0};
+            ztile_mfma_frag(af[q], bf[q], aR, aI);
+            const int cc = cc0 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int rr = r0 + (lane >> 4) + 4 * v;
+              if (rr < rhi && cc < nc) X[rr + cc * n] = csub(X[rr + cc * n], cmk(aR[v], aI[v]));
+            }
           }
         }
       }
@@ -883,23 +907,36 @@ __global__ __launch_bounds__(256) void ztrsm_right_kernel(int n, int nrows, int 
       const int clo = pass == 0 ? kb + nb : 0, chi = pass == 0 ? n : kb;
       const int cols = chi - clo;
       if (cols > 0) {
-        const int tcc = (cols + 15) / 16;
-        for (int tile = wv; tile < trr * tcc; tile += nw) {
-          const int rr0 = (tile % trr) * 16, cc0 = clo + (tile / trr) * 16;
-          v4d aR = {0, 0, 0, 0}, aI = {0, 0, 0, 0};
-          ztile_mfma<4>(
-              [&](int r, int k) -> cplx {
-                return (rr0 + r < nr && k < nb) ? X[rr0 + r + (kb + k) * RB] : cmk(0.0, 0.0);
-              },
-              [&](int k, int c) -> cplx {
-                return (k < nb && cc0 + c < chi) ? LU[(size_t)(kb + k) + (size_t)(cc0 + c) * lda] : cmk(0.0, 0.0);
-              },
-              aR, aI);
-          const int cc = cc0 + (lane & 15);
+        // a wave's tiles four at a time, every LU fragment fetched before the first MFMA
+        const int tcc = (cols + 15) / 16, ntile = trr * tcc;
+        const int fr = lane & 15, kq = lane >> 4;
+        for (int tb = wv; tb < ntile; tb += 4 * nw) {
+          cplx af[4][4], bf[4][4];
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int rr = rr0 + (lane >> 4) + 4 * v;
-            if (rr < nr && cc < chi) X[rr + cc * RB] = csub(X[rr + cc * RB], cmk(aR[v], aI[v]));
+          for (int q = 0; q < 4; ++q) {
+            const int tile = tb + q * nw;
+            const int rr0 = (tile % trr) * 16, cc0 = clo + (tile / trr) * 16;
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+              const int k = 4 * s4 + kq;
+              af[q][s4] = (tile < ntile && rr0 + fr < nr && k < nb) ? X[rr0 + fr + (kb + k) * RB] : cmk(0.0, 0.0);
+              bf[q][s4] = (tile < ntile && k < nb && cc0 + fr < chi) ? LU[(size_t)(kb + k) + (size_t)(cc0 + fr) * lda]
+                                                                     : cmk(0.0, 0.0);
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int tile = tb + q * nw;
+            if (tile >= ntile) break;
+            const int rr0 = (tile % trr) * 16, cc0 = clo + (tile / trr) * 16;
+            v4d aR = {0, 0, 0, 0}, aI = {0, 0, 0, 0};
+            ztile_mfma_frag(af[q], bf[q], aR, aI);
+            const int cc = cc0 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int rr = rr0 + (lane >> 4) + 4 * v;
+              if (rr < nr && cc < chi) X[rr + cc * RB] = csub(X[rr + cc * RB], cmk(aR[v], aI[v]));
+            }
           }
         }
       }
