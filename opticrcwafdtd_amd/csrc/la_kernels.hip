@@ -763,13 +763,13 @@ __global__ __launch_bounds__(256) void diag_inv_kernel(int n, const cplx* __rest
 __global__ __launch_bounds__(256) void ztrsm_left_kernel(int n, int nrhs, int CB, const cplx* __restrict__ LU,
                                                          int lda, long long sA, const int* __restrict__ ipiv,
                                                          const cplx* __restrict__ dinv, long long sD,
-                                                         cplx* __restrict__ Bg, int ldb, long long sB) {
+                                                         cplx* __restrict__ Bg, int ldb, long long sB,
+                                                         int upper_only) {
   extern __shared__ cplx X[];  // n x CB, ld n
   const int page = blockIdx.y, c0 = blockIdx.x * CB, t = threadIdx.x, nt = blockDim.x;
   const int lane = t & 63, wv = t >> 6, nw = nt >> 6;
   const int nblk = (n + 15) / 16;
   LU += (size_t)page * sA;
-  ipiv += (size_t)page * n;
   Bg += (size_t)page * sB;
   const cplx* Linv = dinv + (size_t)page * sD;
   const cplx* Uinv = Linv + (size_t)nblk * 256;
@@ -779,7 +779,8 @@ __global__ __launch_bounds__(256) void ztrsm_left_kernel(int n, int nrhs, int CB
     X[r + c * n] = Bg[(size_t)r + (size_t)(c0 + c) * ldb];
   }
   __syncthreads();
-  if (t < 64) {
+  if (!upper_only && t < 64) {
+    ipiv += (size_t)page * n;
     for (int k = 0; k < n; ++k) {
       const int p = ipiv[k];
       if (p != k)
@@ -792,7 +793,7 @@ __global__ __launch_bounds__(256) void ztrsm_left_kernel(int n, int nrhs, int CB
   }
   __syncthreads();
   const int tcn = (nc + 15) / 16;
-  for (int pass = 0; pass < 2; ++pass) {  // 0: forward with L, 1: backward with U
+  for (int pass = upper_only ? 1 : 0; pass < 2; ++pass) {  // 0: forward with L, 1: backward with U
     for (int bi = 0; bi < nblk; ++bi) {
       const int bk = pass == 0 ? bi : nblk - 1 - bi;
       const int kb = bk * 16, nb = min(16, n - kb);
@@ -1068,10 +1069,57 @@ void orf_zgetrs(hipStream_t st, char side, int n, int nrhs, const cplx* LU, int 
   const long long sD = orf_dinv_stride(n);
   if (side == 'L')
     ztrsm_left_kernel<<<grid, 256, (size_t)n * cb * sizeof(cplx), st>>>(n, nrhs, cb, LU, lda, sA, ipiv,
-                                                                         dinv, sD, B, ldb, sB);
+                                                                         dinv, sD, B, ldb, sB, 0);
   else
     ztrsm_right_kernel<<<grid, 256, (size_t)n * cb * sizeof(cplx), st>>>(n, nrhs, cb, LU, lda, sA, ipiv,
                                                                           dinv, sD, B, ldb, sB);
+}
+
+namespace {
+// X = Z^H per page (n x n).
+__global__ void conj_transpose_pages_kernel(int n, const cplx* __restrict__ Z, long long sZ,
+                                            cplx* __restrict__ X, long long sX) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e % n), c = (int)(e / n);
+  X[(size_t)blockIdx.y * sX + e] = cconj(Z[(size_t)blockIdx.y * sZ + c + (size_t)r * n]);
+}
+
+// X[r, :] *= d[r] per page.
+__global__ void rowscale_real_kernel(int n, const double* __restrict__ d, cplx* __restrict__ X, long long sX) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e % n);
+  X[(size_t)blockIdx.y * sX + e] = cscale(X[(size_t)blockIdx.y * sX + e], d[(size_t)blockIdx.y * n + r]);
+}
+}  // namespace
+
+// W^-1 of eigenvector matrices W = Z Y diag(1 / nrm) from orf_zgeev (Z unitary
+// Schur vectors, Y upper-triangular eigenvectors of the Schur form, nrm the
+// column norms colnorm removed): X = diag(nrm) Y^-1 Z^H, one triangular solve
+// instead of an LU and a full solve.  false (nothing launched) when n is too
+// small for the blocked solve.
+bool orf_eig_inverse(hipStream_t st, int n, const cplx* Z, const cplx* Y, const double* nrm, cplx* X,
+                     cplx* dinv, int batch) {
+  if (n < 32 || batch <= 0) return false;
+  const long long n2 = (long long)n * n;
+  for (int b0 = 0; b0 < batch; b0 += 65535) {
+    const int bb = std::min(65535, batch - b0);
+    const cplx* Zb = Z + (size_t)b0 * n2;
+    const cplx* Yb = Y + (size_t)b0 * n2;
+    cplx* Xb = X + (size_t)b0 * n2;
+    cplx* Db = dinv + (size_t)b0 * orf_dinv_stride(n);
+    const dim3 g((unsigned)((n2 + 255) / 256), bb);
+    conj_transpose_pages_kernel<<<g, 256, 0, st>>>(n, Zb, n2, Xb, n2);
+    launch_dinv(st, n, Yb, n, n2, Db, bb);
+    int cb = rhs_block(n);
+    if (cb > 32) cb = 32;
+    if (cb > 16 && (size_t)n * cb * sizeof(cplx) > (size_t)g_lds_limit / 2) cb = 16;
+    ztrsm_left_kernel<<<dim3((n + cb - 1) / cb, bb), 256, (size_t)n * cb * sizeof(cplx), st>>>(
+        n, n, cb, Yb, n, n2, nullptr, Db, orf_dinv_stride(n), Xb, n, n2, 1);
+    rowscale_real_kernel<<<g, 256, 0, st>>>(n, nrm + (size_t)b0 * n, Xb, n2);
+  }
+  return true;
 }
 
 // ============================================================== eigen-decomposition
@@ -2609,9 +2657,9 @@ __global__ __launch_bounds__(256) void trevc_blocked_kernel(int n, const cplx* _
   }
 }
 
-// Scale every column of W to unit 2-norm.
+// Scale every column of W to unit 2-norm; nrm (optional) receives 1 / scale.
 __global__ __launch_bounds__(256) void colnorm_kernel(int n, cplx* __restrict__ W, int ld,
-                                                      long long sW) {
+                                                      long long sW, double* __restrict__ nrm) {
   const int page = blockIdx.y, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int k = blockIdx.x * (blockDim.x >> 6) + wv;
   if (k >= n) return;
@@ -2621,18 +2669,20 @@ __global__ __launch_bounds__(256) void colnorm_kernel(int n, cplx* __restrict__ 
   s = wave_sum(s);
   const double f = s > 0.0 ? 1.0 / sqrt(s) : 1.0;
   for (int r = lane; r < n; r += 64) col[r] = cscale(col[r], f);
+  if (nrm && lane == 0) nrm[(size_t)page * n + k] = s > 0.0 ? sqrt(s) : 1.0;
 }
+
 }  // namespace
 
 // A is overwritten by the Schur factor T; Zwork and Ywork hold n x n per page.
 void orf_zgeev(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V, int ldv,
                long long sV, cplx* Zwork, cplx* Ywork, cplx* Hwork, int* info, int batch) {
-  orf_zgeev_timed(st, n, A, lda, sA, w, V, ldv, sV, Zwork, Ywork, Hwork, info, batch, OrfTimeHook());
+  orf_zgeev_timed(st, n, A, lda, sA, w, V, ldv, sV, Zwork, Ywork, Hwork, info, batch, OrfTimeHook(), nullptr);
 }
 
 void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V,
                      int ldv, long long sV, cplx* Zwork, cplx* Ywork, cplx* Hwork, int* info,
-                     int batch, const OrfTimeHook& hook) {
+                     int batch, const OrfTimeHook& hook, double* vnorm) {
   if (n <= 0 || batch <= 0) return;
   auto tb = [&](const char* nm) { if (hook) hook(nm, 0); };
   auto te = [&](const char* nm) { if (hook) hook(nm, 1); };
@@ -2725,6 +2775,6 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
   }
   orf_zgemm(st, 'N', 'N', n, n, n, cmk(1.0, 0.0), Zwork, n, sZ, Ywork, n, sZ, cmk(0.0, 0.0), V,
             ldv, sV, batch);
-  colnorm_kernel<<<g2, 256, 0, st>>>(n, V, ldv, sV);
+  colnorm_kernel<<<g2, 256, 0, st>>>(n, V, ldv, sV, vnorm);
   te("eig_vectors");
 }

@@ -35,7 +35,9 @@ void orf_zgeev(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, c
                long long sV, cplx* Zwork, cplx* Ywork, cplx* Hwork, int* info, int batch);
 void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V,
                      int ldv, long long sV, cplx* Zwork, cplx* Ywork, cplx* Hwork, int* info,
-                     int batch, const OrfTimeHook& hook);
+                     int batch, const OrfTimeHook& hook, double* vnorm = nullptr);
+bool orf_eig_inverse(hipStream_t st, int n, const cplx* Z, const cplx* Y, const double* nrm, cplx* X,
+                     cplx* dinv, int batch);
 // Schur QR dispatch knobs: bulges per multishift pass and the size below which the
 // per-rotation LDS kernel is used.
 bool orf_small_inverse(hipStream_t st, int n, const cplx* A, long long sA, cplx* X, long long sX,

@@ -346,6 +346,7 @@ struct Work {
   cplx *X1, *X2, *XB, *R3, *Dm, *RHS2, *LUb;
   cplx *Sa[4], *Sn[4], *Sb[4], *Tm, *M1, *Fm;
   double* kL;
+  double* vnorm;  // column norms the eigensolver removed from W (W = Z Y diag(1 / vnorm))
   cplx* dinv;  // diagonal-block inverses of the most recent LU
   int *ipiv, *info;
   DevSolve* solves;
@@ -457,7 +458,7 @@ static size_t per_solve_bytes(const orf_plan& P) {
   c += n2 * (12 + 3);        // Sa Sn Sb Tm M1 Fm
   c += 3 * (size_t)P.nr + 1; // mu lam Xd eps
   c += (size_t)orf_dinv_stride(P.nr); // diagonal-block inverses
-  return c * sizeof(cplx) + sizeof(double) + sizeof(int) * (P.nr + 2) + sizeof(DevSolve);
+  return c * sizeof(cplx) + sizeof(double) * (1 + (size_t)P.nr) + sizeof(int) * (P.nr + 2) + sizeof(DevSolve);
 }
 
 static Work make_work(orf_plan& P, int Cp) {
@@ -514,6 +515,7 @@ static Work make_work(orf_plan& P, int Cp) {
   w.M1 = (cplx*)get(cs * n2 * Cs);
   w.Fm = (cplx*)get(cs * n2 * Cs);
   w.kL = (double*)get(sizeof(double) * Cs);
+  w.vnorm = (double*)get(sizeof(double) * P.nr * Cs);
   w.dinv = (cplx*)get(cs * std::max((size_t)orf_dinv_stride(P.nr) * Cs, (size_t)orf_dinv_stride(P.M) * Cp));
   w.ipiv = (int*)get(sizeof(int) * (size_t)std::max(P.nr, P.M) * std::max(Cs, Cp));
   w.info = (int*)get(sizeof(int) * (size_t)std::max(Cs, Cp) * 64);
@@ -568,19 +570,33 @@ static void check_info(orf_plan& P, const int* dinfo, int count, const char* wha
 
 // Layer S-matrix (smatrixLayer.m:6-18): W, V, lam per solve page; W0 = I; V0 per
 // solve page in X2 on entry.  Out: S11 -> S11o, S12 -> S12o.
-static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx* lam, cplx* S11o, cplx* S12o) {
+// The first CsEig solve pages hold eigensolver output (W = Z Y diag(1 / vnorm)
+// with Z, Y still in w.Zw, w.Yw): their W^-1 comes from one triangular solve.
+static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx* lam, cplx* S11o, cplx* S12o,
+                          int CsEig) {
   const int n = P.nr;
   const long long n2 = (long long)n * n;
   hipStream_t st = P.st;
-  {
-    STAGE("smat_lu");
-    orf_zgetrf(st, n, Wm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
-    KCHECK();
-  }
-  eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, w.X1, n2);
-  {
+  if (CsEig > 0) {
     STAGE("smat_solve");
-    orf_zgetrs(st, 'L', n, n, Wm, n, n2, w.ipiv, w.dinv, w.X1, n, n2, Cs);  // W \ W0
+    if (!orf_eig_inverse(st, n, w.Zw, w.Yw, w.vnorm, w.X1, w.dinv, CsEig)) CsEig = 0;
+  }
+  if (CsEig < Cs) {  // W \ W0 by LU for the rest
+    const int Cr = Cs - CsEig;
+    cplx* Wr = Wm + (size_t)CsEig * n2;
+    int* ipr = w.ipiv + (size_t)CsEig * n;
+    cplx* dr = w.dinv + (size_t)CsEig * orf_dinv_stride(n);
+    cplx* Xr = w.X1 + (size_t)CsEig * n2;
+    {
+      STAGE("smat_lu");
+      orf_zgetrf(st, n, Wr, n, n2, ipr, nullptr, Cr, dr);
+      KCHECK();
+    }
+    eye_axpby_kernel<<<g1(n2, Cr), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, Xr, n2);
+    {
+      STAGE("smat_solve");
+      orf_zgetrs(st, 'L', n, n, Wr, n, n2, ipr, dr, Xr, n, n2, Cr);
+    }
   }
   {
     STAGE("smat_lu");
@@ -1250,7 +1266,7 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
     orf_zgeev_timed(st, nr, w.A0, nr, n2, w.mu, w.Wm, nr, n2, w.Zw, w.Yw, w.Hv, w.info, Cs,
                     [&](const char* nm, int phase) {
                       if (phase == 0) P.timer.begin(nm, st); else P.timer.end(st);
-                    });
+                    }, w.vnorm);
     KCHECK();
   }
   check_info(P, w.info, Cs, "eigensolver (QR iteration did not converge)");
@@ -1381,7 +1397,7 @@ static void execute_chunk_layers(orf_plan& P, Work& w, int p0, int Cp) {
     KCHECK();
   }
   gather_pages_kernel<<<g1(n2, CsAll), 256, 0, st>>>(n2, w.groupOf, P.dV0.as<cplx>(), n2, w.X2, n2);
-  smatrix_layer(P, w, CsAll, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1]);
+  smatrix_layer(P, w, CsAll, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1], CsP);
   // layer order: Sa[0] = S11 (= S22), Sa[1] = S12 (= S21) of page (j, m, s) at j * CpS + m * nSec + s
   gather_pages_kernel<<<g1(n2, CsAll), 256, 0, st>>>(n2, w.idxA, w.Sb[0], n2, w.Sa[0], n2);
   gather_pages_kernel<<<g1(n2, CsAll), 256, 0, st>>>(n2, w.idxA, w.Sb[1], n2, w.Sa[1], n2);
@@ -1513,7 +1529,7 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         HIPCHECK(hipMemcpyAsync(K->V.p, w.Vm, sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
         HIPCHECK(hipMemcpyAsync(K->lam.p, w.lam, sizeof(cplx) * nr, hipMemcpyDeviceToDevice, st));
       }
-      smatrix_layer(P, w, Cs, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1]);
+      smatrix_layer(P, w, Cs, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1], P.layerAnyPat[j] ? Cs : 0);
       if (P.keepModes) {
         auto* K = P.kept[j];
         K->S11.alloc(sizeof(cplx) * n2);
