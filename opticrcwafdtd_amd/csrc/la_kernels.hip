@@ -662,8 +662,9 @@ __global__ __launch_bounds__(256) void small_inverse_kernel(int n, const cplx* _
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long long mat = (long long)blockIdx.x * 4 + wv;
   const int n2 = 2 * n;
-  cplx* M = gjs + (size_t)wv * n * n2;  // row-major n x 2n
-  if (mat >= count) return;             // whole waves only; no block barrier below
+  cplx* M = gjs + (size_t)wv * (n * n2 + n);  // row-major n x 2n, then the column of factors
+  cplx* fcol = M + n * n2;
+  if (mat >= count) return;  // whole waves only; no block barrier below
   A += (size_t)mat * sA;
   X += (size_t)mat * sX;
   for (int e = lane; e < n * n2; e += 64) {
@@ -671,6 +672,8 @@ __global__ __launch_bounds__(256) void small_inverse_kernel(int n, const cplx* _
     M[e] = c < n ? A[r + (size_t)c * n] : cmk(c - n == r ? 1.0 : 0.0, 0.0);
   }
   wave_sync_lds();
+  // lanes cover rpi rows of 2n columns per pass
+  const int rpi = 64 / n2, lr = lane / n2, lc = lane % n2;
   bool singular = false;
   for (int k = 0; k < n; ++k) {
     // pivot: lane r - k looks at row r
@@ -691,24 +694,14 @@ __global__ __launch_bounds__(256) void small_inverse_kernel(int n, const cplx* _
       }
     wave_sync_lds();
     const cplx rp = singular ? cmk(0.0, 0.0) : crecip(M[k * n2 + k]);
-    wave_sync_lds();  // every lane has the pivot before row k changes
+    // factors of every other row, read before row k or any other row changes
+    if (lane < n) fcol[lane] = lane == k ? cmk(0.0, 0.0) : M[lane * n2 + k];
+    wave_sync_lds();
     for (int c = lane; c < n2; c += 64) M[k * n2 + c] = cmul(M[k * n2 + c], rp);
     wave_sync_lds();
-    // eliminate column k from every other row (factors read before any update)
-    cplx f[SMALLINV_MAX];
-#pragma unroll
-    for (int q = 0; q < SMALLINV_MAX; ++q) f[q] = (q < n && q != k) ? M[q * n2 + k] : cmk(0.0, 0.0);
-    wave_sync_lds();
-    for (int e = lane; e < n * n2; e += 64) {
-      const int rr = e / n2, c = e % n2;
-      if (rr != k) {
-        cplx fr = cmk(0.0, 0.0);
-#pragma unroll
-        for (int q = 0; q < SMALLINV_MAX; ++q)
-          if (q == rr) fr = f[q];
-        M[e] = csub(M[e], cmul(fr, M[k * n2 + c]));
-      }
-    }
+    if (lr < rpi)
+      for (int rr = lr; rr < n; rr += rpi)
+        if (rr != k) M[rr * n2 + lc] = csub(M[rr * n2 + lc], cmul(fcol[rr], M[k * n2 + lc]));
     wave_sync_lds();
   }
   for (int e = lane; e < n * n; e += 64) {
@@ -1024,7 +1017,7 @@ void orf_zgetrf_only(hipStream_t st, int n, cplx* A, int lda, long long sA, int*
 bool orf_small_inverse(hipStream_t st, int n, const cplx* A, long long sA, cplx* X, long long sX,
                        long long count) {
   if (n <= 0 || n > SMALLINV_MAX || count <= 0) return false;
-  const size_t lds = (size_t)4 * n * 2 * n * sizeof(cplx);
+  const size_t lds = (size_t)4 * (n * 2 * n + n) * sizeof(cplx);
   for (long long b0 = 0; b0 < count; b0 += 4LL * 65535 * 4) {
     const long long cnt = std::min<long long>(count - b0, 4LL * 65535 * 4);
     small_inverse_kernel<<<(unsigned)((cnt + 3) / 4), 256, lds, st>>>(n, A + (size_t)b0 * sA, sA,
