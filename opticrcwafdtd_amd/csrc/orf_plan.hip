@@ -708,6 +708,49 @@ static void smatrix_star_cols(orf_plan& P, Work& w, int Cs, cplx* const SA[4], c
   KCHECK();
 }
 
+// Source-column result of S_ref * L * S_trn, with L the composite S-matrix of
+// the layer stack (rcwa2D.m:31-35; smatrixStar.m:4-10 applied to vectors
+// instead of forming S_ref * L).  With a = S_ref, b = S_trn and the gap
+// amplitudes below the incident region c1+ (down) and c1- (up):
+//   K = (I - L22 b11)^-1 L21,   G = L11 + L12 b11 K   (c1- = G c1+),
+//   (I - a22 G) c1+ = a21 C,   R = a11 C + a12 G c1+,   T = b21 K c1+.
+// Two LUs and one n-column solve instead of a full star product (one LU, two
+// full solves, eight products) followed by the column star.
+static void finish_stack(orf_plan& P, Work& w, int Cs, cplx* const a[4], cplx* const L[4], const cplx* b11,
+                         const cplx* b21, int ncol, const int* colIdx, cplx* R, cplx* T) {
+  const int n = P.nr;
+  const long long n2 = (long long)n * n, nc2 = (long long)n * ncol;
+  hipStream_t st = P.st;
+  STAGE("star_final");
+  cplx* Nm = w.M1;   // I - L22 b11, then I - a22 G
+  cplx* Km = w.Tm;   // K
+  cplx* Pm = w.Fm;   // L12 b11
+  cplx* Gm = w.Dm;   // G
+  // three n x ncol vectors per page inside XB's n x n page (page stride n2)
+  cplx* U = w.XB;             // a21 C, then c1+
+  cplx* gv = w.XB + nc2;      // G c1+
+  cplx* kv = w.XB + 2 * nc2;  // K c1+
+  eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, Nm, n2);
+  bulk_gemm(P, 'N', 'N', n, n, n, MONE, L[3], n, n2, b11, n, n2, ONE, Nm, n, n2, Cs);
+  orf_zgetrf(st, n, Nm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
+  HIPCHECK(hipMemcpyAsync(Km, L[2], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+  orf_zgetrs(st, 'L', n, n, Nm, n, n2, w.ipiv, w.dinv, Km, n, n2, Cs);
+  bulk_gemm(P, 'N', 'N', n, n, n, ONE, L[1], n, n2, b11, n, n2, ZERO, Pm, n, n2, Cs);
+  HIPCHECK(hipMemcpyAsync(Gm, L[0], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+  bulk_gemm(P, 'N', 'N', n, n, n, ONE, Pm, n, n2, Km, n, n2, ONE, Gm, n, n2, Cs);
+  eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, Nm, n2);
+  bulk_gemm(P, 'N', 'N', n, n, n, MONE, a[3], n, n2, Gm, n, n2, ONE, Nm, n, n2, Cs);
+  orf_zgetrf(st, n, Nm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
+  select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[2], n2, U, n2);
+  select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[0], n2, R, nc2);
+  orf_zgetrs(st, 'L', n, ncol, Nm, n, n2, w.ipiv, w.dinv, U, n, n2, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Gm, n, n2, U, n, n2, ZERO, gv, n, n2, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, a[1], n, n2, gv, n, n2, ONE, R, n, nc2, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Km, n, n2, U, n, n2, ZERO, kv, n, n2, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, b21, n, n2, kv, n, n2, ZERO, T, n, nc2, Cs);
+  KCHECK();
+}
+
 // ---------------------------------------------------------------- region S-matrices
 // smatrixRegion.m:8-20 with W = W0 = I for every (context, sector) group.
 static void build_regions(orf_plan& P, const std::vector<zc>& epsInc, const std::vector<zc>& epsSub,
@@ -1302,12 +1345,19 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
 // efficiencies.  Expects w.solves / w.groupOf in the standard order
 // (solve page m * nSec + s for mode page m of the chunk).  SB[0], SB[2] receive
 // S_trn; R, T (n x ncol per solve page) are scratch distinct from SA and SB.
-static void finish_chunk(orf_plan& P, Work& w, int p0, int Cp, cplx* const SA[4], cplx* const SB[4], cplx* R, cplx* T) {
+// Stack form (L != nullptr): SA receives S_ref, L is the layers' composite
+// S-matrix and finish_stack gives the result; otherwise SA is the accumulated
+// S_ref * L and the column star with S_trn gives it.
+static void finish_chunk(orf_plan& P, Work& w, int p0, int Cp, cplx* const SA[4], cplx* const SB[4], cplx* R, cplx* T,
+                         cplx* const* L = nullptr) {
   hipStream_t st = P.st;
   const int nr = P.nr, nSec = P.nSec, Cs = Cp * nSec;
   const long long n2 = (long long)nr * nr, G = (long long)P.nCtx * nSec;
   for (int k = 0; k < 4; k += 2)
     gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dStrn.as<cplx>() + k * n2 * G, n2, SB[k], n2);
+  if (L)
+    for (int k = 0; k < 4; ++k)
+      gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dSref.as<cplx>() + k * n2 * G, n2, SA[k], n2);
   const int ncol = P.reduced ? 1 : 2;
   {
     std::vector<int> cols((size_t)Cs * ncol);
@@ -1317,7 +1367,8 @@ static void finish_chunk(orf_plan& P, Work& w, int p0, int Cp, cplx* const SA[4]
     }
     HIPCHECK(hipMemcpyAsync(w.colIdx, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, st));
   }
-  smatrix_star_cols(P, w, Cs, SA, SB, ncol, w.colIdx, R, T);
+  if (L) finish_stack(P, w, Cs, SA, L, SB[0], SB[2], ncol, w.colIdx, R, T);
+  else smatrix_star_cols(P, w, Cs, SA, SB, ncol, w.colIdx, R, T);
   {
     STAGE("output");
     const DevPage* basePages = P.dPagesByLayer[P.nLayers]->as<DevPage>() + p0;
@@ -1447,19 +1498,16 @@ static void execute_chunk_layers(orf_plan& P, Work& w, int p0, int Cp) {
     }
   HIPCHECK(hipMemcpyAsync(w.solves, sv0.data(), sizeof(DevSolve) * CpS, hipMemcpyHostToDevice, st));
   HIPCHECK(hipMemcpyAsync(w.groupOf, grp0.data(), sizeof(int) * CpS, hipMemcpyHostToDevice, st));
-  // S_acc = S_ref * (S_1 * ... * S_L)
+  // result of S_ref * (S_1 * ... * S_L) * S_trn on the source columns; cur holds the
+  // layers' composite (in one of the Sa / Sb sets), the regions go to Sn and the other set
   cplx* Ref[4];
-  cplx* Acc[4];
   cplx* Trn[4];
   for (int k = 0; k < 4; ++k) {
     Ref[k] = w.Sn[k];
-    Acc[k] = inSa ? w.Sb[k] : w.Sa[k];
-    Trn[k] = inSa ? w.Sa[k] : w.Sb[k];
-    gather_pages_kernel<<<g1(n2, CpS), 256, 0, st>>>(n2, w.groupOf, P.dSref.as<cplx>() + k * n2 * G, n2, Ref[k], n2);
+    Trn[k] = inSa ? w.Sb[k] : w.Sa[k];
   }
-  smatrix_star(P, w, CpS, Ref, cur, Acc);
-  // Trn aliases cur (the other set); cur is consumed by the star above
-  finish_chunk(P, w, p0, Cp, Acc, Trn, w.Sn[0], w.Sn[2]);
+  (void)G;
+  finish_chunk(P, w, p0, Cp, Ref, Trn, w.X1, w.X2, cur);
   HIPCHECK(hipStreamSynchronize(st));
 }
 
@@ -1492,9 +1540,11 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
     HIPCHECK(hipMemcpyAsync(w.solves, sv.data(), sizeof(DevSolve) * Cs, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(w.groupOf, grp.data(), sizeof(int) * Cs, hipMemcpyHostToDevice, st));
     const long long G = (long long)P.nCtx * nSec;
-    // S_acc = S_ref of each page's (context, sector)
-    for (int k = 0; k < 4; ++k)
-      gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dSref.as<cplx>() + k * n2 * G, n2, w.Sa[k], n2);
+    // with layers: L = S_1 * ... * S_L accumulates in Sa and the regions join at the
+    // end (finish_stack); without: Sa = S_ref and the column star with S_trn
+    if (P.nLayers == 0)
+      for (int k = 0; k < 4; ++k)
+        gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dSref.as<cplx>() + k * n2 * G, n2, w.Sa[k], n2);
     KCHECK();
     for (int j = 0; j < P.nLayers; ++j) {
       const DevPage* pages = P.dPagesByLayer[j]->as<DevPage>() + p0;
@@ -1538,8 +1588,13 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         HIPCHECK(hipMemcpyAsync(K->S12.p, w.Sb[1], sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
       }
       cplx* SB[4] = {w.Sb[0], w.Sb[1], w.Sb[1], w.Sb[0]};
-      smatrix_star(P, w, Cs, w.Sa, SB, w.Sn);
-      for (int k = 0; k < 4; ++k) std::swap(w.Sa[k], w.Sn[k]);
+      if (j == 0) {
+        for (int k = 0; k < 4; ++k)
+          HIPCHECK(hipMemcpyAsync(w.Sa[k], SB[k], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+      } else {
+        smatrix_star(P, w, Cs, w.Sa, SB, w.Sn);
+        for (int k = 0; k < 4; ++k) std::swap(w.Sa[k], w.Sn[k]);
+      }
       HIPCHECK(hipStreamSynchronize(st));
       if (progress) {
         const double frac = (double)(ch * P.nLayers + j + 1) / (double)(nChunks * P.nLayers);
@@ -1551,7 +1606,8 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         }
       }
     }
-    finish_chunk(P, w, p0, Cp, w.Sa, w.Sb, w.Sn[0], w.Sn[2]);
+    if (P.nLayers == 0) finish_chunk(P, w, p0, Cp, w.Sa, w.Sb, w.Sn[0], w.Sn[2]);
+    else finish_chunk(P, w, p0, Cp, w.Sn, w.Sb, w.X1, w.X2, w.Sa);
     if (progress && P.nLayers == 0) {
       if (progress((double)(ch + 1) / nChunks, "done", user)) fail(ORF_CANCELLED, "cancelled by the progress callback");
     }
