@@ -65,6 +65,10 @@ __global__ void eye_axpby_kernel(int, cplx, cplx, const cplx*, long long, cplx*,
 __global__ void gather_pages_kernel(long long, const int*, const cplx*, long long, cplx*, long long);
 __global__ void eye_stack_kernel(int, long long, cplx*);
 __global__ void select_cols_kernel(int, int, const int*, const cplx*, long long, cplx*, long long);
+__global__ void xr_pair_kernel(int, const cplx*, long long, const cplx*, long long, cplx, cplx, const cplx*, long long,
+                               cplx*, long long);
+__global__ void rx_pair_kernel(int, const cplx*, long long, const cplx*, long long, cplx, cplx, const cplx*, long long,
+                               cplx*, long long);
 __global__ void field_amp_kernel(int, int, const cplx*, const cplx*, const cplx*, double, double, const double*, cplx*,
                                  cplx*);
 __global__ void curl_z_kernel(int, int, const double*, const double*, const cplx*, cplx, cplx*);
@@ -730,16 +734,22 @@ static void finish_stack(orf_plan& P, Work& w, int Cs, cplx* const a[4], cplx* c
   cplx* U = w.XB;             // a21 C, then c1+
   cplx* gv = w.XB + nc2;      // G c1+
   cplx* kv = w.XB + 2 * nc2;  // K c1+
+  // in the full basis the half-space blocks pair each harmonic with its other
+  // polarisation only: their products are O(n^2) (xr_pair / rx_pair)
+  const bool pair = !P.reduced;
   eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, Nm, n2);
-  bulk_gemm(P, 'N', 'N', n, n, n, MONE, L[3], n, n2, b11, n, n2, ONE, Nm, n, n2, Cs);
+  if (pair) xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, L[3], n2, b11, n2, MONE, ONE, Nm, n2, Nm, n2);
+  else bulk_gemm(P, 'N', 'N', n, n, n, MONE, L[3], n, n2, b11, n, n2, ONE, Nm, n, n2, Cs);
   orf_zgetrf(st, n, Nm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   HIPCHECK(hipMemcpyAsync(Km, L[2], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
   orf_zgetrs(st, 'L', n, n, Nm, n, n2, w.ipiv, w.dinv, Km, n, n2, Cs);
-  bulk_gemm(P, 'N', 'N', n, n, n, ONE, L[1], n, n2, b11, n, n2, ZERO, Pm, n, n2, Cs);
+  if (pair) xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, L[1], n2, b11, n2, ONE, ZERO, nullptr, 0, Pm, n2);
+  else bulk_gemm(P, 'N', 'N', n, n, n, ONE, L[1], n, n2, b11, n, n2, ZERO, Pm, n, n2, Cs);
   HIPCHECK(hipMemcpyAsync(Gm, L[0], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
   bulk_gemm(P, 'N', 'N', n, n, n, ONE, Pm, n, n2, Km, n, n2, ONE, Gm, n, n2, Cs);
   eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, Nm, n2);
-  bulk_gemm(P, 'N', 'N', n, n, n, MONE, a[3], n, n2, Gm, n, n2, ONE, Nm, n, n2, Cs);
+  if (pair) rx_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, a[3], n2, Gm, n2, MONE, ONE, Nm, n2, Nm, n2);
+  else bulk_gemm(P, 'N', 'N', n, n, n, MONE, a[3], n, n2, Gm, n, n2, ONE, Nm, n, n2, Cs);
   orf_zgetrf(st, n, Nm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[2], n2, U, n2);
   select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[0], n2, R, nc2);

@@ -558,6 +558,41 @@ __global__ void eye_stack_kernel(int n, long long count, cplx* __restrict__ Y) {
 }
 
 // out[:, q] = S[:, colIdx[page * ncol + q]]  (n x ncol per page)
+// Products with a half-space S-matrix block in the full (m, n, x/y) basis:
+// with W = W0 = I every block couples harmonic j only to itself and to its
+// other polarisation j +- M, so R(:, j) has nonzeros in rows j mod M and
+// j mod M + M only.  Out = alpha X R + beta Cin (xr) or alpha R X + beta Cin
+// (rx), n = 2M, pages along blockIdx.y; Out may be Cin.
+__global__ void xr_pair_kernel(int M, const cplx* __restrict__ X, long long sX, const cplx* __restrict__ R,
+                               long long sR, cplx alpha, cplx beta, const cplx* Cin, long long sC, cplx* Out,
+                               long long sO) {
+  const int n = 2 * M;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int i = (int)(e % n), j = (int)(e / n), j1 = j % M, j2 = j1 + M;
+  X += (size_t)blockIdx.y * sX;
+  R += (size_t)blockIdx.y * sR;
+  const cplx v = cadd(cmul(X[i + (size_t)j1 * n], R[j1 + (size_t)j * n]), cmul(X[i + (size_t)j2 * n], R[j2 + (size_t)j * n]));
+  cplx o = cmul(alpha, v);
+  if (beta.x != 0.0 || beta.y != 0.0) o = cadd(o, cmul(beta, Cin[(size_t)blockIdx.y * sC + e]));
+  Out[(size_t)blockIdx.y * sO + e] = o;
+}
+
+__global__ void rx_pair_kernel(int M, const cplx* __restrict__ R, long long sR, const cplx* __restrict__ X,
+                               long long sX, cplx alpha, cplx beta, const cplx* Cin, long long sC, cplx* Out,
+                               long long sO) {
+  const int n = 2 * M;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int i = (int)(e % n), j = (int)(e / n), i1 = i % M, i2 = i1 + M;
+  X += (size_t)blockIdx.y * sX;
+  R += (size_t)blockIdx.y * sR;
+  const cplx v = cadd(cmul(R[i + (size_t)i1 * n], X[i1 + (size_t)j * n]), cmul(R[i + (size_t)i2 * n], X[i2 + (size_t)j * n]));
+  cplx o = cmul(alpha, v);
+  if (beta.x != 0.0 || beta.y != 0.0) o = cadd(o, cmul(beta, Cin[(size_t)blockIdx.y * sC + e]));
+  Out[(size_t)blockIdx.y * sO + e] = o;
+}
+
 __global__ void select_cols_kernel(int n, int ncol, const int* __restrict__ colIdx, const cplx* __restrict__ S,
                                    long long sS, cplx* __restrict__ out, long long sO) {
   const int sp = blockIdx.y;
