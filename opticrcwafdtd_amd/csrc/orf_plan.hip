@@ -351,6 +351,7 @@ struct Work {
   cplx *Sa[4], *Sn[4], *Sb[4], *Tm, *M1, *Fm;
   double* kL;
   double* vnorm;  // column norms the eigensolver removed from W (W = Z Y diag(1 / vnorm))
+  cplx* Vv;       // four n x 2 vectors per solve page (final assembly), page stride 8 n
   cplx* dinv;  // diagonal-block inverses of the most recent LU
   int *ipiv, *info;
   DevSolve* solves;
@@ -462,6 +463,7 @@ static size_t per_solve_bytes(const orf_plan& P) {
   c += n2 * (12 + 3);        // Sa Sn Sb Tm M1 Fm
   c += 3 * (size_t)P.nr + 1; // mu lam Xd eps
   c += (size_t)orf_dinv_stride(P.nr); // diagonal-block inverses
+  c += 8 * (size_t)P.nr;              // Vv
   return c * sizeof(cplx) + sizeof(double) * (1 + (size_t)P.nr) + sizeof(int) * (P.nr + 2) + sizeof(DevSolve);
 }
 
@@ -520,6 +522,7 @@ static Work make_work(orf_plan& P, int Cp) {
   w.Fm = (cplx*)get(cs * n2 * Cs);
   w.kL = (double*)get(sizeof(double) * Cs);
   w.vnorm = (double*)get(sizeof(double) * P.nr * Cs);
+  w.Vv = (cplx*)get(cs * 8 * P.nr * Cs);
   w.dinv = (cplx*)get(cs * std::max((size_t)orf_dinv_stride(P.nr) * Cs, (size_t)orf_dinv_stride(P.M) * Cp));
   w.ipiv = (int*)get(sizeof(int) * (size_t)std::max(P.nr, P.M) * std::max(Cs, Cp));
   w.info = (int*)get(sizeof(int) * (size_t)std::max(Cs, Cp) * 64);
@@ -576,8 +579,10 @@ static void check_info(orf_plan& P, const int* dinfo, int count, const char* wha
 // solve page in X2 on entry.  Out: S11 -> S11o, S12 -> S12o.
 // The first CsEig solve pages hold eigensolver output (W = Z Y diag(1 / vnorm)
 // with Z, Y still in w.Zw, w.Yw): their W^-1 comes from one triangular solve.
+// unsolved: stop at D (w.Dm) and [E1 | E2] (w.RHS2, page stride 2 n^2), with
+// S11 = D^-1 E1 and S12 = D^-1 E2, for finish_single.
 static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx* lam, cplx* S11o, cplx* S12o,
-                          int CsEig) {
+                          int CsEig, bool unsolved = false) {
   const int n = P.nr;
   const long long n2 = (long long)n * n;
   hipStream_t st = P.st;
@@ -637,6 +642,7 @@ static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx
     orf_zgemm(st, 'N', 'N', n, n, n, MONE, F, n, 3 * n2, w.X2, n, n2, ONE, w.RHS2 + n2, n, 2 * n2, Cs);
   }
   rowscale_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, n, w.Xd, w.RHS2 + n2, 2 * n2);
+  if (unsolved) return;
   {
     STAGE("smat_lu");
     orf_zgetrf(st, n, w.Dm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
@@ -730,10 +736,10 @@ static void finish_stack(orf_plan& P, Work& w, int Cs, cplx* const a[4], cplx* c
   cplx* Km = w.Tm;   // K
   cplx* Pm = w.Fm;   // L12 b11
   cplx* Gm = w.Dm;   // G
-  // three n x ncol vectors per page inside XB's n x n page (page stride n2)
-  cplx* U = w.XB;             // a21 C, then c1+
-  cplx* gv = w.XB + nc2;      // G c1+
-  cplx* kv = w.XB + 2 * nc2;  // K c1+
+  const long long sv = 8LL * n;  // vector page stride in Vv
+  cplx* U = w.Vv;             // a21 C, then c1+
+  cplx* gv = w.Vv + 2 * n;    // G c1+
+  cplx* kv = w.Vv + 4 * n;    // K c1+
   // in the full basis the half-space blocks pair each harmonic with its other
   // polarisation only: their products are O(n^2) (xr_pair / rx_pair)
   const bool pair = !P.reduced;
@@ -751,13 +757,67 @@ static void finish_stack(orf_plan& P, Work& w, int Cs, cplx* const a[4], cplx* c
   if (pair) rx_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, a[3], n2, Gm, n2, MONE, ONE, Nm, n2, Nm, n2);
   else bulk_gemm(P, 'N', 'N', n, n, n, MONE, a[3], n, n2, Gm, n, n2, ONE, Nm, n, n2, Cs);
   orf_zgetrf(st, n, Nm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
-  select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[2], n2, U, n2);
+  select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[2], n2, U, sv);
   select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[0], n2, R, nc2);
-  orf_zgetrs(st, 'L', n, ncol, Nm, n, n2, w.ipiv, w.dinv, U, n, n2, Cs);
-  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Gm, n, n2, U, n, n2, ZERO, gv, n, n2, Cs);
-  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, a[1], n, n2, gv, n, n2, ONE, R, n, nc2, Cs);
-  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Km, n, n2, U, n, n2, ZERO, kv, n, n2, Cs);
-  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, b21, n, n2, kv, n, n2, ZERO, T, n, nc2, Cs);
+  orf_zgetrs(st, 'L', n, ncol, Nm, n, n2, w.ipiv, w.dinv, U, n, sv, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Gm, n, n2, U, n, sv, ZERO, gv, n, sv, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, a[1], n, n2, gv, n, sv, ONE, R, n, nc2, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Km, n, n2, U, n, sv, ZERO, kv, n, sv, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, b21, n, n2, kv, n, sv, ZERO, T, n, nc2, Cs);
+  KCHECK();
+}
+
+// Source-column result for a single layer whose S-matrix is left unsolved,
+// S11 = D^-1 E1, S12 = D^-1 E2 (S22 = S11, S21 = S12): substituting into
+// finish_stack's relations removes D^-1 (one LU and a 2n-column solve):
+//   K = (D - E1 b11)^-1 E2,   H = E1 + E2 b11 K   (G = D^-1 H),
+//   (D - H a22) g = H a21 C,  c1+ = a21 C + a22 g,
+//   R = a11 C + a12 g,        T = b21 K c1+.
+static void finish_single(orf_plan& P, Work& w, int Cs, cplx* const a[4], const cplx* b11, const cplx* b21,
+                          int ncol, const int* colIdx, cplx* R, cplx* T) {
+  const int n = P.nr;
+  const long long n2 = (long long)n * n, nc2 = (long long)n * ncol;
+  hipStream_t st = P.st;
+  STAGE("star_final");
+  const cplx* Dm = w.Dm;
+  const cplx* E1 = w.RHS2;
+  const cplx* E2 = w.RHS2 + n2;  // E1, E2 page stride 2 n2
+  cplx* Nm = w.M1;   // D - E1 b11, then D - H a22
+  cplx* Km = w.Tm;   // K
+  cplx* Pm = w.Fm;   // E2 b11
+  cplx* Hm = w.LUb;  // H
+  const long long sv = 8LL * n;  // vector page stride in Vv
+  cplx* u0 = w.Vv;            // a21 C
+  cplx* gv = w.Vv + 2 * n;    // H a21 C, then g
+  cplx* c1 = w.Vv + 4 * n;    // c1+
+  cplx* kv = w.Vv + 6 * n;    // K c1+
+  const bool pair = !P.reduced;
+  HIPCHECK(hipMemcpyAsync(Nm, Dm, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+  if (pair) xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, E1, 2 * n2, b11, n2, MONE, ONE, Nm, n2, Nm, n2);
+  else bulk_gemm(P, 'N', 'N', n, n, n, MONE, E1, n, 2 * n2, b11, n, n2, ONE, Nm, n, n2, Cs);
+  orf_zgetrf(st, n, Nm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
+  HIPCHECK(hipMemcpy2DAsync(Km, sizeof(cplx) * n2, E2, sizeof(cplx) * 2 * n2, sizeof(cplx) * n2, Cs,
+                            hipMemcpyDeviceToDevice, st));
+  orf_zgetrs(st, 'L', n, n, Nm, n, n2, w.ipiv, w.dinv, Km, n, n2, Cs);
+  if (pair) xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, E2, 2 * n2, b11, n2, ONE, ZERO, nullptr, 0, Pm, n2);
+  else bulk_gemm(P, 'N', 'N', n, n, n, ONE, E2, n, 2 * n2, b11, n, n2, ZERO, Pm, n, n2, Cs);
+  HIPCHECK(hipMemcpy2DAsync(Hm, sizeof(cplx) * n2, E1, sizeof(cplx) * 2 * n2, sizeof(cplx) * n2, Cs,
+                            hipMemcpyDeviceToDevice, st));
+  bulk_gemm(P, 'N', 'N', n, n, n, ONE, Pm, n, n2, Km, n, n2, ONE, Hm, n, n2, Cs);
+  HIPCHECK(hipMemcpyAsync(Nm, Dm, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+  if (pair) xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, Hm, n2, a[3], n2, MONE, ONE, Nm, n2, Nm, n2);
+  else bulk_gemm(P, 'N', 'N', n, n, n, MONE, Hm, n, n2, a[3], n, n2, ONE, Nm, n, n2, Cs);
+  orf_zgetrf(st, n, Nm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
+  select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[2], n2, u0, sv);
+  select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[0], n2, R, nc2);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Hm, n, n2, u0, n, sv, ZERO, gv, n, sv, Cs);
+  orf_zgetrs(st, 'L', n, ncol, Nm, n, n2, w.ipiv, w.dinv, gv, n, sv, Cs);
+  HIPCHECK(hipMemcpy2DAsync(c1, sizeof(cplx) * sv, u0, sizeof(cplx) * sv, sizeof(cplx) * nc2, Cs,
+                            hipMemcpyDeviceToDevice, st));
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, a[3], n, n2, gv, n, sv, ONE, c1, n, sv, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, a[1], n, n2, gv, n, sv, ONE, R, n, nc2, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Km, n, n2, c1, n, sv, ZERO, kv, n, sv, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, b21, n, n2, kv, n, sv, ZERO, T, n, nc2, Cs);
   KCHECK();
 }
 
@@ -1359,13 +1419,13 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
 // S-matrix and finish_stack gives the result; otherwise SA is the accumulated
 // S_ref * L and the column star with S_trn gives it.
 static void finish_chunk(orf_plan& P, Work& w, int p0, int Cp, cplx* const SA[4], cplx* const SB[4], cplx* R, cplx* T,
-                         cplx* const* L = nullptr) {
+                         cplx* const* L = nullptr, bool single = false) {
   hipStream_t st = P.st;
   const int nr = P.nr, nSec = P.nSec, Cs = Cp * nSec;
   const long long n2 = (long long)nr * nr, G = (long long)P.nCtx * nSec;
   for (int k = 0; k < 4; k += 2)
     gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dStrn.as<cplx>() + k * n2 * G, n2, SB[k], n2);
-  if (L)
+  if (L || single)
     for (int k = 0; k < 4; ++k)
       gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dSref.as<cplx>() + k * n2 * G, n2, SA[k], n2);
   const int ncol = P.reduced ? 1 : 2;
@@ -1377,7 +1437,8 @@ static void finish_chunk(orf_plan& P, Work& w, int p0, int Cp, cplx* const SA[4]
     }
     HIPCHECK(hipMemcpyAsync(w.colIdx, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, st));
   }
-  if (L) finish_stack(P, w, Cs, SA, L, SB[0], SB[2], ncol, w.colIdx, R, T);
+  if (single) finish_single(P, w, Cs, SA, SB[0], SB[2], ncol, w.colIdx, R, T);
+  else if (L) finish_stack(P, w, Cs, SA, L, SB[0], SB[2], ncol, w.colIdx, R, T);
   else smatrix_star_cols(P, w, Cs, SA, SB, ncol, w.colIdx, R, T);
   {
     STAGE("output");
@@ -1589,21 +1650,24 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         HIPCHECK(hipMemcpyAsync(K->V.p, w.Vm, sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
         HIPCHECK(hipMemcpyAsync(K->lam.p, w.lam, sizeof(cplx) * nr, hipMemcpyDeviceToDevice, st));
       }
-      smatrix_layer(P, w, Cs, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1], P.layerAnyPat[j] ? Cs : 0);
-      if (P.keepModes) {
-        auto* K = P.kept[j];
-        K->S11.alloc(sizeof(cplx) * n2);
-        K->S12.alloc(sizeof(cplx) * n2);
-        HIPCHECK(hipMemcpyAsync(K->S11.p, w.Sb[0], sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
-        HIPCHECK(hipMemcpyAsync(K->S12.p, w.Sb[1], sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
-      }
-      cplx* SB[4] = {w.Sb[0], w.Sb[1], w.Sb[1], w.Sb[0]};
-      if (j == 0) {
-        for (int k = 0; k < 4; ++k)
-          HIPCHECK(hipMemcpyAsync(w.Sa[k], SB[k], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-      } else {
-        smatrix_star(P, w, Cs, w.Sa, SB, w.Sn);
-        for (int k = 0; k < 4; ++k) std::swap(w.Sa[k], w.Sn[k]);
+      const bool single = P.nLayers == 1 && !P.keepModes;  // finish_single takes D, E1, E2
+      smatrix_layer(P, w, Cs, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1], P.layerAnyPat[j] ? Cs : 0, single);
+      if (!single) {
+        if (P.keepModes) {
+          auto* K = P.kept[j];
+          K->S11.alloc(sizeof(cplx) * n2);
+          K->S12.alloc(sizeof(cplx) * n2);
+          HIPCHECK(hipMemcpyAsync(K->S11.p, w.Sb[0], sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
+          HIPCHECK(hipMemcpyAsync(K->S12.p, w.Sb[1], sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
+        }
+        cplx* SB[4] = {w.Sb[0], w.Sb[1], w.Sb[1], w.Sb[0]};
+        if (j == 0) {
+          for (int k = 0; k < 4; ++k)
+            HIPCHECK(hipMemcpyAsync(w.Sa[k], SB[k], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+        } else {
+          smatrix_star(P, w, Cs, w.Sa, SB, w.Sn);
+          for (int k = 0; k < 4; ++k) std::swap(w.Sa[k], w.Sn[k]);
+        }
       }
       HIPCHECK(hipStreamSynchronize(st));
       if (progress) {
@@ -1617,6 +1681,7 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
       }
     }
     if (P.nLayers == 0) finish_chunk(P, w, p0, Cp, w.Sa, w.Sb, w.Sn[0], w.Sn[2]);
+    else if (P.nLayers == 1 && !P.keepModes) finish_chunk(P, w, p0, Cp, w.Sn, w.Sb, w.X1, w.X2, nullptr, true);
     else finish_chunk(P, w, p0, Cp, w.Sn, w.Sb, w.X1, w.X2, w.Sa);
     if (progress && P.nLayers == 0) {
       if (progress((double)(ch + 1) / nChunks, "done", user)) fail(ORF_CANCELLED, "cancelled by the progress callback");

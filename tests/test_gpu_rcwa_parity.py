@@ -97,6 +97,20 @@ def test_reticolo_phase_library(sym):
     assert np.max(np.abs(np.angle(t / t[0]) - np.angle(np.exp(1j * (ph - ph[0]))))) < 1e-9
 
 
+@pytest.mark.parametrize("N", [0, 1])
+@pytest.mark.parametrize("nl", [1, 2])
+def test_few_harmonics_single_and_stacked(N, nl):
+    """Tiny solved sizes (n = 2 and 18) through the single-layer and the stacked
+    final assembly, oblique incidence."""
+    layers = [orf.makeLayer(0.3, mc(1), [orf.shapeCircle(0.12, mc(3.6 + 0.01j))])]
+    if nl == 2:
+        layers.append(orf.makeLayer(0.2, mc(2.0)))
+    dev = orf.makeStructure([0.4, 0.4], mc(1.45), mc(1), layers)
+    g = orf.rcwa2D([dev], 0.94, 0.2, 0.3, "s", Nx=N, Nsample=(64, 64), Symmetry="none")
+    o = O.rcwa2d([dev], 0.94, 0.2, 0.3, "s", Nx=N, Nsample=(64, 64), Symmetry="none")
+    compare(g, o)
+
+
 def test_homogeneous_conical_equals_tmm():
     th, ph = math.radians(30), math.radians(40)
     dev = orf.makeStructure([0.5, 0.4], mc(1), mc(3.48),
