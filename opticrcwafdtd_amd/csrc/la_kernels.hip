@@ -1530,7 +1530,7 @@ __global__ __launch_bounds__(256) void hess_q_apply_kernel(int n, const cplx* __
 // Complex Givens rotation G = [c s; -conj(s) c] with G [a; b] = [r; 0], c real.
 // Inputs are scaled by max(|a|_1, |b|_1) so the squared norms neither
 // overflow nor underflow; two square roots and three reciprocals.
-ORF_DEV void givens(cplx a, cplx b, double& c, cplx& s, cplx& r) {
+ORF_DEV void givens_scaled(cplx a, cplx b, double& c, cplx& s, cplx& r) {
   const double sc = fmax(cabs1(a), cabs1(b));
   if (cabs1(b) == 0.0 || sc == 0.0) { c = 1.0; s = cmk(0.0, 0.0); r = a; return; }
   const double isc = 1.0 / sc;
@@ -1549,6 +1549,22 @@ ORF_DEV void givens(cplx a, cplx b, double& c, cplx& s, cplx& r) {
   c = aa * irho;
   s = cscale(cmul(ph, cconj(bs)), irho);
   r = cscale(ph, rho * sc);
+}
+
+// The same rotation with two reciprocal square roots and no divisions when the
+// squared norms are safely inside the double range (the QR's rotation chains
+// are latency bound); otherwise the scaled form.
+ORF_DEV void givens(cplx a, cplx b, double& c, cplx& s, cplx& r) {
+  const double na2 = cnorm2(a), nb2 = cnorm2(b), t = na2 + nb2;
+  if (!(na2 > 1e-250 && t < 1e250) || nb2 == 0.0) {
+    givens_scaled(a, b, c, s, r);
+    return;
+  }
+  const double irho = rsqrt(t), ia = rsqrt(na2);
+  const cplx ph = cscale(a, ia);  // a / |a|
+  c = na2 * ia * irho;            // |a| / rho
+  s = cscale(cmul(ph, cconj(b)), irho);
+  r = cscale(ph, t * irho);
 }
 
 // Complex Schur form by single-shift QR on a Hessenberg matrix (in place);
