@@ -574,12 +574,14 @@ __global__ __launch_bounds__(256) void zgetrf_blocked_kernel(int n, cplx* __rest
         Pn[bi + (size_t)t * n] = x;
       }
       __syncthreads();
+      // each thread scales its rows' multiplier and updates the rest of those rows
+      // (row j is read only, so no barrier between the two)
       const cplx rp = crecip(Pn[j + (size_t)j * n]);
-      for (int r = j + 1 + t; r < m; r += nt) Pn[r + (size_t)j * n] = cmul(Pn[r + (size_t)j * n], rp);
-      __syncthreads();
-      for (int e = t; e < (m - j - 1) * (nb - j - 1); e += nt) {
-        const int r = j + 1 + e % (m - j - 1), c = j + 1 + e / (m - j - 1);
-        Pn[r + (size_t)c * n] = csub(Pn[r + (size_t)c * n], cmul(Pn[r + (size_t)j * n], Pn[j + (size_t)c * n]));
+      for (int r = j + 1 + t; r < m; r += nt) {
+        const cplx l = cmul(Pn[r + (size_t)j * n], rp);
+        Pn[r + (size_t)j * n] = l;
+        for (int c = j + 1; c < nb; ++c)
+          Pn[r + (size_t)c * n] = csub(Pn[r + (size_t)c * n], cmul(l, Pn[j + (size_t)c * n]));
       }
       __syncthreads();
     }
