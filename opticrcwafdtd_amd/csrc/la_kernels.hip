@@ -1553,6 +1553,14 @@ ORF_DEV void givens_scaled(cplx a, cplx b, double& c, cplx& s, cplx& r) {
   r = cscale(ph, rho * sc);
 }
 
+// 1 / sqrt(x) with one Newton step on the device reciprocal square root, so the
+// rotations stay unitary to rounding (without it the QR stalls on some pages).
+ORF_DEV double rsqrt_nr(double x) {
+  const double y = rsqrt(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
 // The same rotation with two reciprocal square roots and no divisions when the
 // squared norms are safely inside the double range (the QR's rotation chains
 // are latency bound); otherwise the scaled form.
@@ -1562,7 +1570,7 @@ ORF_DEV void givens(cplx a, cplx b, double& c, cplx& s, cplx& r) {
     givens_scaled(a, b, c, s, r);
     return;
   }
-  const double irho = rsqrt(t), ia = rsqrt(na2);
+  const double irho = rsqrt_nr(t), ia = rsqrt_nr(na2);
   const cplx ph = cscale(a, ia);  // a / |a|
   c = na2 * ia * irho;            // |a| / rho
   s = cscale(cmul(ph, cconj(b)), irho);
