@@ -1474,10 +1474,26 @@ __global__ void panel_extract_kernel(int n, int k0, int ib, const cplx* __restri
     if (r > j) v = V[(size_t)(k0 + j) * n + k0 + 1 + r];
     else if (r == j) v = cmk(1.0, 0.0);
     Vc[(size_t)page * sVc + e] = v;
-  } else if (e < (long long)m * ib + ib * ib) {
+  } else if (Tc && e < (long long)m * ib + ib * ib) {
     const int q = (int)(e - (long long)m * ib), l = q % ib, j = q / ib;
     Tc[(size_t)page * sTc + q] = (l <= j) ? V[(size_t)(k0 + j) * n + l] : cmk(0.0, 0.0);
   }
+}
+
+// T of two consecutive panels (a at ka, b at ka + iba) placed on the diagonal of
+// the (iba + ibb)^2 block T2 (ld iba + ibb), zero below; the upper-right block
+// -Ta (Va^H Vb) Tb is filled in afterwards by two small products.
+__global__ void panel_pair_t_kernel(int n, int ka, int iba, int ibb, const cplx* __restrict__ Vg,
+                                    cplx* __restrict__ T2, long long sT2) {
+  const int page = blockIdx.y, ib = iba + ibb;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ib * ib) return;
+  const int l = q % ib, j = q / ib;
+  const cplx* V = Vg + (size_t)page * n * n;
+  cplx v = cmk(0.0, 0.0);
+  if (j < iba && l <= j) v = V[(size_t)(ka + j) * n + l];
+  else if (j >= iba && l >= iba && l <= j) v = V[(size_t)(ka + j) * n + (l - iba)];
+  T2[(size_t)page * sT2 + q] = v;
 }
 
 // W := Q W with Q = H_0 H_1 ... H_{n-3} the Hessenberg reduction's reflectors,
@@ -2732,25 +2748,48 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
     // MFMA GEMMs on that block (clean V_p, T_p and the products in Ywork)
     const int NB = blockedNB, nref = n - 2;
     const long long sY = sZ;
-    cplx* Vc = Ywork;
-    cplx* X = Ywork + (size_t)n * NB;
-    cplx* Yp = Ywork + (size_t)2 * n * NB;
-    cplx* Tc = Ywork + (size_t)3 * n * NB;
     for (int b0 = 0; b0 < batch; b0 += 65535) {
       const int bb = batch - b0 < 65535 ? batch - b0 : 65535;
       eye_pages_kernel<<<dim3((unsigned)(((long long)n * n + 255) / 256), bb), 256, 0, st>>>(n, Zwork + (size_t)b0 * sZ, sZ);
     }
-    for (int k0 = ((nref - 1) / NB) * NB; k0 >= 0; k0 -= NB) {
-      const int ib = min(NB, nref - k0), m = n - k0 - 1;
+    // two panels per pass where the scratch allows: P_a P_b = I - [Va Vb] T2 [Va Vb]^H with
+    // T2 = [Ta, -Ta Va^H Vb Tb; 0, Tb] (one pass over the trailing block instead of two)
+    const bool pairs = NB == 16 && (long long)3 * n * 2 * NB + 4 * NB * NB + 2 * NB * NB <= (long long)n * n;
+    int k0 = ((nref - 1) / NB) * NB;
+    while (k0 >= 0) {
+      const bool pr = pairs && k0 >= NB;
+      const int ka = pr ? k0 - NB : k0;
+      const int iba = pr ? NB : min(NB, nref - k0), ibb = pr ? min(NB, nref - k0) : 0;
+      const int ib = iba + ibb, m = n - ka - 1;
+      const int W = pairs ? 2 * NB : NB;  // widest panel this reduction uses
+      cplx* Vp = Ywork;                    // m x ib
+      cplx* Xp = Ywork + (size_t)n * W;    // ib x m
+      cplx* Ypp = Ywork + (size_t)2 * n * W;
+      cplx* Tp = Ywork + (size_t)3 * n * W;  // ib x ib
+      cplx* Sp = Tp + W * W;                 // NB x NB
+      cplx* Wp = Sp + NB * NB;
       for (int b0 = 0; b0 < batch; b0 += 65535) {
         const int bb = batch - b0 < 65535 ? batch - b0 : 65535;
         panel_extract_kernel<<<dim3((unsigned)(((long long)m * ib + ib * ib + 255) / 256), bb), 256, 0, st>>>(
-            n, k0, ib, Vh + (size_t)b0 * n * n, Vc + (size_t)b0 * sY, sY, Tc + (size_t)b0 * sY, sY);
+            n, ka, ib, Vh + (size_t)b0 * n * n, Vp + (size_t)b0 * sY, sY, pr ? nullptr : Tp + (size_t)b0 * sY, sY);
+        if (pr)
+          panel_pair_t_kernel<<<dim3((unsigned)((ib * ib + 255) / 256), bb), 256, 0, st>>>(
+              n, ka, iba, ibb, Vh + (size_t)b0 * n * n, Tp + (size_t)b0 * sY, sY);
       }
-      cplx* Q22 = Zwork + (size_t)(k0 + 1) * n + (k0 + 1);
-      orf_zgemm(st, 'C', 'N', ib, m, m, cmk(1.0, 0.0), Vc, m, sY, Q22, n, sZ, cmk(0.0, 0.0), X, ib, sY, batch);
-      orf_zgemm(st, 'N', 'N', ib, m, ib, cmk(1.0, 0.0), Tc, ib, sY, X, ib, sY, cmk(0.0, 0.0), Yp, ib, sY, batch);
-      orf_zgemm(st, 'N', 'N', m, m, ib, cmk(-1.0, 0.0), Vc, m, sY, Yp, ib, sY, cmk(1.0, 0.0), Q22, n, sZ, batch);
+      if (pr) {
+        // S = Va^H Vb, W = S Tb, T2(0:iba, iba:) = -Ta W
+        orf_zgemm(st, 'C', 'N', iba, ibb, m, cmk(1.0, 0.0), Vp, m, sY, Vp + (size_t)m * iba, m, sY, cmk(0.0, 0.0), Sp,
+                  iba, sY, batch);
+        orf_zgemm(st, 'N', 'N', iba, ibb, ibb, cmk(1.0, 0.0), Sp, iba, sY, Tp + (size_t)iba * ib + iba, ib, sY,
+                  cmk(0.0, 0.0), Wp, iba, sY, batch);
+        orf_zgemm(st, 'N', 'N', iba, ibb, iba, cmk(-1.0, 0.0), Tp, ib, sY, Wp, iba, sY, cmk(0.0, 0.0),
+                  Tp + (size_t)iba * ib, ib, sY, batch);
+      }
+      cplx* Q22 = Zwork + (size_t)(ka + 1) * n + (ka + 1);
+      orf_zgemm(st, 'C', 'N', ib, m, m, cmk(1.0, 0.0), Vp, m, sY, Q22, n, sZ, cmk(0.0, 0.0), Xp, ib, sY, batch);
+      orf_zgemm(st, 'N', 'N', ib, m, ib, cmk(1.0, 0.0), Tp, ib, sY, Xp, ib, sY, cmk(0.0, 0.0), Ypp, ib, sY, batch);
+      orf_zgemm(st, 'N', 'N', m, m, ib, cmk(-1.0, 0.0), Vp, m, sY, Ypp, ib, sY, cmk(1.0, 0.0), Q22, n, sZ, batch);
+      k0 = ka - NB;
     }
   } else {
     for (int b0 = 0; b0 < batch; b0 += 65535) {

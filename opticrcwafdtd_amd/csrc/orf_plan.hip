@@ -792,9 +792,12 @@ static void finish_single(orf_plan& P, Work& w, int Cs, cplx* const a[4], const 
   cplx* c1 = w.Vv + 4 * n;    // c1+
   cplx* kv = w.Vv + 6 * n;    // K c1+
   const bool pair = !P.reduced;
-  HIPCHECK(hipMemcpyAsync(Nm, Dm, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-  if (pair) xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, E1, 2 * n2, b11, n2, MONE, ONE, Nm, n2, Nm, n2);
-  else bulk_gemm(P, 'N', 'N', n, n, n, MONE, E1, n, 2 * n2, b11, n, n2, ONE, Nm, n, n2, Cs);
+  if (pair) {
+    xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, E1, 2 * n2, b11, n2, MONE, ONE, Dm, n2, Nm, n2);
+  } else {
+    HIPCHECK(hipMemcpyAsync(Nm, Dm, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+    bulk_gemm(P, 'N', 'N', n, n, n, MONE, E1, n, 2 * n2, b11, n, n2, ONE, Nm, n, n2, Cs);
+  }
   orf_zgetrf(st, n, Nm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   HIPCHECK(hipMemcpy2DAsync(Km, sizeof(cplx) * n2, E2, sizeof(cplx) * 2 * n2, sizeof(cplx) * n2, Cs,
                             hipMemcpyDeviceToDevice, st));
@@ -804,9 +807,12 @@ static void finish_single(orf_plan& P, Work& w, int Cs, cplx* const a[4], const 
   HIPCHECK(hipMemcpy2DAsync(Hm, sizeof(cplx) * n2, E1, sizeof(cplx) * 2 * n2, sizeof(cplx) * n2, Cs,
                             hipMemcpyDeviceToDevice, st));
   bulk_gemm(P, 'N', 'N', n, n, n, ONE, Pm, n, n2, Km, n, n2, ONE, Hm, n, n2, Cs);
-  HIPCHECK(hipMemcpyAsync(Nm, Dm, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-  if (pair) xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, Hm, n2, a[3], n2, MONE, ONE, Nm, n2, Nm, n2);
-  else bulk_gemm(P, 'N', 'N', n, n, n, MONE, Hm, n, n2, a[3], n, n2, ONE, Nm, n, n2, Cs);
+  if (pair) {
+    xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, Hm, n2, a[3], n2, MONE, ONE, Dm, n2, Nm, n2);
+  } else {
+    HIPCHECK(hipMemcpyAsync(Nm, Dm, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
+    bulk_gemm(P, 'N', 'N', n, n, n, MONE, Hm, n, n2, a[3], n, n2, ONE, Nm, n, n2, Cs);
+  }
   orf_zgetrf(st, n, Nm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[2], n2, u0, sv);
   select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[0], n2, R, nc2);
