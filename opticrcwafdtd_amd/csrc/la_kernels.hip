@@ -2178,7 +2178,6 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
   __shared__ cplx rs[2 * MB_NB * MB_SEG];
   __shared__ cplx sm8[64];
   __shared__ cplx shifts[MB_NB];
-  __shared__ cplx rr8[MB_NB];
   __shared__ int redk[4];
   __shared__ int aedres[2];
   const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
@@ -2436,66 +2435,116 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
           double* rcb = rc + (cs & 1) * MB_NB * MB_SEG;
           cplx* rsb = rs + (cs & 1) * MB_NB * MB_SEG;
           const int p0 = l + cs * MB_SEG;
-          for (int q = lane; q < MB_SW * MB_SW; q += 64) {
-            const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
-            win[q] = (r >= 0 && r < n && c >= 0 && c < n) ? HH(r, c) : cmk(0.0, 0.0);
+          // the window's loads go out in batches of up to 13 per lane, all in flight
+          // together (a loop that stores each element before loading the next waits
+          // out one memory latency per trip)
+          constexpr int WLOADS = (MB_SW * MB_SW + 63) / 64;
+          constexpr int WB = WLOADS < 13 ? WLOADS : 13;
+#pragma unroll
+          for (int u0 = 0; u0 < WLOADS; u0 += WB) {
+            cplx wl[WB];
+#pragma unroll
+            for (int u = 0; u < WB; ++u) {
+              int q = lane + 64 * (u0 + u);
+              asm volatile("" : "+v"(q));  // recomputed per segment, not held in registers
+              const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
+              const bool ok = q < MB_SW * MB_SW && r >= 0 && r < n && c >= 0 && c < n;
+              const cplx x = *(ok ? &HH(r, c) : g_strip_sink);
+              wl[u] = ok ? x : cmk(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < WB; ++u)
+              if (lane + 64 * (u0 + u) < MB_SW * MB_SW) win[lane + 64 * (u0 + u)] = wl[u];
           }
           wave_sync_lds();
           // lockstep: at step j bulge b sits at position p0 - 3b + j; the rotations
           // of one step act on disjoint row and column pairs
           for (int j = 0; j < MB_SEG; ++j) {
-            // A: lane b computes bulge b's rotation
-            if (lane < MB_NB) {
-              const int b = lane, k = p0 - 3 * b + j;
-              const bool valid = b < nbul && k >= l && k <= i - 1;
-              double c = 1.0;
-              cplx sn = cmk(0.0, 0.0), r = cmk(0.0, 0.0);
-              if (valid) {
-                const int kk = k - a0;
-                cplx x, y;
-                if (k == l) { x = csub(WN(kk, kk), shifts[b]); y = WN(kk + 1, kk); }
-                else { x = WN(kk, kk - 1); y = WN(kk + 1, kk - 1); }
-                givens(x, y, c, sn, r);
+            // Lane e (and e + 64) takes bulge b = e / MB_SW at window column (row
+            // ops) or row (column ops) e % MB_SW, and computes b's rotation itself
+            // from the same two LDS words as the bulge's other lanes: no lane waits
+            // for a single computing lane, and the column ops reuse the rotation
+            // from registers.  All loads of a phase are issued before its stores
+            // (the pairs of one step are disjoint).
+            constexpr int EPL = (MB_NB * MB_SW + 63) / 64;
+            bool act[EPL];
+            int kr[EPL];
+            double cr[EPL];
+            cplx sr[EPL];
+            {
+              cplx xv[EPL], yv[EPL], h1v[EPL], h2v[EPL];
+              bool rowop[EPL];
+#pragma unroll
+              for (int u = 0; u < EPL; ++u) {
+                const int e = lane + 64 * u;
+                const int b = e / MB_SW, jc = e % MB_SW;
+                const int k = p0 - 3 * b + j;
+                act[u] = e < MB_NB * MB_SW && b < nbul && k >= l && k <= i - 1;
+                const int kk = act[u] ? k - a0 : 1;
+                kr[u] = kk;
+                if (k == l) {
+                  xv[u] = WN(kk, kk);
+                  yv[u] = WN(kk + 1, kk);
+                } else {
+                  xv[u] = WN(kk, kk - 1);
+                  yv[u] = WN(kk + 1, kk - 1);
+                }
+                rowop[u] = act[u] && jc >= kk && a0 + jc < n;
+                const int jr = rowop[u] ? jc : kk;
+                h1v[u] = WN(kk, jr);
+                h2v[u] = WN(kk + 1, jr);
               }
-              rcb[b * MB_SEG + j] = c;
-              rsb[b * MB_SEG + j] = sn;
-              rr8[b] = r;
+#pragma unroll
+              for (int u = 0; u < EPL; ++u) {
+                const int e = lane + 64 * u;
+                const int b = e / MB_SW, jc = e % MB_SW;
+                const int k = p0 - 3 * b + j;
+                double c = 1.0;
+                cplx sn = cmk(0.0, 0.0), r = cmk(0.0, 0.0);
+                if (act[u]) {
+                  const cplx x = k == l ? csub(xv[u], shifts[b]) : xv[u];
+                  givens(x, yv[u], c, sn, r);
+                }
+                cr[u] = c;
+                sr[u] = sn;
+                const int kk = kr[u];
+                if (rowop[u]) {
+                  WN(kk, jc) = cadd(cscale(h1v[u], c), cmul(sn, h2v[u]));
+                  WN(kk + 1, jc) = csub(cscale(h2v[u], c), cmul(cconj(sn), h1v[u]));
+                }
+                if (jc == 0 && e < MB_NB * MB_SW) {  // one lane per bulge: record and fix
+                  rcb[b * MB_SEG + j] = c;
+                  rsb[b * MB_SEG + j] = sn;
+                  if (act[u] && k > l) {
+                    WN(kk, kk - 1) = r;
+                    WN(kk + 1, kk - 1) = cmk(0.0, 0.0);
+                  }
+                }
+              }
             }
             wave_sync_lds();
-            // B: row ops of every bulge on window columns kk..MB_SW-1
-            for (int e = lane; e < MB_NB * MB_SW; e += 64) {
-              const int b = e / MB_SW, jc = e % MB_SW;
-              const int k = p0 - 3 * b + j;
-              if (!(b < nbul && k >= l && k <= i - 1)) continue;
-              const int kk = k - a0;
-              if (jc < kk || a0 + jc >= n) continue;
-              const double c = rcb[b * MB_SEG + j];
-              const cplx sn = rsb[b * MB_SEG + j];
-              const cplx h1 = WN(kk, jc), h2 = WN(kk + 1, jc);
-              WN(kk, jc) = cadd(cscale(h1, c), cmul(sn, h2));
-              WN(kk + 1, jc) = csub(cscale(h2, c), cmul(cconj(sn), h1));
-            }
-            if (lane < MB_NB) {
-              const int b = lane, k = p0 - 3 * b + j;
-              if (b < nbul && k > l && k <= i - 1) {
-                const int kk = k - a0;
-                WN(kk, kk - 1) = rr8[b];
-                WN(kk + 1, kk - 1) = cmk(0.0, 0.0);
+            // column ops of every bulge on window rows 0..min(kk+2, i-a0)
+            {
+              cplx h1v[EPL], h2v[EPL];
+              bool colop[EPL];
+#pragma unroll
+              for (int u = 0; u < EPL; ++u) {
+                const int q = (lane + 64 * u) % MB_SW;
+                const int kk = kr[u];
+                colop[u] = act[u] && q <= min(kk + 2, i - a0) && a0 + q >= 0;
+                const int qr = colop[u] ? q : kk;
+                h1v[u] = WN(qr, kk);
+                h2v[u] = WN(qr, kk + 1);
               }
-            }
-            wave_sync_lds();
-            // C: column ops of every bulge on window rows 0..min(kk+2, i-a0)
-            for (int e = lane; e < MB_NB * MB_SW; e += 64) {
-              const int b = e / MB_SW, q = e % MB_SW;
-              const int k = p0 - 3 * b + j;
-              if (!(b < nbul && k >= l && k <= i - 1)) continue;
-              const int kk = k - a0;
-              if (q > min(kk + 2, i - a0) || a0 + q < 0) continue;
-              const double c = rcb[b * MB_SEG + j];
-              const cplx sn = rsb[b * MB_SEG + j];
-              const cplx h1 = WN(q, kk), h2 = WN(q, kk + 1);
-              WN(q, kk) = cadd(cscale(h1, c), cmul(cconj(sn), h2));
-              WN(q, kk + 1) = csub(cscale(h2, c), cmul(sn, h1));
+#pragma unroll
+              for (int u = 0; u < EPL; ++u) {
+                const int q = (lane + 64 * u) % MB_SW;
+                const int kk = kr[u];
+                if (colop[u]) {
+                  WN(q, kk) = cadd(cscale(h1v[u], cr[u]), cmul(cconj(sr[u]), h2v[u]));
+                  WN(q, kk + 1) = csub(cscale(h2v[u], cr[u]), cmul(sr[u], h1v[u]));
+                }
+              }
             }
             wave_sync_lds();
           }
