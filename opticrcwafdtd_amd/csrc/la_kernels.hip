@@ -1093,10 +1093,11 @@ __global__ void rowscale_real_kernel(int n, const double* __restrict__ d, cplx* 
 // Schur vectors, Y upper-triangular eigenvectors of the Schur form, nrm the
 // column norms colnorm removed): X = diag(nrm) Y^-1 Z^H, one triangular solve
 // instead of an LU and a full solve.  false (nothing launched) when n is too
-// small for the blocked solve.
+// small for the blocked solve (orf_eig_inverse_ok).
+bool orf_eig_inverse_ok(int n) { return n >= 32; }
 bool orf_eig_inverse(hipStream_t st, int n, const cplx* Z, const cplx* Y, const double* nrm, cplx* X,
                      cplx* dinv, int batch) {
-  if (n < 32 || batch <= 0) return false;
+  if (!orf_eig_inverse_ok(n) || batch <= 0) return false;
   const long long n2 = (long long)n * n;
   for (int b0 = 0; b0 < batch; b0 += 65535) {
     const int bb = std::min(65535, batch - b0);

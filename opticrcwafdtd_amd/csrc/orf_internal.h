@@ -36,6 +36,8 @@ void orf_zgeev(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, c
 void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx* w, cplx* V,
                      int ldv, long long sV, cplx* Zwork, cplx* Ywork, cplx* Hwork, int* info,
                      int batch, const OrfTimeHook& hook, double* vnorm = nullptr);
+// whether orf_eig_inverse handles size n
+bool orf_eig_inverse_ok(int n);
 bool orf_eig_inverse(hipStream_t st, int n, const cplx* Z, const cplx* Y, const double* nrm, cplx* X,
                      cplx* dinv, int batch);
 // Schur QR dispatch knobs: bulges per multishift pass and the size below which the

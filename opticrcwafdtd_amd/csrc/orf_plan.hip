@@ -53,6 +53,9 @@ __global__ void lambda_kernel(int, int, const cplx*, cplx*);
 __global__ void qw_assemble_kernel(int, int, const DevSolve*, const DevPage*, const double*, const cplx*, long long,
                                    const cplx*, long long, cplx*, long long);
 __global__ void colscale_inv_kernel(int, int, const cplx*, cplx*, long long);
+__global__ void rowscale_inv_kernel(int, int, const cplx*, cplx*, long long);
+__global__ void pv0_kernel(int, const DevPage*, const double*, const cplx*, long long, const cplx*, long long, cplx*,
+                           long long);
 __global__ void homog_modes_kernel(int, int, int, const DevSolve*, const DevPage*, const DevSector*, const double*,
                                    const cplx*, cplx*, cplx*, long long, cplx*);
 __global__ void sum_diff_kernel(long long, cplx*, cplx*);
@@ -582,13 +585,26 @@ static void check_info(orf_plan& P, const int* dinfo, int count, const char* wha
 // unsolved: stop at D (w.Dm) and [E1 | E2] (w.RHS2, page stride 2 n^2), with
 // S11 = D^-1 E1 and S12 = D^-1 E2, for finish_single.
 static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx* lam, cplx* S11o, cplx* S12o,
-                          int CsEig, bool unsolved = false) {
+                          int CsEig, bool unsolved = false, const DevPage* eigPages = nullptr) {
   const int n = P.nr;
   const long long n2 = (long long)n * n;
   hipStream_t st = P.st;
   if (CsEig > 0) {
     STAGE("smat_solve");
     if (!orf_eig_inverse(st, n, w.Zw, w.Yw, w.vnorm, w.X1, w.dinv, CsEig)) CsEig = 0;
+  }
+  // Leading pages whose V \ V0 comes from the eigensolver: with V = Q W lam^-1 and
+  // P Q = W lam^2 W^-1, V^-1 = lam^-1 W^-1 P, so V \ V0 = lam^-1 W^-1 (P V0) — one
+  // product instead of an LU and an n-column solve.  Full basis only ([eps]^-1 is
+  // kept in G by the mode stage; eigPages are the mode pages of those solves).
+  const int CsV = (eigPages && !P.reduced) ? CsEig : 0;
+  if (CsV > 0) {
+    STAGE("smat_gemm");
+    const long long M2 = (long long)P.M * P.M;
+    pv0_kernel<<<g1(n2, CsV), 256, 0, st>>>(P.M, eigPages, P.dKxy.as<double>(), w.G, 2 * M2, w.X2, n2, w.R3, n2);
+    orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.X1, n, n2, w.R3, n, n2, ZERO, w.X2, n, n2, CsV);
+    rowscale_inv_kernel<<<g1(n2, CsV), 256, 0, st>>>(n, n, lam, w.X2, n2);
+    KCHECK();
   }
   if (CsEig < Cs) {  // W \ W0 by LU for the rest
     const int Cr = Cs - CsEig;
@@ -607,13 +623,19 @@ static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx
       orf_zgetrs(st, 'L', n, n, Wr, n, n2, ipr, dr, Xr, n, n2, Cr);
     }
   }
-  {
-    STAGE("smat_lu");
-    orf_zgetrf(st, n, Vm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
-  }
-  {
-    STAGE("smat_solve");
-    orf_zgetrs(st, 'L', n, n, Vm, n, n2, w.ipiv, w.dinv, w.X2, n, n2, Cs);  // V \ V0
+  if (CsV < Cs) {  // V \ V0 by LU for the rest
+    const int Cr = Cs - CsV;
+    cplx* Vr = Vm + (size_t)CsV * n2;
+    int* ipr = w.ipiv + (size_t)CsV * n;
+    cplx* dr = w.dinv + (size_t)CsV * orf_dinv_stride(n);
+    {
+      STAGE("smat_lu");
+      orf_zgetrf(st, n, Vr, n, n2, ipr, nullptr, Cr, dr);
+    }
+    {
+      STAGE("smat_solve");
+      orf_zgetrs(st, 'L', n, n, Vr, n, n2, ipr, dr, w.X2 + (size_t)CsV * n2, n, n2, Cr);
+    }
   }
   sum_diff_kernel<<<g1(n2 * Cs), 256, 0, st>>>(n2 * Cs, w.X1, w.X2);  // X1 = A, X2 = B
   smat_rhs3_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, lam, w.kL, w.X1, w.X2, n2, w.R3, 3 * n2, w.Xd);  // [XB XA B]
@@ -1312,7 +1334,9 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
 // Modes of Cp patterned mode pages (pages[m], slot permittivities se/si at
 // m * nslotMax), solve pages w.solves[0 .. Cs): convolution matrices
 // (Laurent / Li), Omega^2 = P Q, eigen-decomposition, W, V = Q W lam^-1 and
-// lam into w.Wm, w.Vm, w.lam.  keep (single-page field solves) receives [eps].
+// lam into w.Wm, w.Vm, w.lam (V only for sector solves, kept modes and sizes
+// the eigensolver's W^-1 does not cover; full-basis solves leave [eps]^-1 in
+// w.G instead).  keep (single-page field solves) receives [eps].
 static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cplx* se, const cplx* si, int Cp, int Cs,
                             orf_plan::Kept* keep) {
   hipStream_t st = P.st;
@@ -1371,6 +1395,10 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
     orf_zgetrs(st, 'L', M, 2 * M, w.E, M, M2, w.ipiv, w.dinv, w.G, M, 2 * M2, Cp);
     omega_assemble_kernel<<<g1(nf2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.G, 2 * M2, w.Exx, w.Eyy,
                                                        M2, w.Omega, nf2);
+    if (!P.reduced) {  // [eps]^-1 into G for the layer S-matrix (V^-1 = lam^-1 W^-1 P)
+      eye_axpby_kernel<<<g1(M2, Cp), 256, 0, st>>>(M, ONE, ZERO, nullptr, 0, w.G, 2 * M2);
+      orf_zgetrs(st, 'L', M, M, w.E, M, M2, w.ipiv, w.dinv, w.G, M, 2 * M2, Cp);
+    }
     KCHECK();
     if (P.reduced) {
       sector_project_kernel<<<g1(n2, Cs), 256, 0, st>>>(nf, nr, w.solves, P.dSectors.as<DevSector>(), 0, 0,
@@ -1397,6 +1425,12 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
       sector_expand_kernel<<<g1(nfr, Cs), 256, 0, st>>>(nf, nr, w.solves, P.dSectors.as<DevSector>(), w.Wm, n2,
                                                          w.Wf, nfr);
       Wf = w.Wf;
+    }
+    // V = Q W / lam; the layer S-matrix of a full-basis solve does without it
+    // (V^-1 = lam^-1 W^-1 P), so it is formed only for sector solves and kept modes
+    if (!P.reduced && !keep && orf_eig_inverse_ok(nr)) {
+      KCHECK();
+      return;
     }
     // T1 = Eyy W2, T2 = Exx W1 per sector
     for (int s = 0; s < nSec; ++s) {
@@ -1525,7 +1559,7 @@ static void execute_chunk_layers(orf_plan& P, Work& w, int p0, int Cp) {
     KCHECK();
   }
   gather_pages_kernel<<<g1(n2, CsAll), 256, 0, st>>>(n2, w.groupOf, P.dV0.as<cplx>(), n2, w.X2, n2);
-  smatrix_layer(P, w, CsAll, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1], CsP);
+  smatrix_layer(P, w, CsAll, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1], CsP, false, w.pagesA);
   // layer order: Sa[0] = S11 (= S22), Sa[1] = S12 (= S21) of page (j, m, s) at j * CpS + m * nSec + s
   gather_pages_kernel<<<g1(n2, CsAll), 256, 0, st>>>(n2, w.idxA, w.Sb[0], n2, w.Sa[0], n2);
   gather_pages_kernel<<<g1(n2, CsAll), 256, 0, st>>>(n2, w.idxA, w.Sb[1], n2, w.Sa[1], n2);
@@ -1657,7 +1691,8 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         HIPCHECK(hipMemcpyAsync(K->lam.p, w.lam, sizeof(cplx) * nr, hipMemcpyDeviceToDevice, st));
       }
       const bool single = P.nLayers == 1 && !P.keepModes;  // finish_single takes D, E1, E2
-      smatrix_layer(P, w, Cs, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1], P.layerAnyPat[j] ? Cs : 0, single);
+      smatrix_layer(P, w, Cs, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1], P.layerAnyPat[j] ? Cs : 0, single,
+                    P.layerAnyPat[j] ? pages : nullptr);
       if (!single) {
         if (P.keepModes) {
           auto* K = P.kept[j];

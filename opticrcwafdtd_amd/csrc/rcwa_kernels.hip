@@ -337,6 +337,45 @@ __global__ void qw_assemble_kernel(int M, int ncol, const DevSolve* __restrict__
   Out[(size_t)sp * sO + e] = v;
 }
 
+// X[r, :] /= lam[r]
+__global__ void rowscale_inv_kernel(int nr, int ncol, const cplx* __restrict__ lam, cplx* __restrict__ X,
+                                    long long sX) {
+  const int sp = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nr * ncol) return;
+  const int r = e % nr;
+  X[(size_t)sp * sX + e] = cdiv(X[(size_t)sp * sX + e], lam[(size_t)sp * nr + r]);
+}
+
+// X = P V0 per page in the full basis, with P = [[Kx e Ky, I - Kx e Kx],
+// [Ky e Ky - I, -Ky e Kx]] (e = [eps]^-1, M x M, page stride sE; pqMatrices.m:2-4)
+// and V0 the V of a uniform gap, whose four blocks are diagonal (row k of
+// column c is nonzero only for k = c mod M): two terms per entry.
+__global__ void pv0_kernel(int M, const DevPage* __restrict__ pages, const double* __restrict__ kxy,
+                           const cplx* __restrict__ Einv, long long sE, const cplx* __restrict__ V0,
+                           long long sV, cplx* __restrict__ X, long long sX) {
+  const int pg = blockIdx.y;
+  const int n = 2 * M;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n) return;
+  const int r = e % n, c = e / n;
+  const int p = r % M, q = c % M;
+  const double* kx = kxy + (size_t)pages[pg].lam * 2 * M;
+  const double* ky = kx + M;
+  const cplx ein = Einv[(size_t)pg * sE + p + (size_t)M * q];
+  const cplx* v0 = V0 + (size_t)pg * sV + (size_t)n * c;
+  const double d = p == q ? 1.0 : 0.0;
+  cplx a, b;  // P[r, q], P[r, q + M]
+  if (r < M) {
+    a = cscale(ein, kx[p] * ky[q]);
+    b = csub(cmk(d, 0.0), cscale(ein, kx[p] * kx[q]));
+  } else {
+    a = csub(cscale(ein, ky[p] * ky[q]), cmk(d, 0.0));
+    b = cscale(ein, -ky[p] * kx[q]);
+  }
+  X[(size_t)pg * sX + e] = cadd(cmul(a, v0[q]), cmul(b, v0[q + M]));
+}
+
 // X[:, c] /= lam[c]
 __global__ void colscale_inv_kernel(int nr, int ncol, const cplx* __restrict__ lam, cplx* __restrict__ X,
                                     long long sX) {
