@@ -34,6 +34,23 @@ ORF_DEV void wave_sync_lds() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// Copy of count elements from global memory into LDS (element e: src(e) into
+// dst(e)) by threads t = 0..nt-1, each thread's loads in batches of 8 that are
+// all in flight together: a plain loop that stores each element before loading
+// the next waits out one memory latency per element.  Indices are clamped into
+// range, so the loads need no branches.
+template <class FS, class FD>
+ORF_DEV void lds_fill(int t, int nt, int count, FS src, FD dst) {
+  for (int e0 = t; e0 < count; e0 += 8 * nt) {
+    cplx v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = src(min(e0 + u * nt, count - 1));
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (e0 + u * nt < count) dst(e0 + u * nt) = v[u];
+  }
+}
+
 // ============================================================== ZGEMM (MFMA)
 namespace {
 constexpr int GT_M = 64, GT_N = 64, GT_K = 16, GT_PAD = 2;
@@ -322,10 +339,8 @@ __global__ __launch_bounds__(256) void zgetrf_kernel(int n, cplx* __restrict__ A
   if (use_lds) {
     A = lds_mat;
     ld = n;
-    for (int e = t; e < n * n; e += nt) {
-      int r = e % n, c = e / n;
-      A[e] = Ag[(size_t)r + (size_t)c * lda];
-    }
+    lds_fill(t, nt, n * n, [&](int e) { return Ag[(size_t)(e % n) + (size_t)(e / n) * lda]; },
+             [&](int e) -> cplx& { return A[e]; });
     __syncthreads();
   }
   int sing = 0;
@@ -392,10 +407,8 @@ __global__ __launch_bounds__(256) void zgetrs_left_kernel(int n, int nrhs, int C
   ipiv += (size_t)page * n;
   Bg += (size_t)page * sB;
   const int nc = min(CB, nrhs - c0);
-  for (int e = t; e < n * nc; e += nt) {
-    int r = e % n, c = e / n;
-    X[r + c * n] = Bg[(size_t)r + (size_t)(c0 + c) * ldb];
-  }
+  lds_fill(t, nt, n * nc, [&](int e) { return Bg[(size_t)(e % n) + (size_t)(c0 + e / n) * ldb]; },
+           [&](int e) -> cplx& { return X[e]; });
   __syncthreads();
   // row interchanges in order (one wave, CB <= 64 lanes)
   if (t < 64) {
@@ -448,10 +461,8 @@ __global__ __launch_bounds__(256) void zgetrs_right_kernel(int n, int nrows, int
   ipiv += (size_t)page * n;
   Bg += (size_t)page * sB;
   const int nr = min(RB, nrows - r0);
-  for (int e = t; e < nr * n; e += nt) {
-    int r = e % nr, c = e / nr;
-    X[r + c * RB] = Bg[(size_t)(r0 + r) + (size_t)c * ldb];
-  }
+  lds_fill(t, nt, nr * n, [&](int e) { return Bg[(size_t)(r0 + e % nr) + (size_t)(e / nr) * ldb]; },
+           [&](int e) -> cplx& { return X[e % nr + (e / nr) * RB]; });
   __syncthreads();
   // Z U = B: column j final after dividing by U[j,j]; then B[:, j'>j] -= Z[:, j] U[j, j']
   for (int j = 0; j < n; ++j) {
@@ -549,10 +560,8 @@ __global__ __launch_bounds__(256) void zgetrf_blocked_kernel(int n, cplx* __rest
   int sing = 0;
   for (int kb = 0; kb < n; kb += LB) {
     const int nb = min(LB, n - kb), m = n - kb;
-    for (int e = t; e < m * nb; e += nt) {
-      const int r = e % m, c = e / m;
-      Pn[r + (size_t)c * n] = A[(size_t)(kb + r) + (size_t)(kb + c) * lda];
-    }
+    lds_fill(t, nt, m * nb, [&](int e) { return A[(size_t)(kb + e % m) + (size_t)(kb + e / m) * lda]; },
+             [&](int e) -> cplx& { return Pn[e % m + (size_t)(e / m) * n]; });
     __syncthreads();
     for (int j = 0; j < nb; ++j) {
       double best = -1.0;
@@ -769,10 +778,8 @@ __global__ __launch_bounds__(256) void ztrsm_left_kernel(int n, int nrhs, int CB
   const cplx* Linv = dinv + (size_t)page * sD;
   const cplx* Uinv = Linv + (size_t)nblk * 256;
   const int nc = min(CB, nrhs - c0);
-  for (int e = t; e < n * nc; e += nt) {
-    const int r = e % n, c = e / n;
-    X[r + c * n] = Bg[(size_t)r + (size_t)(c0 + c) * ldb];
-  }
+  lds_fill(t, nt, n * nc, [&](int e) { return Bg[(size_t)(e % n) + (size_t)(c0 + e / n) * ldb]; },
+           [&](int e) -> cplx& { return X[e]; });
   __syncthreads();
   if (!upper_only && t < 64) {
     ipiv += (size_t)page * n;
@@ -872,10 +879,8 @@ __global__ __launch_bounds__(256) void ztrsm_right_kernel(int n, int nrows, int 
   const cplx* Linv = dinv + (size_t)page * sD;
   const cplx* Uinv = Linv + (size_t)nblk * 256;
   const int nr = min(RB, nrows - r0g);
-  for (int e = t; e < nr * n; e += nt) {
-    const int r = e % nr, c = e / nr;
-    X[r + c * RB] = Bg[(size_t)(r0g + r) + (size_t)c * ldb];
-  }
+  lds_fill(t, nt, nr * n, [&](int e) { return Bg[(size_t)(r0g + e % nr) + (size_t)(e / nr) * ldb]; },
+           [&](int e) -> cplx& { return X[e % nr + (e / nr) * RB]; });
   __syncthreads();
   const int trr = (nr + 15) / 16;
   for (int pass = 0; pass < 2; ++pass) {  // 0: U left to right, 1: L right to left
@@ -1145,7 +1150,8 @@ __global__ __launch_bounds__(256) void hessenberg_noq_kernel(int n, cplx* __rest
   if (use_lds) {
     A = lds_mat;
     ld = n;
-    for (int e = t; e < n * n; e += nt) A[e] = Ag[(size_t)(e % n) + (size_t)(e / n) * lda];
+    lds_fill(t, nt, n * n, [&](int e) { return Ag[(size_t)(e % n) + (size_t)(e / n) * lda]; },
+             [&](int e) -> cplx& { return A[e]; });
   }
   __syncthreads();
   for (int k = 0; k < n - 2; ++k) {
@@ -1391,13 +1397,23 @@ __global__ __launch_bounds__(256) void hessenberg_blocked_kernel(int n, cplx* __
         cplx yr[NB];
 #pragma unroll
         for (int j = 0; j < NB; ++j) yr[j] = j < ib ? YY(r, j) : cmk(0.0, 0.0);
-#pragma unroll 4
-        for (int q = q0; q < n; ++q) {
-          cplx s = cmk(0.0, 0.0);
+        // eight columns' loads in flight before their stores (a store may alias a
+        // later column's load, so a plain loop waits out one latency per column)
+        for (int qb = q0; qb < n; qb += 8) {
+          cplx a8[8];
 #pragma unroll
-          for (int j = 0; j < NB; ++j)
-            if (j < ib) s = cadd(s, cmulc(VS(q, j), yr[j]));
-          AA(r, q) = csub(AA(r, q), s);
+          for (int u = 0; u < 8; ++u) a8[u] = AA(r, min(qb + u, n - 1));
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int q = qb + u;
+            if (q < n) {
+              cplx s = cmk(0.0, 0.0);
+#pragma unroll
+              for (int j = 0; j < NB; ++j)
+                if (j < ib) s = cadd(s, cmulc(VS(q, j), yr[j]));
+              AA(r, q) = csub(a8[u], s);
+            }
+          }
         }
       }
     }
@@ -1615,11 +1631,12 @@ __global__ __launch_bounds__(256) void schur_qr_kernel(int n, cplx* __restrict__
   if (use_lds) {
     H = lds_mat;
     ld = n;
-    for (int e = t; e < n * n; e += nt) H[e] = Hg[(size_t)(e % n) + (size_t)(e / n) * ldh];
+    lds_fill(t, nt, n * n, [&](int e) { return Hg[(size_t)(e % n) + (size_t)(e / n) * ldh]; },
+             [&](int e) -> cplx& { return H[e]; });
   }
   if (zInLds) {
     Z = lds_mat + (size_t)n * n;
-    for (int e = t; e < n * n; e += nt) Z[e] = Zp[e];
+    lds_fill(t, nt, n * n, [&](int e) { return Zp[e]; }, [&](int e) -> cplx& { return Z[e]; });
   }
   __syncthreads();
 #define HH(r, c) H[(size_t)(r) + (size_t)(c) * ld]
@@ -2194,11 +2211,12 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
     H = ldsNext;
     ld = n;
     ldsNext += (size_t)n * n;
-    for (int e = t; e < n * n; e += nt) H[e] = Hg[(size_t)(e % n) + (size_t)(e / n) * ldh];
+    lds_fill(t, nt, n * n, [&](int e) { return Hg[(size_t)(e % n) + (size_t)(e / n) * ldh]; },
+             [&](int e) -> cplx& { return H[e]; });
   }
   if (zInLds) {
     Z = ldsNext;
-    for (int e = t; e < n * n; e += nt) Z[e] = Zp[e];
+    lds_fill(t, nt, n * n, [&](int e) { return Zp[e]; }, [&](int e) -> cplx& { return Z[e]; });
   }
   __syncthreads();
 #define HH(r, c) H[(size_t)(r) + (size_t)(c) * ld]
