@@ -34,6 +34,27 @@ ORF_DEV void wave_sync_lds() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// Workgroups are dealt to the 8 XCDs round robin by linear id (x fastest).  For a
+// grid of T tiles per page and P pages, page and tile of linear id L such that
+// the T tiles of page p all run on XCD p % 8 and share its L2 (pages after the
+// last full group of 8 keep the plain order).
+ORF_DEV void xcd_page_tile(long long L, int T, int P, int& page, int& tile) {
+  const long long full = (long long)(P / 8) * 8 * T;
+  if (L < full) {
+    const long long slot = L >> 3;
+    page = (int)((slot / T) * 8 + (L & 7));
+    tile = (int)(slot % T);
+  } else {
+    const long long r = L - full;
+    page = (P / 8) * 8 + (int)(r / T);
+    tile = (int)(r % T);
+  }
+}
+// 2-D grid (x: tiles of a page, y: pages).
+ORF_DEV void xcd_page_tile2(int& page, int& tile) {
+  xcd_page_tile((long long)blockIdx.x + (long long)gridDim.x * blockIdx.y, gridDim.x, gridDim.y, page, tile);
+}
+
 // Copy of count elements from global memory into LDS (element e: src(e) into
 // dst(e)) by threads t = 0..nt-1, each thread's loads in batches of 8 that are
 // all in flight together: a plain loop that stores each element before loading
@@ -71,11 +92,13 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(
     long long sC) {
   __shared__ double Asr[GT_K][GT_M + GT_PAD], Asi[GT_K][GT_M + GT_PAD];
   __shared__ double Bsr[GT_K][GT_N + GT_PAD], Bsi[GT_K][GT_N + GT_PAD];
-  const int page = blockIdx.z;
+  int page, tile;
+  xcd_page_tile((long long)blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z),
+                gridDim.x * gridDim.y, gridDim.z, page, tile);
   A += (size_t)page * sA;
   B += (size_t)page * sB;
   C += (size_t)page * sC;
-  const int m0 = blockIdx.y * GT_M, n0 = blockIdx.x * GT_N;
+  const int m0 = (tile / gridDim.x) * GT_M, n0 = (tile % gridDim.x) * GT_N;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
   v4d accR[2][2], accI[2][2];
@@ -174,11 +197,13 @@ __global__ __launch_bounds__(256) void zgemm32_mfma_kernel(
   constexpr int PAD = 4;
   __shared__ float Asr[GT_K][GT_M + PAD], Asi[GT_K][GT_M + PAD];
   __shared__ float Bsr[GT_K][GT_N + PAD], Bsi[GT_K][GT_N + PAD];
-  const int page = blockIdx.z;
+  int page, tile;
+  xcd_page_tile((long long)blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z),
+                gridDim.x * gridDim.y, gridDim.z, page, tile);
   A += (size_t)page * sA;
   B += (size_t)page * sB;
   C += (size_t)page * sC;
-  const int m0 = blockIdx.y * GT_M, n0 = blockIdx.x * GT_N;
+  const int m0 = (tile / gridDim.x) * GT_M, n0 = (tile % gridDim.x) * GT_N;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
   v4f accR[2][2], accI[2][2];
@@ -728,7 +753,9 @@ __global__ __launch_bounds__(256) void small_inverse_kernel(int n, const cplx* _
 __global__ __launch_bounds__(256) void diag_inv_kernel(int n, const cplx* __restrict__ LU, int lda,
                                                        long long sA, cplx* __restrict__ dinv, long long sD) {
   __shared__ cplx T[256], X[256];
-  const int blk = blockIdx.x, page = blockIdx.y, t = threadIdx.x;
+  int page, blk;
+  xcd_page_tile2(page, blk);
+  const int t = threadIdx.x;
   const int nblk = (n + 15) / 16, kb = blk * 16, nb = min(16, n - kb);
   LU += (size_t)page * sA;
   cplx* out = dinv + (size_t)page * sD;
@@ -770,7 +797,9 @@ __global__ __launch_bounds__(256) void ztrsm_left_kernel(int n, int nrhs, int CB
                                                          cplx* __restrict__ Bg, int ldb, long long sB,
                                                          int upper_only) {
   extern __shared__ cplx X[];  // n x CB, ld n
-  const int page = blockIdx.y, c0 = blockIdx.x * CB, t = threadIdx.x, nt = blockDim.x;
+  int page, cbk;
+  xcd_page_tile2(page, cbk);
+  const int c0 = cbk * CB, t = threadIdx.x, nt = blockDim.x;
   const int lane = t & 63, wv = t >> 6, nw = nt >> 6;
   const int nblk = (n + 15) / 16;
   LU += (size_t)page * sA;
@@ -870,7 +899,9 @@ __global__ __launch_bounds__(256) void ztrsm_right_kernel(int n, int nrows, int 
                                                           const cplx* __restrict__ dinv, long long sD,
                                                           cplx* __restrict__ Bg, int ldb, long long sB) {
   extern __shared__ cplx X[];  // RB x n, ld RB
-  const int page = blockIdx.y, r0g = blockIdx.x * RB, t = threadIdx.x, nt = blockDim.x;
+  int page, rbk;
+  xcd_page_tile2(page, rbk);
+  const int r0g = rbk * RB, t = threadIdx.x, nt = blockDim.x;
   const int lane = t & 63, wv = t >> 6, nw = nt >> 6;
   const int nblk = (n + 15) / 16;
   LU += (size_t)page * sA;
@@ -2695,7 +2726,9 @@ __global__ __launch_bounds__(256) void trevc_blocked_kernel(int n, const cplx* _
   extern __shared__ cplx tvs[];
   cplx* tcol = tvs;                    // 3 x n: staged columns
   cplx* xjb = tvs + 3 * (size_t)n;     // 2 x TV_EB: x_j of every eigenvector of the block
-  const int page = blockIdx.y, k0 = blockIdx.x * TV_EB, t = threadIdx.x;
+  int page, kbk;
+  xcd_page_tile2(page, kbk);
+  const int k0 = kbk * TV_EB, t = threadIdx.x;
   const int e = t >> 4, g = t & 15;
   const int k = k0 + e;
   const bool act = k < n;
