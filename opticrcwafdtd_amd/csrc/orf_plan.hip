@@ -70,8 +70,8 @@ __global__ void eye_stack_kernel(int, long long, cplx*);
 __global__ void select_cols_kernel(int, int, const int*, const cplx*, long long, cplx*, long long);
 __global__ void xr_pair_kernel(int, const cplx*, long long, const cplx*, long long, cplx, cplx, const cplx*, long long,
                                cplx*, long long);
-__global__ void rx_pair_kernel(int, const cplx*, long long, const cplx*, long long, cplx, cplx, const cplx*, long long,
-                               cplx*, long long);
+__global__ void rx_pair_kernel(int, int, const cplx*, long long, const cplx*, long long, cplx, cplx, const cplx*,
+                               long long, cplx*, long long);
 __global__ void field_amp_kernel(int, int, const cplx*, const cplx*, const cplx*, double, double, const double*, cplx*,
                                  cplx*);
 __global__ void curl_z_kernel(int, int, const double*, const double*, const cplx*, cplx, cplx*);
@@ -776,16 +776,21 @@ static void finish_stack(orf_plan& P, Work& w, int Cs, cplx* const a[4], cplx* c
   HIPCHECK(hipMemcpyAsync(Gm, L[0], sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
   bulk_gemm(P, 'N', 'N', n, n, n, ONE, Pm, n, n2, Km, n, n2, ONE, Gm, n, n2, Cs);
   eye_axpby_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, Nm, n2);
-  if (pair) rx_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, a[3], n2, Gm, n2, MONE, ONE, Nm, n2, Nm, n2);
+  if (pair) rx_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, n, a[3], n2, Gm, n2, MONE, ONE, Nm, n2, Nm, n2);
   else bulk_gemm(P, 'N', 'N', n, n, n, MONE, a[3], n, n2, Gm, n, n2, ONE, Nm, n, n2, Cs);
   orf_zgetrf(st, n, Nm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[2], n2, U, sv);
   select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[0], n2, R, nc2);
   orf_zgetrs(st, 'L', n, ncol, Nm, n, n2, w.ipiv, w.dinv, U, n, sv, Cs);
   orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Gm, n, n2, U, n, sv, ZERO, gv, n, sv, Cs);
-  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, a[1], n, n2, gv, n, sv, ONE, R, n, nc2, Cs);
   orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Km, n, n2, U, n, sv, ZERO, kv, n, sv, Cs);
-  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, b21, n, n2, kv, n, sv, ZERO, T, n, nc2, Cs);
+  if (pair) {
+    rx_pair_kernel<<<g1(nc2, Cs), 256, 0, st>>>(P.M, ncol, a[1], n2, gv, sv, ONE, ONE, R, nc2, R, nc2);
+    rx_pair_kernel<<<g1(nc2, Cs), 256, 0, st>>>(P.M, ncol, b21, n2, kv, sv, ONE, ZERO, nullptr, 0, T, nc2);
+  } else {
+    orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, a[1], n, n2, gv, n, sv, ONE, R, n, nc2, Cs);
+    orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, b21, n, n2, kv, n, sv, ZERO, T, n, nc2, Cs);
+  }
   KCHECK();
 }
 
@@ -840,12 +845,19 @@ static void finish_single(orf_plan& P, Work& w, int Cs, cplx* const a[4], const 
   select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[0], n2, R, nc2);
   orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Hm, n, n2, u0, n, sv, ZERO, gv, n, sv, Cs);
   orf_zgetrs(st, 'L', n, ncol, Nm, n, n2, w.ipiv, w.dinv, gv, n, sv, Cs);
-  HIPCHECK(hipMemcpy2DAsync(c1, sizeof(cplx) * sv, u0, sizeof(cplx) * sv, sizeof(cplx) * nc2, Cs,
-                            hipMemcpyDeviceToDevice, st));
-  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, a[3], n, n2, gv, n, sv, ONE, c1, n, sv, Cs);
-  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, a[1], n, n2, gv, n, sv, ONE, R, n, nc2, Cs);
-  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Km, n, n2, c1, n, sv, ZERO, kv, n, sv, Cs);
-  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, b21, n, n2, kv, n, sv, ZERO, T, n, nc2, Cs);
+  if (pair) {  // products with half-space blocks: two terms per entry
+    rx_pair_kernel<<<g1(nc2, Cs), 256, 0, st>>>(P.M, ncol, a[3], n2, gv, sv, ONE, ONE, u0, sv, c1, sv);
+    rx_pair_kernel<<<g1(nc2, Cs), 256, 0, st>>>(P.M, ncol, a[1], n2, gv, sv, ONE, ONE, R, nc2, R, nc2);
+    orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Km, n, n2, c1, n, sv, ZERO, kv, n, sv, Cs);
+    rx_pair_kernel<<<g1(nc2, Cs), 256, 0, st>>>(P.M, ncol, b21, n2, kv, sv, ONE, ZERO, nullptr, 0, T, nc2);
+  } else {
+    HIPCHECK(hipMemcpy2DAsync(c1, sizeof(cplx) * sv, u0, sizeof(cplx) * sv, sizeof(cplx) * nc2, Cs,
+                              hipMemcpyDeviceToDevice, st));
+    orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, a[3], n, n2, gv, n, sv, ONE, c1, n, sv, Cs);
+    orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, a[1], n, n2, gv, n, sv, ONE, R, n, nc2, Cs);
+    orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Km, n, n2, c1, n, sv, ZERO, kv, n, sv, Cs);
+    orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, b21, n, n2, kv, n, sv, ZERO, T, n, nc2, Cs);
+  }
   KCHECK();
 }
 

@@ -617,12 +617,12 @@ __global__ void xr_pair_kernel(int M, const cplx* __restrict__ X, long long sX, 
   Out[(size_t)blockIdx.y * sO + e] = o;
 }
 
-__global__ void rx_pair_kernel(int M, const cplx* __restrict__ R, long long sR, const cplx* __restrict__ X,
+__global__ void rx_pair_kernel(int M, int ncol, const cplx* __restrict__ R, long long sR, const cplx* __restrict__ X,
                                long long sX, cplx alpha, cplx beta, const cplx* Cin, long long sC, cplx* Out,
                                long long sO) {
   const int n = 2 * M;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long long)n * n) return;
+  if (e >= (long long)n * ncol) return;
   const int i = (int)(e % n), j = (int)(e / n), i1 = i % M, i2 = i1 + M;
   X += (size_t)blockIdx.y * sX;
   R += (size_t)blockIdx.y * sR;
