@@ -655,11 +655,25 @@ __global__ __launch_bounds__(256) void zgetrf_blocked_kernel(int n, cplx* __rest
       }
     }
     __syncthreads();
-    // trailing update on MFMA tiles
+    // trailing update on MFMA tiles; a wave's next tile of A is loaded while it
+    // works on the current one, so the memory latency overlaps the MFMAs
     if (nr > 0) {
-      const int tr = (nr + 15) / 16;
-      for (int tile = wv; tile < tr * tr; tile += nt >> 6) {
+      const int tr = (nr + 15) / 16, ntl = tr * tr, step = nt >> 6;
+      auto load_tile = [&](int tile, cplx* cv) {
+        const int r0 = (tile % tr) * 16, cc = min((tile / tr) * 16 + (lane & 15), nr - 1);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int rr = min(r0 + (lane >> 4) + 4 * v, nr - 1);
+          cv[v] = A[(size_t)(kb + nb + rr) + (size_t)(kb + nb + cc) * lda];
+        }
+      };
+      cplx cv[4];
+      if (wv < ntl) load_tile(wv, cv);
+      for (int tile = wv; tile < ntl; tile += step) {
+        cplx cn[4];
+        load_tile(min(tile + step, ntl - 1), cn);
         const int r0 = (tile % tr) * 16, c0 = (tile / tr) * 16;
+        const int cc = c0 + (lane & 15);
         v4d aR = {0, 0, 0, 0}, aI = {0, 0, 0, 0};
         ztile_mfma<LB / 4>(
                    [&](int r, int k) -> cplx {
@@ -669,15 +683,14 @@ __global__ __launch_bounds__(256) void zgetrf_blocked_kernel(int n, cplx* __rest
                      return (k < nb && c0 + c < nr) ? U[(size_t)k * n + c0 + c] : cmk(0.0, 0.0);
                    },
                    aR, aI);
-        const int cc = c0 + (lane & 15);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int rr = r0 + (lane >> 4) + 4 * v;
-          if (rr < nr && cc < nr) {
-            cplx* p = A + (size_t)(kb + nb + rr) + (size_t)(kb + nb + cc) * lda;
-            *p = csub(*p, cmk(aR[v], aI[v]));
-          }
+          if (rr < nr && cc < nr)
+            A[(size_t)(kb + nb + rr) + (size_t)(kb + nb + cc) * lda] = csub(cv[v], cmk(aR[v], aI[v]));
         }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cv[v] = cn[v];
       }
     }
     __syncthreads();
