@@ -1278,6 +1278,20 @@ __global__ __launch_bounds__(256) void hessenberg_noq_kernel(int n, cplx* __rest
 // reflectors are staged in LDS (dynamic, n x NB) next to the current column.
 // Output layout as hessenberg_noq_kernel: H in A (zeros below the
 // subdiagonal), reflector k in column k of Vg (v[k+1] = 1), tau[k].
+// x - sum_{j < i} Y(r, j) sw[j] (Y column-major with ld n): the loads of Y go
+// out eight at a time, all in flight together.
+ORF_DEV cplx ydot_sub(const cplx* Y, int n, int r, int i, cplx x, const cplx* sw) {
+  for (int j0 = 0; j0 < i; j0 += 8) {
+    cplx y8[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) y8[u] = Y[(size_t)r + (size_t)min(j0 + u, i - 1) * n];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (j0 + u < i) x = csub(x, cmul(y8[u], sw[j0 + u]));
+  }
+  return x;
+}
+
 template <int NB>
 __global__ __launch_bounds__(256) void hessenberg_blocked_kernel(int n, cplx* __restrict__ Ag, int lda,
                                                                  long long sA, cplx* __restrict__ Vg,
@@ -1306,11 +1320,7 @@ __global__ __launch_bounds__(256) void hessenberg_blocked_kernel(int n, cplx* __
       // (a) column c with the panel's earlier reflectors: right (A Q: b -= Y V(c,:)^H), then left (Q^H b)
       if (t < i) sw[t] = cconj(VS(c, t));
       __syncthreads();
-      for (int r = k0 + 1 + t; r < n; r += nt) {
-        cplx x = AA(r, c);
-        for (int j = 0; j < i; ++j) x = csub(x, cmul(YY(r, j), sw[j]));
-        bv[r] = x;
-      }
+      for (int r = k0 + 1 + t; r < n; r += nt) bv[r] = ydot_sub(Y, n, r, i, AA(r, c), sw);
       __syncthreads();
       if (i > 0) {
         for (int j = w; j < i; j += 4) {  // w_j = V(:, j)^H b, a wave per reflector
@@ -1392,8 +1402,7 @@ __global__ __launch_bounds__(256) void hessenberg_blocked_kernel(int n, cplx* __
         }
         for (; q < n; ++q) y8[0] = cadd(y8[0], cmul(AA(r, q), bv[q]));
         cplx y = cadd(cadd(cadd(y8[0], y8[1]), cadd(y8[2], y8[3])), cadd(cadd(y8[4], y8[5]), cadd(y8[6], y8[7])));
-        for (int j = 0; j < i; ++j) y = csub(y, cmul(YY(r, j), sw[j]));
-        YY(r, i) = cmul(tk, y);
+        YY(r, i) = cmul(tk, ydot_sub(Y, n, r, i, y, sw));
       }
       if (t < i) {
         cplx s = cmk(0.0, 0.0);
