@@ -53,14 +53,12 @@ __global__ void lambda_kernel(int, int, const cplx*, cplx*);
 __global__ void qw_assemble_kernel(int, int, const DevSolve*, const DevPage*, const double*, const cplx*, long long,
                                    const cplx*, long long, cplx*, long long);
 __global__ void colscale_inv_kernel(int, int, const cplx*, cplx*, long long);
-__global__ void rowscale_inv_kernel(int, int, const cplx*, cplx*, long long);
 __global__ void pv0_kernel(int, const DevPage*, const double*, const cplx*, long long, const cplx*, long long, cplx*,
                            long long);
 __global__ void homog_modes_kernel(int, int, int, const DevSolve*, const DevPage*, const DevSector*, const double*,
                                    const cplx*, cplx*, cplx*, long long, cplx*);
-__global__ void sum_diff_kernel(long long, cplx*, cplx*);
-__global__ void smat_rhs3_kernel(int, const cplx*, const double*, const cplx*, const cplx*, long long, cplx*,
-                                 long long, cplx*);
+__global__ void smat_abx_kernel(int, const cplx*, const double*, cplx*, cplx*, long long, int, cplx*, cplx*, long long,
+                                cplx*);
 __global__ void rowscale_kernel(int, int, const cplx*, cplx*, long long);
 __global__ void rowscale_add_kernel(int, const cplx*, const cplx*, long long, double, const cplx*, long long, double,
                                     cplx*, long long);
@@ -602,8 +600,7 @@ static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx
     STAGE("smat_gemm");
     const long long M2 = (long long)P.M * P.M;
     pv0_kernel<<<g1(n2, CsV), 256, 0, st>>>(P.M, eigPages, P.dKxy.as<double>(), w.G, 2 * M2, w.X2, n2, w.R3, n2);
-    orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.X1, n, n2, w.R3, n, n2, ZERO, w.X2, n, n2, CsV);
-    rowscale_inv_kernel<<<g1(n2, CsV), 256, 0, st>>>(n, n, lam, w.X2, n2);
+    orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.X1, n, n2, w.R3, n, n2, ZERO, w.X2, n, n2, CsV);  // lam scaling below
     KCHECK();
   }
   if (CsEig < Cs) {  // W \ W0 by LU for the rest
@@ -637,13 +634,12 @@ static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx
       orf_zgetrs(st, 'L', n, n, Vr, n, n2, ipr, dr, w.X2 + (size_t)CsV * n2, n, n2, Cr);
     }
   }
-  sum_diff_kernel<<<g1(n2 * Cs), 256, 0, st>>>(n2 * Cs, w.X1, w.X2);  // X1 = A, X2 = B
-  smat_rhs3_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, lam, w.kL, w.X1, w.X2, n2, w.R3, 3 * n2, w.Xd);  // [XB XA B]
+  // X1 = A, X2 = B, LUb = A, R3 = [XB XA B]
+  smat_abx_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, lam, w.kL, w.X1, w.X2, n2, CsV, w.LUb, w.R3, 3 * n2, w.Xd);
   cplx* XBm = w.R3;
   cplx* XAm = w.R3 + n2;
   cplx* F = w.R3 + 2 * n2;
   // F = B A^-1 (one right solve; X B A^-1 X A = X F X A and B A^-1 B = F B)
-  HIPCHECK(hipMemcpyAsync(w.LUb, w.X1, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
   {
     STAGE("smat_lu");
     orf_zgetrf(st, n, w.LUb, n, n2, w.ipiv, nullptr, Cs, w.dinv);
@@ -807,12 +803,11 @@ static void finish_single(orf_plan& P, Work& w, int Cs, cplx* const a[4], const 
   hipStream_t st = P.st;
   STAGE("star_final");
   const cplx* Dm = w.Dm;
-  const cplx* E1 = w.RHS2;
-  const cplx* E2 = w.RHS2 + n2;  // E1, E2 page stride 2 n2
+  cplx* E1 = w.RHS2;       // E1, then H in place
+  cplx* E2 = w.RHS2 + n2;  // E2, then K in place (E1, E2 page stride 2 n2)
+  const long long s2 = 2 * n2;
   cplx* Nm = w.M1;   // D - E1 b11, then D - H a22
-  cplx* Km = w.Tm;   // K
   cplx* Pm = w.Fm;   // E2 b11
-  cplx* Hm = w.LUb;  // H
   const long long sv = 8LL * n;  // vector page stride in Vv
   cplx* u0 = w.Vv;            // a21 C
   cplx* gv = w.Vv + 2 * n;    // H a21 C, then g
@@ -820,42 +815,41 @@ static void finish_single(orf_plan& P, Work& w, int Cs, cplx* const a[4], const 
   cplx* kv = w.Vv + 6 * n;    // K c1+
   const bool pair = !P.reduced;
   if (pair) {
-    xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, E1, 2 * n2, b11, n2, MONE, ONE, Dm, n2, Nm, n2);
+    xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, E1, s2, b11, n2, MONE, ONE, Dm, n2, Nm, n2);
   } else {
     HIPCHECK(hipMemcpyAsync(Nm, Dm, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-    bulk_gemm(P, 'N', 'N', n, n, n, MONE, E1, n, 2 * n2, b11, n, n2, ONE, Nm, n, n2, Cs);
+    bulk_gemm(P, 'N', 'N', n, n, n, MONE, E1, n, s2, b11, n, n2, ONE, Nm, n, n2, Cs);
   }
   orf_zgetrf(st, n, Nm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
-  HIPCHECK(hipMemcpy2DAsync(Km, sizeof(cplx) * n2, E2, sizeof(cplx) * 2 * n2, sizeof(cplx) * n2, Cs,
-                            hipMemcpyDeviceToDevice, st));
-  orf_zgetrs(st, 'L', n, n, Nm, n, n2, w.ipiv, w.dinv, Km, n, n2, Cs);
-  if (pair) xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, E2, 2 * n2, b11, n2, ONE, ZERO, nullptr, 0, Pm, n2);
-  else bulk_gemm(P, 'N', 'N', n, n, n, ONE, E2, n, 2 * n2, b11, n, n2, ZERO, Pm, n, n2, Cs);
-  HIPCHECK(hipMemcpy2DAsync(Hm, sizeof(cplx) * n2, E1, sizeof(cplx) * 2 * n2, sizeof(cplx) * n2, Cs,
-                            hipMemcpyDeviceToDevice, st));
-  bulk_gemm(P, 'N', 'N', n, n, n, ONE, Pm, n, n2, Km, n, n2, ONE, Hm, n, n2, Cs);
+  // E2 b11 before E2 turns into K
+  if (pair) xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, E2, s2, b11, n2, ONE, ZERO, nullptr, 0, Pm, n2);
+  else bulk_gemm(P, 'N', 'N', n, n, n, ONE, E2, n, s2, b11, n, n2, ZERO, Pm, n, n2, Cs);
+  cplx* Km = E2;
+  orf_zgetrs(st, 'L', n, n, Nm, n, n2, w.ipiv, w.dinv, Km, n, s2, Cs);
+  cplx* Hm = E1;
+  bulk_gemm(P, 'N', 'N', n, n, n, ONE, Pm, n, n2, Km, n, s2, ONE, Hm, n, s2, Cs);
   if (pair) {
-    xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, Hm, n2, a[3], n2, MONE, ONE, Dm, n2, Nm, n2);
+    xr_pair_kernel<<<g1(n2, Cs), 256, 0, st>>>(P.M, Hm, s2, a[3], n2, MONE, ONE, Dm, n2, Nm, n2);
   } else {
     HIPCHECK(hipMemcpyAsync(Nm, Dm, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-    bulk_gemm(P, 'N', 'N', n, n, n, MONE, Hm, n, n2, a[3], n, n2, ONE, Nm, n, n2, Cs);
+    bulk_gemm(P, 'N', 'N', n, n, n, MONE, Hm, n, s2, a[3], n, n2, ONE, Nm, n, n2, Cs);
   }
   orf_zgetrf(st, n, Nm, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[2], n2, u0, sv);
   select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[0], n2, R, nc2);
-  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Hm, n, n2, u0, n, sv, ZERO, gv, n, sv, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Hm, n, s2, u0, n, sv, ZERO, gv, n, sv, Cs);
   orf_zgetrs(st, 'L', n, ncol, Nm, n, n2, w.ipiv, w.dinv, gv, n, sv, Cs);
   if (pair) {  // products with half-space blocks: two terms per entry
     rx_pair_kernel<<<g1(nc2, Cs), 256, 0, st>>>(P.M, ncol, a[3], n2, gv, sv, ONE, ONE, u0, sv, c1, sv);
     rx_pair_kernel<<<g1(nc2, Cs), 256, 0, st>>>(P.M, ncol, a[1], n2, gv, sv, ONE, ONE, R, nc2, R, nc2);
-    orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Km, n, n2, c1, n, sv, ZERO, kv, n, sv, Cs);
+    orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Km, n, s2, c1, n, sv, ZERO, kv, n, sv, Cs);
     rx_pair_kernel<<<g1(nc2, Cs), 256, 0, st>>>(P.M, ncol, b21, n2, kv, sv, ONE, ZERO, nullptr, 0, T, nc2);
   } else {
     HIPCHECK(hipMemcpy2DAsync(c1, sizeof(cplx) * sv, u0, sizeof(cplx) * sv, sizeof(cplx) * nc2, Cs,
                               hipMemcpyDeviceToDevice, st));
     orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, a[3], n, n2, gv, n, sv, ONE, c1, n, sv, Cs);
     orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, a[1], n, n2, gv, n, sv, ONE, R, n, nc2, Cs);
-    orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Km, n, n2, c1, n, sv, ZERO, kv, n, sv, Cs);
+    orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Km, n, s2, c1, n, sv, ZERO, kv, n, sv, Cs);
     orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, b21, n, n2, kv, n, sv, ZERO, T, n, nc2, Cs);
   }
   KCHECK();

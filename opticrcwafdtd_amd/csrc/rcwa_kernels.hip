@@ -337,16 +337,6 @@ __global__ void qw_assemble_kernel(int M, int ncol, const DevSolve* __restrict__
   Out[(size_t)sp * sO + e] = v;
 }
 
-// X[r, :] /= lam[r]
-__global__ void rowscale_inv_kernel(int nr, int ncol, const cplx* __restrict__ lam, cplx* __restrict__ X,
-                                    long long sX) {
-  const int sp = blockIdx.y;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nr * ncol) return;
-  const int r = e % nr;
-  X[(size_t)sp * sX + e] = cdiv(X[(size_t)sp * sX + e], lam[(size_t)sp * nr + r]);
-}
-
 // X = P V0 per page in the full basis, with P = [[Kx e Ky, I - Kx e Kx],
 // [Ky e Ky - I, -Ky e Kx]] (e = [eps]^-1, M x M, page stride sE; pqMatrices.m:2-4)
 // and V0 the V of a uniform gap, whose four blocks are diagonal (row k of
@@ -441,26 +431,27 @@ __global__ void homog_modes_kernel(int M, int nr, int reduced, const DevSolve* _
 }
 
 // ------------------------------------------------------------------ S-matrix helpers
-// A = X1 + X2, B = X1 - X2 (in place: X1 -> A, X2 -> B)
-__global__ void sum_diff_kernel(long long total, cplx* __restrict__ X1, cplx* __restrict__ X2) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= total) return;
-  const cplx a = X1[e], b = X2[e];
-  X1[e] = cadd(a, b);
-  X2[e] = csub(a, b);
-}
-
-// X = exp(-lam kL); R3 = [X.*B, X.*A, B] (n x 3n); also keeps X (n) per page.
-__global__ void smat_rhs3_kernel(int n, const cplx* __restrict__ lam, const double* __restrict__ kL,
-                                 const cplx* __restrict__ A, const cplx* __restrict__ B, long long sM,
-                                 cplx* __restrict__ R3, long long sR, cplx* __restrict__ Xd) {
+// Layer S inputs in one pass: with X1 = W \ W0 and X2 = V \ V0 (rows of the
+// first nScaled pages still to be divided by lam), A = X1 + X2 and B = X1 - X2
+// overwrite X1 and X2, Ac receives a copy of A (for its LU), and with
+// X = exp(-lam kL), R3 = [X.*B, X.*A, B] (n x 3n); Xd keeps X (n) per page.
+__global__ void smat_abx_kernel(int n, const cplx* __restrict__ lam, const double* __restrict__ kL,
+                                cplx* __restrict__ X1, cplx* __restrict__ X2, long long sM, int nScaled,
+                                cplx* __restrict__ Ac, cplx* __restrict__ R3, long long sR, cplx* __restrict__ Xd) {
   const int sp = blockIdx.y;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n * n) return;
   const int r = e % n;
   const cplx l = lam[(size_t)sp * n + r];
+  const size_t o = (size_t)sp * sM + e;
+  const cplx x1 = X1[o];
+  cplx x2 = X2[o];
+  if (sp < nScaled) x2 = cdiv(x2, l);
+  const cplx a = cadd(x1, x2), b = csub(x1, x2);
+  X1[o] = a;
+  X2[o] = b;
+  Ac[o] = a;
   const cplx x = cexp(cscale(l, -kL[sp]));
-  const cplx a = A[(size_t)sp * sM + e], b = B[(size_t)sp * sM + e];
   cplx* R = R3 + (size_t)sp * sR;
   R[e] = cmul(x, b);
   R[e + (size_t)n * n] = cmul(x, a);
