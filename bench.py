@@ -90,7 +90,8 @@ def schur_qr_flops(n):
 
 
 def solve_flops(n, M):
-    """Canonical real flops of one single-patterned-layer solve:
+    """Canonical real flops of one single-patterned-layer solve (the dense-algebra model of the
+    standard RCWA chain, independent of the rearrangements the GPU path uses):
     [eps]^-1 8 M^3 + P Q 8 n^3 + eig 100 n^3 + V 8 n^3 + layer S 90 2/3 n^3
     + 2 star products 85 1/3 n^3 each."""
     return 8.0 * M ** 3 + (8 + 100 + 8 + 90 + 2.0 / 3 + 2 * (85 + 1.0 / 3)) * n ** 3
@@ -216,8 +217,8 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every unit cell is an independent solve, so N GPUs would each run "
-              "their own share of the batch (see DESIGN.md); multi-GPU runs are not part of this build.",
-              file=sys.stderr)
+              "their own share of the batch with no collective on the critical path; only --gpus 1 is "
+              "supported.", file=sys.stderr)
         sys.exit(2)
     cpu = None if args.no_cpu else cpu_baseline(args.symmetry, args.cpu_workers)
     r = run(args, args.symmetry)

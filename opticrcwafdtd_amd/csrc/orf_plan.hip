@@ -2009,7 +2009,7 @@ int orf_init(int device, orf_ctx** out) {
     hipDeviceProp_t prop;
     HIPCHECK(hipGetDeviceProperties(&prop, device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-      fail(ORF_DEVICE_ERROR, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+      fail(ORF_DEVICE_ERROR, std::string("device is ") + prop.gcnArchName + ", the library is compiled for gfx950 only");
     HIPCHECK(hipSetDevice(device));
     orf_ctx* c = new orf_ctx();
     c->device = device;

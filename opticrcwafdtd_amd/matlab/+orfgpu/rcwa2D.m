@@ -96,7 +96,7 @@ for b = 1:B
     end
 end
 
-% sampling: the automatic rule of this build (edge-aligned, NsMin = max(256, 8(4N+1)))
+% sampling: the automatic rule of this library (edge-aligned, NsMin = max(256, 8(4N+1)))
 Ns = opts.Nsample;
 if isscalar(Ns) && Ns == 0
     Ns = [orf.alignedSampleCount(devs, period, 1, max(256, 8 * (4 * opts.Nx + 1))), 1];

@@ -224,7 +224,7 @@ def alignedSampleCount(devs: Sequence[dict], period: Sequence[float], dim: int, 
 def default_sample_counts(devs: Sequence[dict], Nx: int, Ny: int) -> tuple:
     """Automatic [Nsx Nsy] for ``Nsample = 0``.
 
-    The reference's own automatic rule is not published; this build uses
+    The reference's own automatic rule is not published; this package uses
     NsMin = max(256, 8 (4N + 1)) per periodic axis, edge-aligned for
     rectangles, and a single sample along an infinite period.
     """

@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the product path may import this module:
 only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
-use it, and only as the checker / the timed CPU baseline.
+use it, and only as the reference the results are compared with / the timed CPU
+baseline.
 
 The reference (MoonUsagi/OpticRCWAFDTD, MATLAB) ships the solver core as
 P-code, which is never read.  This module restates the algorithm from the
@@ -319,7 +320,7 @@ def aligned_sample_count(devs, period, dim, NsMin):
 
 
 def default_sample_counts(devs, Nx, Ny):
-    """Automatic Nsample of this build: NsMin = max(256, 8(4N+1)), edge-aligned."""
+    """Automatic Nsample of the GPU library: NsMin = max(256, 8(4N+1)), edge-aligned."""
     period = devs[0]["period"]
     out = []
     for dim, N in ((1, Nx), (2, Ny)):
