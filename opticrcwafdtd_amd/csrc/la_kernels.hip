@@ -808,7 +808,10 @@ __global__ __launch_bounds__(256) void ztrsm_left_kernel(int n, int nrhs, int CB
                                                          int lda, long long sA, const int* __restrict__ ipiv,
                                                          const cplx* __restrict__ dinv, long long sD,
                                                          cplx* __restrict__ Bg, int ldb, long long sB,
-                                                         int upper_only) {
+                                                         int upper_only, const cplx* __restrict__ Zt,
+                                                         const double* __restrict__ rsc) {
+  // Zt (optional): B = Zt^H (n x n pages at stride sB) instead of Bg's contents;
+  // rsc (optional): rows of the result scaled by rsc (n per page)
   extern __shared__ cplx X[];  // n x CB, ld n
   int page, cbk;
   xcd_page_tile2(page, cbk);
@@ -820,8 +823,14 @@ __global__ __launch_bounds__(256) void ztrsm_left_kernel(int n, int nrhs, int CB
   const cplx* Linv = dinv + (size_t)page * sD;
   const cplx* Uinv = Linv + (size_t)nblk * 256;
   const int nc = min(CB, nrhs - c0);
-  lds_fill(t, nt, n * nc, [&](int e) { return Bg[(size_t)(e % n) + (size_t)(c0 + e / n) * ldb]; },
-           [&](int e) -> cplx& { return X[e]; });
+  if (Zt) {  // X(r, c) = conj(Zt(c0 + c, r)): consecutive lanes read along a column of Zt
+    const cplx* Zp = Zt + (size_t)page * sB;
+    lds_fill(t, nt, n * nc, [&](int e) { return cconj(Zp[(size_t)(c0 + e % nc) + (size_t)(e / nc) * n]); },
+             [&](int e) -> cplx& { return X[e / nc + (e % nc) * n]; });
+  } else {
+    lds_fill(t, nt, n * nc, [&](int e) { return Bg[(size_t)(e % n) + (size_t)(c0 + e / n) * ldb]; },
+             [&](int e) -> cplx& { return X[e]; });
+  }
   __syncthreads();
   if (!upper_only && t < 64) {
     ipiv += (size_t)page * n;
@@ -899,9 +908,11 @@ __global__ __launch_bounds__(256) void ztrsm_left_kernel(int n, int nrhs, int CB
       __syncthreads();
     }
   }
+  const double* rs = rsc ? rsc + (size_t)page * n : nullptr;
   for (int e = t; e < n * nc; e += nt) {
     const int r = e % n, c = e / n;
-    Bg[(size_t)r + (size_t)(c0 + c) * ldb] = X[r + c * n];
+    const cplx x = X[r + c * n];
+    Bg[(size_t)r + (size_t)(c0 + c) * ldb] = rs ? cscale(x, rs[r]) : x;
   }
 }
 
@@ -1113,30 +1124,11 @@ void orf_zgetrs(hipStream_t st, char side, int n, int nrhs, const cplx* LU, int 
   const long long sD = orf_dinv_stride(n);
   if (side == 'L')
     ztrsm_left_kernel<<<grid, 256, (size_t)n * cb * sizeof(cplx), st>>>(n, nrhs, cb, LU, lda, sA, ipiv,
-                                                                         dinv, sD, B, ldb, sB, 0);
+                                                                         dinv, sD, B, ldb, sB, 0, nullptr, nullptr);
   else
     ztrsm_right_kernel<<<grid, 256, (size_t)n * cb * sizeof(cplx), st>>>(n, nrhs, cb, LU, lda, sA, ipiv,
                                                                           dinv, sD, B, ldb, sB);
 }
-
-namespace {
-// X = Z^H per page (n x n).
-__global__ void conj_transpose_pages_kernel(int n, const cplx* __restrict__ Z, long long sZ,
-                                            cplx* __restrict__ X, long long sX) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long long)n * n) return;
-  const int r = (int)(e % n), c = (int)(e / n);
-  X[(size_t)blockIdx.y * sX + e] = cconj(Z[(size_t)blockIdx.y * sZ + c + (size_t)r * n]);
-}
-
-// X[r, :] *= d[r] per page.
-__global__ void rowscale_real_kernel(int n, const double* __restrict__ d, cplx* __restrict__ X, long long sX) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long long)n * n) return;
-  const int r = (int)(e % n);
-  X[(size_t)blockIdx.y * sX + e] = cscale(X[(size_t)blockIdx.y * sX + e], d[(size_t)blockIdx.y * n + r]);
-}
-}  // namespace
 
 // W^-1 of eigenvector matrices W = Z Y diag(1 / nrm) from orf_zgeev (Z unitary
 // Schur vectors, Y upper-triangular eigenvectors of the Schur form, nrm the
@@ -1154,15 +1146,13 @@ bool orf_eig_inverse(hipStream_t st, int n, const cplx* Z, const cplx* Y, const 
     const cplx* Yb = Y + (size_t)b0 * n2;
     cplx* Xb = X + (size_t)b0 * n2;
     cplx* Db = dinv + (size_t)b0 * orf_dinv_stride(n);
-    const dim3 g((unsigned)((n2 + 255) / 256), bb);
-    conj_transpose_pages_kernel<<<g, 256, 0, st>>>(n, Zb, n2, Xb, n2);
     launch_dinv(st, n, Yb, n, n2, Db, bb);
     int cb = rhs_block(n);
     if (cb > 32) cb = 32;
     if (cb > 16 && (size_t)n * cb * sizeof(cplx) > (size_t)g_lds_limit / 2) cb = 16;
+    // the solve reads Z^H straight from Z and scales its rows by nrm on the way out
     ztrsm_left_kernel<<<dim3((n + cb - 1) / cb, bb), 256, (size_t)n * cb * sizeof(cplx), st>>>(
-        n, n, cb, Yb, n, n2, nullptr, Db, orf_dinv_stride(n), Xb, n, n2, 1);
-    rowscale_real_kernel<<<g, 256, 0, st>>>(n, nrm + (size_t)b0 * n, Xb, n2);
+        n, n, cb, Yb, n, n2, nullptr, Db, orf_dinv_stride(n), Xb, n, n2, 1, Zb, nrm + (size_t)b0 * n);
   }
   return true;
 }
