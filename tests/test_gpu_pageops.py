@@ -100,7 +100,8 @@ def test_pagemldivide_needs_pivoting():
     np.testing.assert_allclose(np.transpose(X, (0, 2, 1))[0], np.linalg.inv(A[0]), atol=1e-13)
 
 
-@pytest.mark.parametrize("n,batch", [(6, 3), (25, 4), (61, 3), (98, 2), (131, 2), (242, 1), (339, 1), (600, 1)])
+@pytest.mark.parametrize("n,batch", [(6, 3), (25, 4), (61, 3), (61, 520), (98, 2), (131, 2), (242, 1), (339, 1),
+                                     (600, 1)])
 def test_pageeig(n, batch):
     rng = np.random.default_rng(4)
     A = _crand(rng, batch, n, n)
