@@ -86,7 +86,7 @@ ORF_DEV cplx load_op(const cplx* __restrict__ X, int ld, int r, int c) {
 }
 
 template <int OPA, int OPB>
-__global__ __launch_bounds__(256) void zgemm_mfma_kernel(
+__global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(
     int M, int N, int K, cplx alpha, const cplx* __restrict__ A, int lda, long long sA,
     const cplx* __restrict__ B, int ldb, long long sB, cplx beta, cplx* __restrict__ C, int ldc,
     long long sC) {
@@ -101,13 +101,16 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(
   const int m0 = (tile / gridDim.x) * GT_M, n0 = (tile % gridDim.x) * GT_N;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
-  v4d accR[2][2], accI[2][2];
+  // three real products per complex one (Gauss): P1 = Ar Br, P2 = Ai Bi,
+  // P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2
+  v4d acc1[2][2], acc2[2][2], acc3[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      accR[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-      accI[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      acc1[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      acc2[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      acc3[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
     }
   // this thread's staging slots: op(A)[ra_r[q], ra_k[q]] and op(B)[rb_k[q], rb_c[q]]
   int ra_r[4], ra_k[4], rb_k[4], rb_c[4];
@@ -155,10 +158,9 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
-          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accR[i][j], 0, 0, 0);
-          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
-          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
+          acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], acc1[i][j], 0, 0, 0);
+          acc2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], acc2[i][j], 0, 0, 0);
+          acc3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i] + ai[i], br[j] + bi[j], acc3[i][j], 0, 0, 0);
         }
     }
     __syncthreads();
@@ -174,7 +176,8 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(
         int gr = m0 + wm + 16 * i + (lane >> 4) + 4 * v;
         int gc = n0 + wn + 16 * j + (lane & 15);
         if (gr < M && gc < N) {
-          cplx acc = cmk(accR[i][j][v], accI[i][j][v]);
+          const double p1 = acc1[i][j][v], p2 = acc2[i][j][v];
+          cplx acc = cmk(p1 - p2, acc3[i][j][v] - p1 - p2);
           cplx out = cmul(alpha, acc);
           size_t idx = (size_t)gr + (size_t)gc * ldc;
           if (!beta0) out = cadd(out, cmul(beta, C[idx]));
