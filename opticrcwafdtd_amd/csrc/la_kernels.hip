@@ -536,6 +536,25 @@ __global__ __launch_bounds__(256) void zgetrs_right_kernel(int n, int nrows, int
 // fa(row, k) and fb(k, col) return operand entries (zero outside the matrix).
 // Lane layout of v_mfma_f64_16x16x4f64: A[l%16][l/16], B[l/16][l%16],
 // C/D rows l/16 + 4v (v = 0..3), column l%16.
+// acc += A B on one 16x16 tile from KS fragments per operand, three real MFMA
+// products per complex one (Gauss): P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi),
+// Re += P1 - P2, Im += P3 - P1 - P2 (three short independent chains).
+template <int KS>
+ORF_DEV void ztile_mfma_frag3(const cplx* av, const cplx* bv, v4d& accR, v4d& accI) {
+  v4d p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].x, bv[s].x, p1, 0, 0, 0);
+    p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].y, bv[s].y, p2, 0, 0, 0);
+    p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].x + av[s].y, bv[s].x + bv[s].y, p3, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    accR[v] += p1[v] - p2[v];
+    accI[v] += p3[v] - p1[v] - p2[v];
+  }
+}
+
 template <int KS, class FA, class FB>
 ORF_DEV void ztile_mfma(FA fa, FB fb, v4d& accR, v4d& accI) {
   const int lane = threadIdx.x & 63;
@@ -546,26 +565,13 @@ ORF_DEV void ztile_mfma(FA fa, FB fb, v4d& accR, v4d& accI) {
     av[s] = fa(r, 4 * s + kq);
     bv[s] = fb(4 * s + kq, r);
   }
-#pragma unroll
-  for (int s = 0; s < KS; ++s) {
-    accR = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].x, bv[s].x, accR, 0, 0, 0);
-    accR = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[s].y, bv[s].y, accR, 0, 0, 0);
-    accI = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].x, bv[s].y, accI, 0, 0, 0);
-    accI = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].y, bv[s].x, accI, 0, 0, 0);
-  }
+  ztile_mfma_frag3<KS>(av, bv, accR, accI);
 }
 
 // The MFMA chain of ztile_mfma<4> on fragments already in registers.
 ORF_DEV void ztile_mfma_frag(const cplx* av, const cplx* bv, v4d& accR, v4d& accI) {
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    accR = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].x, bv[s].x, accR, 0, 0, 0);
-    accR = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[s].y, bv[s].y, accR, 0, 0, 0);
-    accI = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].x, bv[s].y, accI, 0, 0, 0);
-    accI = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].y, bv[s].x, accI, 0, 0, 0);
-  }
+  ztile_mfma_frag3<4>(av, bv, accR, accI);
 }
-
 // Right-looking blocked LU with partial pivoting, one workgroup per page.
 // The LB-wide panel and the LB rows of U12 live in LDS; the trailing update
 // A22 -= L21 U12 runs on MFMA tiles; row interchanges reach the other
