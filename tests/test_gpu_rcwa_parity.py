@@ -153,6 +153,22 @@ def test_batch_chunking_multi_lambda_multi_pol():
             np.testing.assert_allclose(full["T"][..., li, pi], one["T"], atol=1e-12)
 
 
+def test_large_batch_multishift_qr_equals_small_batch_and_oracle():
+    """640 parity-sector solve pages (n = 61) take the multishift QR; chunks of 128 take the
+    per-rotation one.  Both agree with each other and with the oracle at the FP64 bar."""
+    devs = pillars(np.linspace(0.04, 0.18, 160))
+    big = orf.rcwa2D(devs, 0.94, 0, 0, "s", Nx=5, Nsample=(256, 256), Symmetry="auto")
+    small = orf.rcwa2D(devs, 0.94, 0, 0, "s", Nx=5, Nsample=(256, 256), Symmetry="auto", BatchSize=32)
+    assert big["symmetry"] and small["symmetry"]
+    for k in ("R", "T", "Rtot", "Ttot", "r", "t"):
+        np.testing.assert_allclose(big[k], small[k], atol=TOL, rtol=0)
+    sel = [0, 79, 159]
+    o = O.rcwa2d([devs[k] for k in sel], 0.94, 0, 0, "s", Nx=5, Nsample=(256, 256), Symmetry="auto")
+    g = {"m": big["m"], "n": big["n"], "R": big["R"][:, sel], "T": big["T"][:, sel], "Rtot": big["Rtot"][sel],
+         "Ttot": big["Ttot"][sel], "r": big["r"][:, :, sel], "t": big["t"][:, :, sel]}
+    compare(g, o)
+
+
 def test_progress_and_cancel():
     si, air = mc(3.5), mc(1)
     dev = orf.makeStructure([0.5, 0.5], air, mc(1.45),
