@@ -1399,19 +1399,17 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
     omega_rhs_kernel<<<g1(2 * M2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.Exx, w.Eyy, M2, w.G, 2 * M2);
     orf_zgetrf(st, M, w.E, M, M2, w.ipiv, nullptr, Cp, w.dinv);
     orf_zgetrs(st, 'L', M, 2 * M, w.E, M, M2, w.ipiv, w.dinv, w.G, M, 2 * M2, Cp);
+    // the full basis solves Omega^2 itself: assembled straight into the eigensolver's input
     omega_assemble_kernel<<<g1(nf2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.G, 2 * M2, w.Exx, w.Eyy,
-                                                       M2, w.Omega, nf2);
+                                                       M2, P.reduced ? w.Omega : w.A0, nf2);
     if (!P.reduced) {  // [eps]^-1 into G for the layer S-matrix (V^-1 = lam^-1 W^-1 P)
       eye_axpby_kernel<<<g1(M2, Cp), 256, 0, st>>>(M, ONE, ZERO, nullptr, 0, w.G, 2 * M2);
       orf_zgetrs(st, 'L', M, M, w.E, M, M2, w.ipiv, w.dinv, w.G, M, 2 * M2, Cp);
     }
     KCHECK();
-    if (P.reduced) {
+    if (P.reduced)
       sector_project_kernel<<<g1(n2, Cs), 256, 0, st>>>(nf, nr, w.solves, P.dSectors.as<DevSector>(), 0, 0,
                                                          w.Omega, nf2, 0, w.A0, n2);
-    } else {
-      HIPCHECK(hipMemcpyAsync(w.A0, w.Omega, sizeof(cplx) * n2 * Cs, hipMemcpyDeviceToDevice, st));
-    }
     KCHECK();
   }
   {
