@@ -594,12 +594,13 @@ static void smatrix_layer(orf_plan& P, Work& w, int Cs, cplx* Wm, cplx* Vm, cplx
   // Leading pages whose V \ V0 comes from the eigensolver: with V = Q W lam^-1 and
   // P Q = W lam^2 W^-1, V^-1 = lam^-1 W^-1 P, so V \ V0 = lam^-1 W^-1 (P V0) — one
   // product instead of an LU and an n-column solve.  Full basis only ([eps]^-1 is
-  // kept in G by the mode stage; eigPages are the mode pages of those solves).
+  // kept in Omega's pages by the mode stage; eigPages are the mode pages of those solves).
   const int CsV = (eigPages && !P.reduced) ? CsEig : 0;
   if (CsV > 0) {
     STAGE("smat_gemm");
     const long long M2 = (long long)P.M * P.M;
-    pv0_kernel<<<g1(n2, CsV), 256, 0, st>>>(P.M, eigPages, P.dKxy.as<double>(), w.G, 2 * M2, w.X2, n2, w.R3, n2);
+    const long long nf2 = (long long)P.nf * P.nf;
+    pv0_kernel<<<g1(n2, CsV), 256, 0, st>>>(P.M, eigPages, P.dKxy.as<double>(), w.Omega, nf2, w.X2, n2, w.R3, n2);
     orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.X1, n, n2, w.R3, n, n2, ZERO, w.X2, n, n2, CsV);  // lam scaling below
     KCHECK();
   }
@@ -1342,7 +1343,7 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
 // (Laurent / Li), Omega^2 = P Q, eigen-decomposition, W, V = Q W lam^-1 and
 // lam into w.Wm, w.Vm, w.lam (V only for sector solves, kept modes and sizes
 // the eigensolver's W^-1 does not cover; full-basis solves leave [eps]^-1 in
-// w.G instead).  keep (single-page field solves) receives [eps].
+// w.Omega instead).  keep (single-page field solves) receives [eps].
 static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cplx* se, const cplx* si, int Cp, int Cs,
                             orf_plan::Kept* keep) {
   hipStream_t st = P.st;
@@ -1398,14 +1399,20 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
     STAGE("omega");
     omega_rhs_kernel<<<g1(2 * M2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.Exx, w.Eyy, M2, w.G, 2 * M2);
     orf_zgetrf(st, M, w.E, M, M2, w.ipiv, nullptr, Cp, w.dinv);
-    orf_zgetrs(st, 'L', M, 2 * M, w.E, M, M2, w.ipiv, w.dinv, w.G, M, 2 * M2, Cp);
-    // the full basis solves Omega^2 itself: assembled straight into the eigensolver's input
-    omega_assemble_kernel<<<g1(nf2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.G, 2 * M2, w.Exx, w.Eyy,
-                                                       M2, P.reduced ? w.Omega : w.A0, nf2);
-    if (!P.reduced) {  // [eps]^-1 into G for the layer S-matrix (V^-1 = lam^-1 W^-1 P)
-      eye_axpby_kernel<<<g1(M2, Cp), 256, 0, st>>>(M, ONE, ZERO, nullptr, 0, w.G, 2 * M2);
-      orf_zgetrs(st, 'L', M, M, w.E, M, M2, w.ipiv, w.dinv, w.G, M, 2 * M2, Cp);
+    const cplx* Gm = w.G;  // [eps]^-1 [Kx Exx, Ky Eyy]
+    if (P.reduced) {
+      orf_zgetrs(st, 'L', M, 2 * M, w.E, M, M2, w.ipiv, w.dinv, w.G, M, 2 * M2, Cp);
+    } else {
+      // The full basis keeps [eps]^-1 for the layer S-matrix (V^-1 = lam^-1 W^-1 P) in
+      // the first M^2 of each page of Omega (free: Omega^2 goes straight into the
+      // eigensolver's input), and forms G from it by one product in the next 2 M^2.
+      eye_axpby_kernel<<<g1(M2, Cp), 256, 0, st>>>(M, ONE, ZERO, nullptr, 0, w.Omega, nf2);
+      orf_zgetrs(st, 'L', M, M, w.E, M, M2, w.ipiv, w.dinv, w.Omega, M, nf2, Cp);
+      orf_zgemm(st, 'N', 'N', M, 2 * M, M, ONE, w.Omega, M, nf2, w.G, M, 2 * M2, ZERO, w.Omega + M2, M, nf2, Cp);
+      Gm = w.Omega + M2;
     }
+    omega_assemble_kernel<<<g1(nf2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), Gm, P.reduced ? 2 * M2 : nf2,
+                                                       w.Exx, w.Eyy, M2, P.reduced ? w.Omega : w.A0, nf2);
     KCHECK();
     if (P.reduced)
       sector_project_kernel<<<g1(n2, Cs), 256, 0, st>>>(nf, nr, w.solves, P.dSectors.as<DevSector>(), 0, 0,
