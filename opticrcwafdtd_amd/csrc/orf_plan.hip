@@ -59,6 +59,11 @@ __global__ void homog_modes_kernel(int, int, int, const DevSolve*, const DevPage
                                    const cplx*, cplx*, cplx*, long long, cplx*);
 __global__ void smat_abx_kernel(int, const cplx*, const double*, cplx*, cplx*, long long, int, cplx*, cplx*, long long,
                                 cplx*);
+__global__ void direct_blocks_kernel(int, const DevSolve*, const DevPage*, const double*, const cplx*, const cplx*,
+                                     const cplx*, const double*, const cplx*, const cplx*, const cplx*, long long, cplx*,
+                                     cplx*, cplx*, cplx*, cplx*);
+__global__ void direct_vec_kernel(int, int, const cplx*, const double*, const cplx*, const cplx*, cplx*, cplx*,
+                                  long long);
 __global__ void rowscale_kernel(int, int, const cplx*, cplx*, long long);
 __global__ void rowscale_add_kernel(int, const cplx*, const cplx*, long long, double, const cplx*, long long, double,
                                     cplx*, long long);
@@ -410,6 +415,9 @@ struct orf_plan {
     bool patterned = false;
   };
   bool keepModes = false;
+  // one patterned layer in the full basis: solved on its mode amplitudes
+  // (finish_direct) without forming the layer S-matrix
+  bool directSingle = false;
   std::vector<Kept*> kept;
   // layer-parallel execution (several layers): every layer's modes and S-matrix in one
   // batch, then the star products as a tree; host copies of the per-layer tables
@@ -853,6 +861,52 @@ static void finish_single(orf_plan& P, Work& w, int Cs, cplx* const a[4], const 
     orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, Km, n, s2, c1, n, sv, ZERO, kv, n, sv, Cs);
     orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, b21, n, n2, kv, n, sv, ZERO, T, n, nc2, Cs);
   }
+  KCHECK();
+}
+
+// One patterned layer, full basis: R and T on the source columns straight from
+// the layer's modes (W in w.Wm, T12 = [Eyy W2; Exx W1] in w.T12, lam, kL) and the
+// gap's V0 (w.X2), without the layer S-matrix (smatrixLayer.m:6-18 and the two
+// star products of rcwa2D.m give the same R, T).  direct_blocks_kernel states
+// the system [A1 B1; B2 A2] [h+; h-] = [a21 C; 0]; its Schur complement on h+:
+//   K = A2^-1 B2,  (A1 - B1 K) h+ = a21 C,  h- = -K h+,
+// then c1- = W S1 + U D1, c2+ = W S2 + U D2 (direct_vec_kernel), R = a11 C + a12 c1-,
+// T = b21 c2+.  Two n x n LUs, one n-column solve and one product.
+static void finish_direct(orf_plan& P, Work& w, int Cs, const DevPage* pages, cplx* const a[4], const cplx* b11,
+                          const cplx* b21, int ncol, const int* colIdx, cplx* R, cplx* T) {
+  const int n = P.nr, M = P.M;
+  const long long n2 = (long long)n * n, nc2 = (long long)n * ncol;
+  hipStream_t st = P.st;
+  STAGE("star_final");
+  cplx* U = w.XB;
+  cplx* A1 = w.Dm;   // then the Schur complement A1 - B1 K and its LU
+  cplx* B1 = w.LUb;
+  cplx* B2 = w.Fm;   // then K
+  cplx* A2 = w.M1;   // then its LU
+  const int nb = (n + 7) / 8;  // kDirectCols columns per thread
+  direct_blocks_kernel<<<g1((long long)M * nb, Cs), 256, 0, st>>>(
+      M, w.solves, pages, P.dKxy.as<double>(), w.Wm, w.T12, w.lam, w.kL, w.X2,
+      a[3], b11, n2, U, A1, B1, B2, A2);
+  KCHECK();
+  orf_zgetrf(st, n, A2, n, n2, w.ipiv, nullptr, Cs, w.dinv);
+  orf_zgetrs(st, 'L', n, n, A2, n, n2, w.ipiv, w.dinv, B2, n, n2, Cs);
+  orf_zgemm(st, 'N', 'N', n, n, n, MONE, B1, n, n2, B2, n, n2, ONE, A1, n, n2, Cs);
+  orf_zgetrf(st, n, A1, n, n2, w.ipiv, nullptr, Cs, w.dinv);
+  // vectors in w.Tm (page stride n2 >= 16 n): h+, h-, S (2 ncol), D (2 ncol), Y (2 ncol)
+  cplx* hp = w.Tm;
+  cplx* hm = w.Tm + nc2;
+  cplx* Sv = w.Tm + 2 * nc2;
+  cplx* Dv = w.Tm + 4 * nc2;
+  cplx* Yv = w.Tm + 6 * nc2;
+  select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[2], n2, hp, n2);
+  orf_zgetrs(st, 'L', n, ncol, A1, n, n2, w.ipiv, w.dinv, hp, n, n2, Cs);
+  orf_zgemm(st, 'N', 'N', n, ncol, n, MONE, B2, n, n2, hp, n, n2, ZERO, hm, n, n2, Cs);
+  direct_vec_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, w.lam, w.kL, hp, hm, Sv, Dv, n2);
+  orf_zgemm(st, 'N', 'N', n, 2 * ncol, n, ONE, w.Wm, n, n2, Sv, n, n2, ZERO, Yv, n, n2, Cs);
+  orf_zgemm(st, 'N', 'N', n, 2 * ncol, n, ONE, U, n, n2, Dv, n, n2, ONE, Yv, n, n2, Cs);
+  select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[0], n2, R, nc2);
+  rx_pair_kernel<<<g1(nc2, Cs), 256, 0, st>>>(M, ncol, a[1], n2, Yv, n2, ONE, ONE, R, nc2, R, nc2);
+  rx_pair_kernel<<<g1(nc2, Cs), 256, 0, st>>>(M, ncol, b21, n2, Yv + nc2, n2, ONE, ZERO, nullptr, 0, T, nc2);
   KCHECK();
 }
 
@@ -1314,6 +1368,7 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   int cp = (int)std::max<size_t>(1, budget / std::max<size_t>(perMode, 1));
   cp = std::min(cp, 32768 / P.nSec);  // grid-dimension limit of the page-batched launches
   P.layerParallel = P.nLayers >= 2 && !P.keepModes;
+  P.directSingle = P.nLayers == 1 && !P.keepModes && !P.reduced && P.layerAnyPat[0] && P.nr >= 16;
   if (P.layerParallel) {
     // every layer of a page is in the chunk at once: cap the chunk at kLayerChunkSolves
     // solve pages (enough to fill the device) so that its workspace stays moderate
@@ -1439,7 +1494,7 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
     }
     // V = Q W / lam; the layer S-matrix of a full-basis solve does without it
     // (V^-1 = lam^-1 W^-1 P), so it is formed only for sector solves and kept modes
-    if (!P.reduced && !keep && orf_eig_inverse_ok(nr)) {
+    if (!P.reduced && !keep && !P.directSingle && orf_eig_inverse_ok(nr)) {
       KCHECK();
       return;
     }
@@ -1449,6 +1504,10 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
                 w.T12 + s * nfr, nf, nSec * nfr, Cp);
       bulk_gemm(P, 'N', 'N', M, nr, M, ONE, w.Exx, M, M2, Wf + s * nfr, nf, nSec * nfr, ZERO,
                 w.T12 + s * nfr + M, nf, nSec * nfr, Cp);
+    }
+    if (P.directSingle) {  // finish_direct forms Q W itself from W and T12
+      KCHECK();
+      return;
     }
     qw_assemble_kernel<<<g1(nfr, Cs), 256, 0, st>>>(M, nr, w.solves, pages, P.dKxy.as<double>(), Wf, nfr, w.T12,
                                                     nfr, w.QW, nfr);
@@ -1488,7 +1547,9 @@ static void finish_chunk(orf_plan& P, Work& w, int p0, int Cp, cplx* const SA[4]
     }
     HIPCHECK(hipMemcpyAsync(w.colIdx, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, st));
   }
-  if (single) finish_single(P, w, Cs, SA, SB[0], SB[2], ncol, w.colIdx, R, T);
+  if (single && P.directSingle)
+    finish_direct(P, w, Cs, P.dPagesByLayer[0]->as<DevPage>() + p0, SA, SB[0], SB[2], ncol, w.colIdx, R, T);
+  else if (single) finish_single(P, w, Cs, SA, SB[0], SB[2], ncol, w.colIdx, R, T);
   else if (L) finish_stack(P, w, Cs, SA, L, SB[0], SB[2], ncol, w.colIdx, R, T);
   else smatrix_star_cols(P, w, Cs, SA, SB, ncol, w.colIdx, R, T);
   {
@@ -1702,8 +1763,9 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         HIPCHECK(hipMemcpyAsync(K->lam.p, w.lam, sizeof(cplx) * nr, hipMemcpyDeviceToDevice, st));
       }
       const bool single = P.nLayers == 1 && !P.keepModes;  // finish_single takes D, E1, E2
-      smatrix_layer(P, w, Cs, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1], P.layerAnyPat[j] ? Cs : 0, single,
-                    P.layerAnyPat[j] ? pages : nullptr);
+      if (!P.directSingle)  // finish_direct needs no layer S-matrix
+        smatrix_layer(P, w, Cs, w.Wm, w.Vm, w.lam, w.Sb[0], w.Sb[1], P.layerAnyPat[j] ? Cs : 0, single,
+                      P.layerAnyPat[j] ? pages : nullptr);
       if (!single) {
         if (P.keepModes) {
           auto* K = P.kept[j];

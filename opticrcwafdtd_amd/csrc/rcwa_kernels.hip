@@ -459,6 +459,90 @@ __global__ void smat_abx_kernel(int n, const cplx* __restrict__ lam, const doubl
   if (e < n) Xd[(size_t)sp * n + e] = cexp(cscale(lam[(size_t)sp * n + e], -kL[sp]));
 }
 
+// One patterned layer between the two half-spaces, full basis, solved as one
+// linear system on the layer's own mode amplitudes instead of through its
+// S-matrix.  Layer fields [W W; -V V] (V = Q W / lam), gap fields [I I; -V0 V0]
+// at both faces, X = exp(-lam kL).  With U = V0^-1 V, Wp = W + U, Wm = W - U the
+// gap amplitudes at the faces are
+//   c1+ = Wp h+ + Wm X h-,  c1- = Wm h+ + Wp X h-   (z = 0)
+//   c2+ = Wp X h+ + Wm h-,  c2- = Wm X h+ + Wp h-   (z = d)
+// for mode amplitudes h = c / 2, and c1+ = a21 C + a22 c1-, c2- = b11 c2+ give
+//   [A1 B1; B2 A2] [h+; h-] = [a21 C; 0],
+//   A1 = Wp - a22 Wm, B1 = (Wm - a22 Wp) X, B2 = (Wm - b11 Wp) X, A2 = Wp - b11 Wm.
+// This kernel writes U and the four blocks.  V0, a22 and b11 couple harmonic p
+// only with p + M (2 x 2 per harmonic); Q W comes from W and
+// T12 = [Eyy W2; Exx W1] (pqMatrices.m:5-7).  One thread per harmonic p and
+// group of kDirectCols columns: rows p and p + M of every output.
+constexpr int kDirectCols = 8;
+__global__ void direct_blocks_kernel(int M, const DevSolve* __restrict__ solves, const DevPage* __restrict__ pages,
+                                     const double* __restrict__ kxy, const cplx* __restrict__ W,
+                                     const cplx* __restrict__ T12, const cplx* __restrict__ lam,
+                                     const double* __restrict__ kL, const cplx* __restrict__ V0,
+                                     const cplx* __restrict__ a22, const cplx* __restrict__ b11, long long s,
+                                     cplx* __restrict__ U, cplx* __restrict__ A1, cplx* __restrict__ B1,
+                                     cplx* __restrict__ B2, cplx* __restrict__ A2) {
+  const int sp = blockIdx.y;
+  const int n = 2 * M, nb = (n + kDirectCols - 1) / kDirectCols;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M * nb) return;
+  const int p = e % M, q = p + M, j0 = (e / M) * kDirectCols, j1 = min(j0 + kDirectCols, n);
+  const size_t o = (size_t)sp * s;
+  const double* kx = kxy + (size_t)pages[solves[sp].mode].lam * 2 * M;
+  const double kxp = kx[p], kyp = kx[M + p];
+  // 2 x 2 blocks (p, p), (p, q), (q, p), (q, q)
+  const size_t ipp = o + p + (size_t)n * p, ipq = o + p + (size_t)n * q, iqp = o + q + (size_t)n * p,
+               iqq = o + q + (size_t)n * q;
+  const cplx a0 = a22[ipp], a1 = a22[ipq], a2 = a22[iqp], a3 = a22[iqq];
+  const cplx b0 = b11[ipp], b1 = b11[ipq], b2 = b11[iqp], b3 = b11[iqq];
+  const cplx v0 = V0[ipp], v1 = V0[ipq], v2 = V0[iqp], v3 = V0[iqq];
+  const cplx idet = crecip(csub(cmul(v0, v3), cmul(v1, v2)));
+  const cplx g0 = cmul(v3, idet), g1 = cneg(cmul(v1, idet)), g2 = cneg(cmul(v2, idet)), g3 = cmul(v0, idet);
+  const double kl = kL[sp];
+  for (int j = j0; j < j1; ++j) {
+    const size_t c = o + (size_t)n * j;
+    const cplx w1 = W[c + p], w2 = W[c + q];
+    const cplx q1 = csub(cadd(cscale(w1, kxp * kyp), T12[c + p]), cscale(w2, kxp * kxp));
+    const cplx q2 = csub(csub(cscale(w1, kyp * kyp), T12[c + q]), cscale(w2, kyp * kxp));
+    const cplx l = lam[(size_t)sp * n + j];
+    const cplx il = crecip(l);
+    const cplx u1 = cmul(cadd(cmul(g0, q1), cmul(g1, q2)), il);
+    const cplx u2 = cmul(cadd(cmul(g2, q1), cmul(g3, q2)), il);
+    const cplx p1 = cadd(w1, u1), p2 = cadd(w2, u2), m1 = csub(w1, u1), m2 = csub(w2, u2);
+    const cplx x = cexp(cscale(l, -kl));
+    U[c + p] = u1;
+    U[c + q] = u2;
+    A1[c + p] = csub(p1, cadd(cmul(a0, m1), cmul(a1, m2)));
+    A1[c + q] = csub(p2, cadd(cmul(a2, m1), cmul(a3, m2)));
+    B1[c + p] = cmul(x, csub(m1, cadd(cmul(a0, p1), cmul(a1, p2))));
+    B1[c + q] = cmul(x, csub(m2, cadd(cmul(a2, p1), cmul(a3, p2))));
+    B2[c + p] = cmul(x, csub(m1, cadd(cmul(b0, p1), cmul(b1, p2))));
+    B2[c + q] = cmul(x, csub(m2, cadd(cmul(b2, p1), cmul(b3, p2))));
+    A2[c + p] = csub(p1, cadd(cmul(b0, m1), cmul(b1, m2)));
+    A2[c + q] = csub(p2, cadd(cmul(b2, m1), cmul(b3, m2)));
+  }
+}
+
+// Right-hand sides of the face amplitudes from the mode amplitudes h+ (Hp) and
+// h- (Hm), n x ncol each: c1- = W S1 + U D1 and c2+ = W S2 + U D2 with
+// S = [h+ + X h-, X h+ + h-] and D = [X h- - h+, X h+ - h-] (n x 2 ncol each).
+__global__ void direct_vec_kernel(int n, int ncol, const cplx* __restrict__ lam, const double* __restrict__ kL,
+                                  const cplx* __restrict__ Hp, const cplx* __restrict__ Hm, cplx* __restrict__ S,
+                                  cplx* __restrict__ D, long long s) {
+  const int sp = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * ncol) return;
+  const int i = e % n;
+  const size_t o = (size_t)sp * s;
+  const cplx x = cexp(cscale(lam[(size_t)sp * n + i], -kL[sp]));
+  const cplx hp = Hp[o + e], hm = Hm[o + e];
+  const cplx xp = cmul(x, hp), xm = cmul(x, hm);
+  const size_t e2 = e + (size_t)n * ncol;
+  S[o + e] = cadd(hp, xm);
+  S[o + e2] = cadd(xp, hm);
+  D[o + e] = csub(xm, hp);
+  D[o + e2] = csub(xp, hm);
+}
+
 // out = s1 diag(X) P + s2 C  (n x n per page; out may alias P or C)
 __global__ void rowscale_add_kernel(int n, const cplx* __restrict__ Xd, const cplx* P, long long sP, double s1,
                                     const cplx* C, long long sC, double s2, cplx* out, long long sO) {
