@@ -60,8 +60,11 @@ __global__ void homog_modes_kernel(int, int, int, const DevSolve*, const DevPage
 __global__ void smat_abx_kernel(int, const cplx*, const double*, cplx*, cplx*, long long, int, cplx*, cplx*, long long,
                                 cplx*);
 __global__ void direct_blocks_kernel(int, const DevSolve*, const DevPage*, const double*, const cplx*, const cplx*,
-                                     const cplx*, const double*, const cplx*, const cplx*, const cplx*, long long, cplx*,
-                                     cplx*, cplx*, cplx*, cplx*);
+                                     const cplx*, const double*, const int*, const cplx*, const cplx*, const cplx*,
+                                     long long, cplx*, cplx*, cplx*, cplx*, cplx*);
+__global__ void direct_rhs_kernel(int, int, const int*, const int*, const cplx*, cplx*, long long);
+__global__ void direct_out_kernel(int, int, const int*, const int*, const cplx*, const cplx*, const cplx*, const cplx*,
+                                  long long, cplx*, cplx*);
 __global__ void direct_vec_kernel(int, int, const cplx*, const double*, const cplx*, const cplx*, cplx*, cplx*,
                                   long long);
 __global__ void rowscale_kernel(int, int, const cplx*, cplx*, long long);
@@ -872,10 +875,15 @@ static void finish_single(orf_plan& P, Work& w, int Cs, cplx* const a[4], const 
 //   K = A2^-1 B2,  (A1 - B1 K) h+ = a21 C,  h- = -K h+,
 // then c1- = W S1 + U D1, c2+ = W S2 + U D2 (direct_vec_kernel), R = a11 C + a12 c1-,
 // T = b21 c2+.  Two n x n LUs, one n-column solve and one product.
-static void finish_direct(orf_plan& P, Work& w, int Cs, const DevPage* pages, cplx* const a[4], const cplx* b11,
-                          const cplx* b21, int ncol, const int* colIdx, cplx* R, cplx* T) {
+static void finish_direct(orf_plan& P, Work& w, int Cs, const DevPage* pages, int ncol, const int* colIdx, cplx* R,
+                          cplx* T) {
   const int n = P.nr, M = P.M;
-  const long long n2 = (long long)n * n, nc2 = (long long)n * ncol;
+  const long long n2 = (long long)n * n, nc2 = (long long)n * ncol, G = (long long)P.nCtx * P.nSec;
+  // half-space blocks and the gap's V0 per context (page w.groupOf[sp])
+  const cplx* a = P.dSref.as<cplx>();
+  const cplx* b = P.dStrn.as<cplx>();
+  const cplx *a11 = a, *a12 = a + n2 * G, *a21 = a + 2 * n2 * G, *a22 = a + 3 * n2 * G;
+  const cplx *b11 = b, *b21 = b + 2 * n2 * G;
   hipStream_t st = P.st;
   STAGE("star_final");
   cplx* U = w.XB;
@@ -885,8 +893,8 @@ static void finish_direct(orf_plan& P, Work& w, int Cs, const DevPage* pages, cp
   cplx* A2 = w.M1;   // then its LU
   const int nb = (n + 7) / 8;  // kDirectCols columns per thread
   direct_blocks_kernel<<<g1((long long)M * nb, Cs), 256, 0, st>>>(
-      M, w.solves, pages, P.dKxy.as<double>(), w.Wm, w.T12, w.lam, w.kL, w.X2,
-      a[3], b11, n2, U, A1, B1, B2, A2);
+      M, w.solves, pages, P.dKxy.as<double>(), w.Wm, w.T12, w.lam, w.kL, w.groupOf, P.dV0.as<cplx>(), a22, b11, n2,
+      U, A1, B1, B2, A2);
   KCHECK();
   orf_zgetrf(st, n, A2, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   orf_zgetrs(st, 'L', n, n, A2, n, n2, w.ipiv, w.dinv, B2, n, n2, Cs);
@@ -898,15 +906,13 @@ static void finish_direct(orf_plan& P, Work& w, int Cs, const DevPage* pages, cp
   cplx* Sv = w.Tm + 2 * nc2;
   cplx* Dv = w.Tm + 4 * nc2;
   cplx* Yv = w.Tm + 6 * nc2;
-  select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[2], n2, hp, n2);
+  direct_rhs_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, w.groupOf, a21, hp, n2);
   orf_zgetrs(st, 'L', n, ncol, A1, n, n2, w.ipiv, w.dinv, hp, n, n2, Cs);
   orf_zgemm(st, 'N', 'N', n, ncol, n, MONE, B2, n, n2, hp, n, n2, ZERO, hm, n, n2, Cs);
   direct_vec_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, w.lam, w.kL, hp, hm, Sv, Dv, n2);
   orf_zgemm(st, 'N', 'N', n, 2 * ncol, n, ONE, w.Wm, n, n2, Sv, n, n2, ZERO, Yv, n, n2, Cs);
   orf_zgemm(st, 'N', 'N', n, 2 * ncol, n, ONE, U, n, n2, Dv, n, n2, ONE, Yv, n, n2, Cs);
-  select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, a[0], n2, R, nc2);
-  rx_pair_kernel<<<g1(nc2, Cs), 256, 0, st>>>(M, ncol, a[1], n2, Yv, n2, ONE, ONE, R, nc2, R, nc2);
-  rx_pair_kernel<<<g1(nc2, Cs), 256, 0, st>>>(M, ncol, b21, n2, Yv + nc2, n2, ONE, ZERO, nullptr, 0, T, nc2);
+  direct_out_kernel<<<g1(nc2, Cs), 256, 0, st>>>(M, ncol, colIdx, w.groupOf, a11, a12, b21, Yv, n2, R, T);
   KCHECK();
 }
 
@@ -1533,9 +1539,10 @@ static void finish_chunk(orf_plan& P, Work& w, int p0, int Cp, cplx* const SA[4]
   hipStream_t st = P.st;
   const int nr = P.nr, nSec = P.nSec, Cs = Cp * nSec;
   const long long n2 = (long long)nr * nr, G = (long long)P.nCtx * nSec;
-  for (int k = 0; k < 4; k += 2)
+  const bool direct = single && P.directSingle;  // reads the per-context tables itself
+  for (int k = 0; k < 4 && !direct; k += 2)
     gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dStrn.as<cplx>() + k * n2 * G, n2, SB[k], n2);
-  if (L || single)
+  if ((L || single) && !direct)
     for (int k = 0; k < 4; ++k)
       gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dSref.as<cplx>() + k * n2 * G, n2, SA[k], n2);
   const int ncol = P.reduced ? 1 : 2;
@@ -1547,8 +1554,7 @@ static void finish_chunk(orf_plan& P, Work& w, int p0, int Cp, cplx* const SA[4]
     }
     HIPCHECK(hipMemcpyAsync(w.colIdx, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, st));
   }
-  if (single && P.directSingle)
-    finish_direct(P, w, Cs, P.dPagesByLayer[0]->as<DevPage>() + p0, SA, SB[0], SB[2], ncol, w.colIdx, R, T);
+  if (direct) finish_direct(P, w, Cs, P.dPagesByLayer[0]->as<DevPage>() + p0, ncol, w.colIdx, R, T);
   else if (single) finish_single(P, w, Cs, SA, SB[0], SB[2], ncol, w.colIdx, R, T);
   else if (L) finish_stack(P, w, Cs, SA, L, SB[0], SB[2], ncol, w.colIdx, R, T);
   else smatrix_star_cols(P, w, Cs, SA, SB, ncol, w.colIdx, R, T);
@@ -1752,7 +1758,8 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         kL[sp] = P.k0[P.pageCtx[pg]] * P.thick[(size_t)j * P.B + P.pageB[pg]];
       }
       HIPCHECK(hipMemcpyAsync(w.kL, kL.data(), sizeof(double) * Cs, hipMemcpyHostToDevice, st));
-      gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dV0.as<cplx>(), n2, w.X2, n2);
+      if (!P.directSingle)
+        gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dV0.as<cplx>(), n2, w.X2, n2);
       if (P.keepModes) {
         auto* K = P.kept[j];
         K->W.alloc(sizeof(cplx) * n2);

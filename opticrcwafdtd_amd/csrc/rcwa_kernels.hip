@@ -470,17 +470,19 @@ __global__ void smat_abx_kernel(int n, const cplx* __restrict__ lam, const doubl
 //   [A1 B1; B2 A2] [h+; h-] = [a21 C; 0],
 //   A1 = Wp - a22 Wm, B1 = (Wm - a22 Wp) X, B2 = (Wm - b11 Wp) X, A2 = Wp - b11 Wm.
 // This kernel writes U and the four blocks.  V0, a22 and b11 couple harmonic p
-// only with p + M (2 x 2 per harmonic); Q W comes from W and
-// T12 = [Eyy W2; Exx W1] (pqMatrices.m:5-7).  One thread per harmonic p and
-// group of kDirectCols columns: rows p and p + M of every output.
+// only with p + M (2 x 2 per harmonic) and are read from the per-context tables
+// (page grp[sp], stride n^2); Q W comes from W and T12 = [Eyy W2; Exx W1]
+// (pqMatrices.m:5-7).  One thread per harmonic p and group of kDirectCols
+// columns: rows p and p + M of every output.
 constexpr int kDirectCols = 8;
 __global__ void direct_blocks_kernel(int M, const DevSolve* __restrict__ solves, const DevPage* __restrict__ pages,
                                      const double* __restrict__ kxy, const cplx* __restrict__ W,
                                      const cplx* __restrict__ T12, const cplx* __restrict__ lam,
-                                     const double* __restrict__ kL, const cplx* __restrict__ V0,
-                                     const cplx* __restrict__ a22, const cplx* __restrict__ b11, long long s,
-                                     cplx* __restrict__ U, cplx* __restrict__ A1, cplx* __restrict__ B1,
-                                     cplx* __restrict__ B2, cplx* __restrict__ A2) {
+                                     const double* __restrict__ kL, const int* __restrict__ grp,
+                                     const cplx* __restrict__ V0, const cplx* __restrict__ a22,
+                                     const cplx* __restrict__ b11, long long s, cplx* __restrict__ U,
+                                     cplx* __restrict__ A1, cplx* __restrict__ B1, cplx* __restrict__ B2,
+                                     cplx* __restrict__ A2) {
   const int sp = blockIdx.y;
   const int n = 2 * M, nb = (n + kDirectCols - 1) / kDirectCols;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -489,9 +491,10 @@ __global__ void direct_blocks_kernel(int M, const DevSolve* __restrict__ solves,
   const size_t o = (size_t)sp * s;
   const double* kx = kxy + (size_t)pages[solves[sp].mode].lam * 2 * M;
   const double kxp = kx[p], kyp = kx[M + p];
-  // 2 x 2 blocks (p, p), (p, q), (q, p), (q, q)
-  const size_t ipp = o + p + (size_t)n * p, ipq = o + p + (size_t)n * q, iqp = o + q + (size_t)n * p,
-               iqq = o + q + (size_t)n * q;
+  // 2 x 2 blocks (p, p), (p, q), (q, p), (q, q) of the context's tables
+  const size_t og = (size_t)grp[sp] * n * n;
+  const size_t ipp = og + p + (size_t)n * p, ipq = og + p + (size_t)n * q, iqp = og + q + (size_t)n * p,
+               iqq = og + q + (size_t)n * q;
   const cplx a0 = a22[ipp], a1 = a22[ipq], a2 = a22[iqp], a3 = a22[iqq];
   const cplx b0 = b11[ipp], b1 = b11[ipq], b2 = b11[iqp], b3 = b11[iqq];
   const cplx v0 = V0[ipp], v1 = V0[ipq], v2 = V0[iqp], v3 = V0[iqq];
@@ -541,6 +544,38 @@ __global__ void direct_vec_kernel(int n, int ncol, const cplx* __restrict__ lam,
   S[o + e2] = cadd(xp, hm);
   D[o + e] = csub(xm, hp);
   D[o + e2] = csub(xp, hm);
+}
+
+// h+ right-hand side a21 C: columns colIdx of the context's a21 (page grp[sp]).
+__global__ void direct_rhs_kernel(int n, int ncol, const int* __restrict__ colIdx, const int* __restrict__ grp,
+                                  const cplx* __restrict__ a21, cplx* __restrict__ out, long long sO) {
+  const int sp = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * ncol) return;
+  const int r = e % n, q = e / n;
+  out[(size_t)sp * sO + e] = a21[(size_t)grp[sp] * n * n + r + (size_t)n * colIdx[sp * ncol + q]];
+}
+
+// R = a11 C + a12 c1-, T = b21 c2+ with Y = [c1- c2+] (n x 2 ncol, page stride sY);
+// a11, a12, b21 per context (page grp[sp]), pair structure as in rx_pair_kernel.
+__global__ void direct_out_kernel(int M, int ncol, const int* __restrict__ colIdx, const int* __restrict__ grp,
+                                  const cplx* __restrict__ a11, const cplx* __restrict__ a12,
+                                  const cplx* __restrict__ b21, const cplx* __restrict__ Y, long long sY,
+                                  cplx* __restrict__ R, cplx* __restrict__ T) {
+  const int n = 2 * M;
+  const int sp = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * ncol) return;
+  const int i = e % n, j = e / n, i1 = i % M, i2 = i1 + M;
+  const size_t og = (size_t)grp[sp] * n * n;
+  const cplx* y = Y + (size_t)sp * sY;
+  const cplx* c1 = y + (size_t)n * j;
+  const cplx* c2 = y + (size_t)n * (ncol + j);
+  const cplx r = cadd(a11[og + i + (size_t)n * colIdx[sp * ncol + j]],
+                      cadd(cmul(a12[og + i + (size_t)n * i1], c1[i1]), cmul(a12[og + i + (size_t)n * i2], c1[i2])));
+  const cplx t = cadd(cmul(b21[og + i + (size_t)n * i1], c2[i1]), cmul(b21[og + i + (size_t)n * i2], c2[i2]));
+  R[(size_t)sp * n * ncol + e] = r;
+  T[(size_t)sp * n * ncol + e] = t;
 }
 
 // out = s1 diag(X) P + s2 C  (n x n per page; out may alias P or C)
