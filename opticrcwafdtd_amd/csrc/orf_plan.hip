@@ -1405,8 +1405,19 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
 // lam into w.Wm, w.Vm, w.lam (V only for sector solves, kept modes and sizes
 // the eigensolver's W^-1 does not cover; full-basis solves leave [eps]^-1 in
 // w.Omega instead).  keep (single-page field solves) receives [eps].
+// Pages of a chunk that share a label grid and slot permittivities (the same
+// structure at wavelengths where its materials do not disperse) share [eps],
+// Exx and Eyy: the convolution stage runs once per distinct pair (Cu pages,
+// tables uPages / uSe / uSi) and uMap[m] names the distinct page of mode page m.
+struct ConvUnique {
+  int Cu = 0;
+  const DevPage* uPages = nullptr;
+  const cplx *uSe = nullptr, *uSi = nullptr;
+  const int* uMap = nullptr;
+};
+
 static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cplx* se, const cplx* si, int Cp, int Cs,
-                            orf_plan::Kept* keep) {
+                            orf_plan::Kept* keep, const ConvUnique* cu = nullptr) {
   hipStream_t st = P.st;
   const int M = P.M, nf = P.nf, nr = P.nr, nSec = P.nSec;
   const long long M2 = (long long)M * M, nf2 = (long long)nf * nf, n2 = (long long)nr * nr;
@@ -1414,20 +1425,25 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
   const int Wx = 2 * P.Nx + 1, Wy = 2 * P.Ny + 1, Hx = 4 * P.Nx + 1, Hy = 4 * P.Ny + 1;
   {
     STAGE("conv");
-    laurent_kernel<<<g1(M2, Cp), 256, 0, st>>>(P.Nx, P.Ny, M, pages, P.dChi2.as<cplx>(), se, P.nslotMax, w.E, M2);
-    if (keep) {
-      keep->E.alloc(sizeof(cplx) * M2);
-      keep->patterned = true;
-      HIPCHECK(hipMemcpyAsync(keep->E.p, w.E, sizeof(cplx) * M2, hipMemcpyDeviceToDevice, st));
-    }
+    // with distinct pages: [eps] into G and Exx, Eyy into Omega (free until the
+    // next stage), then copied out to every mode page
+    const bool uq = cu && cu->Cu < Cp;
+    const int Cc = uq ? cu->Cu : Cp;
+    const DevPage* cpages = uq ? cu->uPages : pages;
+    const cplx* cse = uq ? cu->uSe : se;
+    const cplx* csi = uq ? cu->uSi : si;
+    cplx* cE = uq ? w.G : w.E;
+    cplx* cExx = uq ? w.Omega : w.Exx;
+    cplx* cEyy = uq ? w.Omega + (size_t)Cc * M2 : w.Eyy;
+    laurent_kernel<<<g1(M2, Cc), 256, 0, st>>>(P.Nx, P.Ny, M, cpages, P.dChi2.as<cplx>(), cse, P.nslotMax, cE, M2);
     if (P.factorization == ORF_FACT_LI) {
       // Exx: inverse rule along x at each y-sample, Laurent along y
       const long long perx = (long long)Wx * Wx * P.Nsy;
-      li_toeplitz_kernel<<<g1(perx, Cp), 256, 0, st>>>(0, P.Nx, P.Nsy, Hx, pages, P.dChiX.as<cplx>(),
-                                                        (long long)P.nslotMax * Hx * P.Nsy, si, P.nslotMax, w.Toe);
+      li_toeplitz_kernel<<<g1(perx, Cc), 256, 0, st>>>(0, P.Nx, P.Nsy, Hx, cpages, P.dChiX.as<cplx>(),
+                                                        (long long)P.nslotMax * Hx * P.Nsy, csi, P.nslotMax, w.Toe);
       KCHECK();
       // invert every Toeplitz block: LU then solve against the identity (in Yw scratch)
-      const int nb = Cp * P.Nsy;
+      const int nb = Cc * P.Nsy;
       cplx* inv = w.Inv;
       if (!orf_small_inverse(st, Wx, w.Toe, (long long)Wx * Wx, inv, (long long)Wx * Wx, nb)) {
         eye_stack_kernel<<<g1((long long)Wx * Wx * nb), 256, 0, st>>>(Wx, (long long)nb, inv);
@@ -1436,23 +1452,33 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
       }
       // G = Fy * [A_j]^T : (Hy x Nsy) * (Nsy x Wx^2)
       bulk_gemm(P, 'N', 'T', Hy, Wx * Wx, P.Nsy, ONE, P.dFy.as<cplx>(), Hy, 0, inv, Wx * Wx,
-                (long long)Wx * Wx * P.Nsy, ZERO, w.GH, Hy, (long long)Hy * Wx * Wx, Cp);
+                (long long)Wx * Wx * P.Nsy, ZERO, w.GH, Hy, (long long)Hy * Wx * Wx, Cc);
       // Eyy: inverse rule along y at each x-sample, Laurent along x
       const long long pery = (long long)Wy * Wy * P.Nsx;
-      li_toeplitz_kernel<<<g1(pery, Cp), 256, 0, st>>>(1, P.Ny, P.Nsx, Hy, pages, P.dChiY.as<cplx>(),
-                                                        (long long)P.nslotMax * Hy * P.Nsx, si, P.nslotMax, w.Toe);
-      const int nb2 = Cp * P.Nsx;
+      li_toeplitz_kernel<<<g1(pery, Cc), 256, 0, st>>>(1, P.Ny, P.Nsx, Hy, cpages, P.dChiY.as<cplx>(),
+                                                        (long long)P.nslotMax * Hy * P.Nsx, csi, P.nslotMax, w.Toe);
+      const int nb2 = Cc * P.Nsx;
       if (!orf_small_inverse(st, Wy, w.Toe, (long long)Wy * Wy, inv, (long long)Wy * Wy, nb2)) {
         eye_stack_kernel<<<g1((long long)Wy * Wy * nb2), 256, 0, st>>>(Wy, (long long)nb2, inv);
         orf_zgetrf(st, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, nb2, nullptr);
         orf_zgetrs(st, 'L', Wy, Wy, w.Toe, Wy, (long long)Wy * Wy, w.ipivT, nullptr, inv, Wy, (long long)Wy * Wy, nb2);
       }
       bulk_gemm(P, 'N', 'T', Hx, Wy * Wy, P.Nsx, ONE, P.dFx.as<cplx>(), Hx, 0, inv, Wy * Wy,
-                (long long)Wy * Wy * P.Nsx, ZERO, w.HH, Hx, (long long)Hx * Wy * Wy, Cp);
-      li_gather_kernel<<<g1(M2, Cp), 256, 0, st>>>(P.Nx, P.Ny, M, w.GH, w.HH, w.Exx, w.Eyy, M2);
+                (long long)Wy * Wy * P.Nsx, ZERO, w.HH, Hx, (long long)Hx * Wy * Wy, Cc);
+      li_gather_kernel<<<g1(M2, Cc), 256, 0, st>>>(P.Nx, P.Ny, M, w.GH, w.HH, cExx, cEyy, M2);
     } else {
-      HIPCHECK(hipMemcpyAsync(w.Exx, w.E, sizeof(cplx) * M2 * Cp, hipMemcpyDeviceToDevice, st));
-      HIPCHECK(hipMemcpyAsync(w.Eyy, w.E, sizeof(cplx) * M2 * Cp, hipMemcpyDeviceToDevice, st));
+      HIPCHECK(hipMemcpyAsync(cExx, cE, sizeof(cplx) * M2 * Cc, hipMemcpyDeviceToDevice, st));
+      HIPCHECK(hipMemcpyAsync(cEyy, cE, sizeof(cplx) * M2 * Cc, hipMemcpyDeviceToDevice, st));
+    }
+    if (uq) {
+      gather_pages_kernel<<<g1(M2, Cp), 256, 0, st>>>(M2, cu->uMap, cE, M2, w.E, M2);
+      gather_pages_kernel<<<g1(M2, Cp), 256, 0, st>>>(M2, cu->uMap, cExx, M2, w.Exx, M2);
+      gather_pages_kernel<<<g1(M2, Cp), 256, 0, st>>>(M2, cu->uMap, cEyy, M2, w.Eyy, M2);
+    }
+    if (keep) {
+      keep->E.alloc(sizeof(cplx) * M2);
+      keep->patterned = true;
+      HIPCHECK(hipMemcpyAsync(keep->E.p, w.E, sizeof(cplx) * M2, hipMemcpyDeviceToDevice, st));
     }
     KCHECK();
   }
@@ -1740,7 +1766,46 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
       if (P.layerAnyPat[j]) {
         const cplx* se = P.dSlotEps[j]->as<cplx>() + (size_t)p0 * P.nslotMax;
         const cplx* si = P.dSlotInv[j]->as<cplx>() + (size_t)p0 * P.nslotMax;
-        patterned_modes(P, w, pages, se, si, Cp, Cs, P.keepModes ? P.kept[j] : nullptr);
+        // distinct (label grid, slot permittivities) pairs of the chunk, first occurrence order
+        const int ns = P.nslotMax;
+        std::vector<DevPage> uPg;
+        std::vector<cplx> uSe, uSi;
+        std::vector<int> uMap(Cp);
+        {
+          std::map<std::vector<double>, int> seen;
+          for (int m = 0; m < Cp; ++m) {
+            const DevPage& pg = P.hPages[j][p0 + m];
+            std::vector<double> key{(double)pg.grid, (double)pg.nslot};
+            for (int q = 0; q < pg.nslot; ++q) {
+              const cplx e = P.hSe[j][(size_t)(p0 + m) * ns + q];
+              key.push_back(e.x);
+              key.push_back(e.y);
+            }
+            auto it = seen.find(key);
+            if (it == seen.end()) {
+              it = seen.emplace(key, (int)uPg.size()).first;
+              uPg.push_back(pg);
+              for (int q = 0; q < ns; ++q) {
+                uSe.push_back(P.hSe[j][(size_t)(p0 + m) * ns + q]);
+                uSi.push_back(P.hSi[j][(size_t)(p0 + m) * ns + q]);
+              }
+            }
+            uMap[m] = it->second;
+          }
+        }
+        ConvUnique cu;
+        cu.Cu = (int)uPg.size();
+        if (cu.Cu < Cp) {
+          HIPCHECK(hipMemcpyAsync(w.pagesA, uPg.data(), sizeof(DevPage) * uPg.size(), hipMemcpyHostToDevice, st));
+          HIPCHECK(hipMemcpyAsync(w.seA, uSe.data(), sizeof(cplx) * uSe.size(), hipMemcpyHostToDevice, st));
+          HIPCHECK(hipMemcpyAsync(w.siA, uSi.data(), sizeof(cplx) * uSi.size(), hipMemcpyHostToDevice, st));
+          HIPCHECK(hipMemcpyAsync(w.idxA, uMap.data(), sizeof(int) * Cp, hipMemcpyHostToDevice, st));
+          cu.uPages = w.pagesA;
+          cu.uSe = w.seA;
+          cu.uSi = w.siA;
+          cu.uMap = w.idxA;
+        }
+        patterned_modes(P, w, pages, se, si, Cp, Cs, P.keepModes ? P.kept[j] : nullptr, &cu);
       } else {
         STAGE("modes");
         std::vector<cplx> eps(Cs);
