@@ -1423,12 +1423,12 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
   const long long M2 = (long long)M * M, nf2 = (long long)nf * nf, n2 = (long long)nr * nr;
   const long long nfr = (long long)nf * nr;
   const int Wx = 2 * P.Nx + 1, Wy = 2 * P.Ny + 1, Hx = 4 * P.Nx + 1, Hy = 4 * P.Ny + 1;
+  // with distinct pages: [eps] into G and Exx, Eyy into Omega (free until the
+  // omega stage), Exx and Eyy then copied out to every mode page
+  const bool uq = cu && cu->Cu < Cp;
+  const int Cc = uq ? cu->Cu : Cp;
   {
     STAGE("conv");
-    // with distinct pages: [eps] into G and Exx, Eyy into Omega (free until the
-    // next stage), then copied out to every mode page
-    const bool uq = cu && cu->Cu < Cp;
-    const int Cc = uq ? cu->Cu : Cp;
     const DevPage* cpages = uq ? cu->uPages : pages;
     const cplx* cse = uq ? cu->uSe : se;
     const cplx* csi = uq ? cu->uSi : si;
@@ -1470,8 +1470,7 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
       HIPCHECK(hipMemcpyAsync(cExx, cE, sizeof(cplx) * M2 * Cc, hipMemcpyDeviceToDevice, st));
       HIPCHECK(hipMemcpyAsync(cEyy, cE, sizeof(cplx) * M2 * Cc, hipMemcpyDeviceToDevice, st));
     }
-    if (uq) {
-      gather_pages_kernel<<<g1(M2, Cp), 256, 0, st>>>(M2, cu->uMap, cE, M2, w.E, M2);
+    if (uq) {  // [eps] stays compact: the omega stage inverts it there
       gather_pages_kernel<<<g1(M2, Cp), 256, 0, st>>>(M2, cu->uMap, cExx, M2, w.Exx, M2);
       gather_pages_kernel<<<g1(M2, Cp), 256, 0, st>>>(M2, cu->uMap, cEyy, M2, w.Eyy, M2);
     }
@@ -1484,22 +1483,29 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
   }
   {
     STAGE("omega");
-    omega_rhs_kernel<<<g1(2 * M2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.Exx, w.Eyy, M2, w.G, 2 * M2);
-    orf_zgetrf(st, M, w.E, M, M2, w.ipiv, nullptr, Cp, w.dinv);
-    const cplx* Gm = w.G;  // [eps]^-1 [Kx Exx, Ky Eyy]
-    if (P.reduced) {
-      orf_zgetrs(st, 'L', M, 2 * M, w.E, M, M2, w.ipiv, w.dinv, w.G, M, 2 * M2, Cp);
+    // [eps]^-1 explicitly, once per distinct page, into the first M^2 of each page
+    // of Omega (the full basis's layer S-matrix uses it: V^-1 = lam^-1 W^-1 P), then
+    // G = [eps]^-1 [Kx Exx, Ky Eyy] by one product
+    if (uq) {  // compact [eps] in G; its compact inverse in E (not expanded, so free)
+      orf_zgetrf(st, M, w.G, M, M2, w.ipiv, nullptr, Cc, w.dinv);
+      eye_axpby_kernel<<<g1(M2, Cc), 256, 0, st>>>(M, ONE, ZERO, nullptr, 0, w.E, M2);
+      orf_zgetrs(st, 'L', M, M, w.G, M, M2, w.ipiv, w.dinv, w.E, M, M2, Cc);
+      gather_pages_kernel<<<g1(M2, Cp), 256, 0, st>>>(M2, cu->uMap, w.E, M2, w.Omega, nf2);
     } else {
-      // The full basis keeps [eps]^-1 for the layer S-matrix (V^-1 = lam^-1 W^-1 P) in
-      // the first M^2 of each page of Omega (free: Omega^2 goes straight into the
-      // eigensolver's input), and forms G from it by one product in the next 2 M^2.
+      orf_zgetrf(st, M, w.E, M, M2, w.ipiv, nullptr, Cp, w.dinv);
       eye_axpby_kernel<<<g1(M2, Cp), 256, 0, st>>>(M, ONE, ZERO, nullptr, 0, w.Omega, nf2);
       orf_zgetrs(st, 'L', M, M, w.E, M, M2, w.ipiv, w.dinv, w.Omega, M, nf2, Cp);
-      orf_zgemm(st, 'N', 'N', M, 2 * M, M, ONE, w.Omega, M, nf2, w.G, M, 2 * M2, ZERO, w.Omega + M2, M, nf2, Cp);
-      Gm = w.Omega + M2;
     }
-    omega_assemble_kernel<<<g1(nf2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), Gm, P.reduced ? 2 * M2 : nf2,
-                                                       w.Exx, w.Eyy, M2, P.reduced ? w.Omega : w.A0, nf2);
+    // full basis: right-hand side [Kx Exx, Ky Eyy] in G, G beside [eps]^-1 in Omega
+    // (Omega^2 goes straight into A0); sector bases assemble Omega^2 in Omega and
+    // project it, so there the right-hand side sits beside [eps]^-1 and G in G
+    cplx* rhs = P.reduced ? w.Omega + M2 : w.G;
+    cplx* Gm = P.reduced ? w.G : w.Omega + M2;
+    const long long sR = P.reduced ? nf2 : 2 * M2, sG = P.reduced ? 2 * M2 : nf2;
+    omega_rhs_kernel<<<g1(2 * M2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), w.Exx, w.Eyy, M2, rhs, sR);
+    orf_zgemm(st, 'N', 'N', M, 2 * M, M, ONE, w.Omega, M, nf2, rhs, M, sR, ZERO, Gm, M, sG, Cp);
+    omega_assemble_kernel<<<g1(nf2, Cp), 256, 0, st>>>(M, pages, P.dKxy.as<double>(), Gm, sG, w.Exx, w.Eyy, M2,
+                                                       P.reduced ? w.Omega : w.A0, nf2);
     KCHECK();
     if (P.reduced)
       sector_project_kernel<<<g1(n2, Cs), 256, 0, st>>>(nf, nr, w.solves, P.dSectors.as<DevSector>(), 0, 0,
