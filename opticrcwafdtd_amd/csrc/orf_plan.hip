@@ -63,6 +63,9 @@ __global__ void direct_blocks_kernel(int, const DevSolve*, const DevPage*, const
                                      const cplx*, const double*, const int*, const cplx*, const cplx*, const cplx*,
                                      long long, cplx*, cplx*, cplx*, cplx*, cplx*);
 __global__ void direct_rhs_kernel(int, int, const int*, const int*, const cplx*, cplx*, long long);
+__global__ void wpm_kernel(int, const cplx*, const cplx*, long long, cplx*, long long);
+__global__ void dense_blocks_kernel(int, const cplx*, const double*, const cplx*, const cplx*, const cplx*,
+                                    const cplx*, long long, cplx*, cplx*, cplx*, cplx*);
 __global__ void direct_out_kernel(int, int, const int*, const int*, const cplx*, const cplx*, const cplx*, const cplx*,
                                   long long, cplx*, cplx*);
 __global__ void direct_vec_kernel(int, int, const cplx*, const double*, const cplx*, const cplx*, cplx*, cplx*,
@@ -403,6 +406,7 @@ struct orf_plan {
   std::vector<int> sectorSrcCol;               // reduced basis: source column per sector
   // shared per (context, sector) region data, nr x nr each
   DBuf dV0, dSref, dStrn;                      // V0 [nCtx*nSec], S [4][nCtx*nSec]
+  DBuf dV0inv;                                 // V0^-1 [nCtx*nSec] (sector bases)
   // chunking
   int chunkPages = 0;
   // results (device)
@@ -418,8 +422,8 @@ struct orf_plan {
     bool patterned = false;
   };
   bool keepModes = false;
-  // one patterned layer in the full basis: solved on its mode amplitudes
-  // (finish_direct) without forming the layer S-matrix
+  // one patterned layer: solved on its mode amplitudes (finish_direct)
+  // without forming the layer S-matrix
   bool directSingle = false;
   std::vector<Kept*> kept;
   // layer-parallel execution (several layers): every layer's modes and S-matrix in one
@@ -555,7 +559,7 @@ static Work make_work(orf_plan& P, int Cp) {
 static size_t plan_bytes(const orf_plan& P) {
   size_t b = 0;
   for (const DBuf* d : {&P.dShapes, &P.dVerts, &P.dGrids, &P.dLabels, &P.dFx, &P.dFy, &P.dChiX, &P.dChiY, &P.dChi2,
-                        &P.dKxy, &P.dKz, &P.dKzi0, &P.dSectorData, &P.dSectors, &P.dSources, &P.dV0, &P.dSref,
+                        &P.dKxy, &P.dKz, &P.dKzi0, &P.dSectorData, &P.dSectors, &P.dSources, &P.dV0, &P.dV0inv, &P.dSref,
                         &P.dStrn, &P.dR, &P.dT, &P.dRtot, &P.dTtot, &P.dr, &P.dt})
     b += d->bytes;
   for (auto* v : {&P.dPagesByLayer, &P.dSlotEps, &P.dSlotInv, &P.secArrays, &P.ws})
@@ -867,16 +871,19 @@ static void finish_single(orf_plan& P, Work& w, int Cs, cplx* const a[4], const 
   KCHECK();
 }
 
-// One patterned layer, full basis: R and T on the source columns straight from
-// the layer's modes (W in w.Wm, T12 = [Eyy W2; Exx W1] in w.T12, lam, kL) and the
-// gap's V0 (w.X2), without the layer S-matrix (smatrixLayer.m:6-18 and the two
-// star products of rcwa2D.m give the same R, T).  direct_blocks_kernel states
-// the system [A1 B1; B2 A2] [h+; h-] = [a21 C; 0]; its Schur complement on h+:
+// One patterned layer: R and T on the source columns straight from the layer's
+// modes (W in w.Wm, lam, kL; full basis: T12 = [Eyy W2; Exx W1] in w.T12 and the
+// per-context V0 and half-space tables; sector bases: V in w.Vm, V0^-1 in w.X2
+// and the half-space blocks gathered per solve in SA, SB), without the layer
+// S-matrix (smatrixLayer.m:6-18 and the two star products of rcwa2D.m give the
+// same R, T).  direct_blocks_kernel (full basis) or dense_blocks_kernel (sector
+// bases) states the system [A1 B1; B2 A2] [h+; h-] = [a21 C; 0]; its Schur
+// complement on h+:
 //   K = A2^-1 B2,  (A1 - B1 K) h+ = a21 C,  h- = -K h+,
 // then c1- = W S1 + U D1, c2+ = W S2 + U D2 (direct_vec_kernel), R = a11 C + a12 c1-,
 // T = b21 c2+.  Two n x n LUs, one n-column solve and one product.
-static void finish_direct(orf_plan& P, Work& w, int Cs, const DevPage* pages, int ncol, const int* colIdx, cplx* R,
-                          cplx* T) {
+static void finish_direct(orf_plan& P, Work& w, int Cs, const DevPage* pages, cplx* const SA[4], cplx* const SB[4],
+                          int ncol, const int* colIdx, cplx* R, cplx* T) {
   const int n = P.nr, M = P.M;
   const long long n2 = (long long)n * n, nc2 = (long long)n * ncol, G = (long long)P.nCtx * P.nSec;
   // half-space blocks and the gap's V0 per context (page w.groupOf[sp])
@@ -891,10 +898,23 @@ static void finish_direct(orf_plan& P, Work& w, int Cs, const DevPage* pages, in
   cplx* B1 = w.LUb;
   cplx* B2 = w.Fm;   // then K
   cplx* A2 = w.M1;   // then its LU
-  const int nb = (n + 7) / 8;  // kDirectCols columns per thread
-  direct_blocks_kernel<<<g1((long long)M * nb, Cs), 256, 0, st>>>(
-      M, w.solves, pages, P.dKxy.as<double>(), w.Wm, w.T12, w.lam, w.kL, w.groupOf, P.dV0.as<cplx>(), a22, b11, n2,
-      U, A1, B1, B2, A2);
+  if (!P.reduced) {
+    const int nb = (n + 7) / 8;  // kDirectCols columns per thread
+    direct_blocks_kernel<<<g1((long long)M * nb, Cs), 256, 0, st>>>(
+        M, w.solves, pages, P.dKxy.as<double>(), w.Wm, w.T12, w.lam, w.kL, w.groupOf, P.dV0.as<cplx>(), a22, b11, n2,
+        U, A1, B1, B2, A2);
+  } else {
+    // sector bases: V in w.Vm, V0^-1 per solve in w.X2, dense half-space blocks
+    // (gathered per solve in SA, SB): U = V0^-1 V, [W+ W-] in RHS2, a22 [W+ W-]
+    // in R3, b11 W+ and b11 W- in Sa[0], Sa[1]
+    orf_zgemm(st, 'N', 'N', n, n, n, ONE, w.X2, n, n2, w.Vm, n, n2, ZERO, U, n, n2, Cs);
+    wpm_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, w.Wm, U, n2, w.RHS2, 2 * n2);
+    orf_zgemm(st, 'N', 'N', n, 2 * n, n, ONE, SA[3], n, n2, w.RHS2, n, 2 * n2, ZERO, w.R3, n, 3 * n2, Cs);
+    orf_zgemm(st, 'N', 'N', n, n, n, ONE, SB[0], n, n2, w.RHS2, n, 2 * n2, ZERO, w.Sa[0], n, n2, Cs);
+    orf_zgemm(st, 'N', 'N', n, n, n, ONE, SB[0], n, n2, w.RHS2 + n2, n, 2 * n2, ZERO, w.Sa[1], n, n2, Cs);
+    dense_blocks_kernel<<<g1(n2, Cs), 256, 0, st>>>(n, w.lam, w.kL, w.RHS2, w.R3, w.Sa[0], w.Sa[1], n2, A1, B1, B2,
+                                                     A2);
+  }
   KCHECK();
   orf_zgetrf(st, n, A2, n, n2, w.ipiv, nullptr, Cs, w.dinv);
   orf_zgetrs(st, 'L', n, n, A2, n, n2, w.ipiv, w.dinv, B2, n, n2, Cs);
@@ -906,13 +926,20 @@ static void finish_direct(orf_plan& P, Work& w, int Cs, const DevPage* pages, in
   cplx* Sv = w.Tm + 2 * nc2;
   cplx* Dv = w.Tm + 4 * nc2;
   cplx* Yv = w.Tm + 6 * nc2;
-  direct_rhs_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, w.groupOf, a21, hp, n2);
+  if (!P.reduced) direct_rhs_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, w.groupOf, a21, hp, n2);
+  else select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, SA[2], n2, hp, n2);
   orf_zgetrs(st, 'L', n, ncol, A1, n, n2, w.ipiv, w.dinv, hp, n, n2, Cs);
   orf_zgemm(st, 'N', 'N', n, ncol, n, MONE, B2, n, n2, hp, n, n2, ZERO, hm, n, n2, Cs);
   direct_vec_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, w.lam, w.kL, hp, hm, Sv, Dv, n2);
   orf_zgemm(st, 'N', 'N', n, 2 * ncol, n, ONE, w.Wm, n, n2, Sv, n, n2, ZERO, Yv, n, n2, Cs);
   orf_zgemm(st, 'N', 'N', n, 2 * ncol, n, ONE, U, n, n2, Dv, n, n2, ONE, Yv, n, n2, Cs);
-  direct_out_kernel<<<g1(nc2, Cs), 256, 0, st>>>(M, ncol, colIdx, w.groupOf, a11, a12, b21, Yv, n2, R, T);
+  if (!P.reduced) {
+    direct_out_kernel<<<g1(nc2, Cs), 256, 0, st>>>(M, ncol, colIdx, w.groupOf, a11, a12, b21, Yv, n2, R, T);
+  } else {
+    select_cols_kernel<<<g1(nc2, Cs), 256, 0, st>>>(n, ncol, colIdx, SA[0], n2, R, nc2);
+    orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, SA[1], n, n2, Yv, n, n2, ONE, R, n, nc2, Cs);
+    orf_zgemm(st, 'N', 'N', n, ncol, n, ONE, SB[2], n, n2, Yv + nc2, n, n2, ZERO, T, n, nc2, Cs);
+  }
   KCHECK();
 }
 
@@ -957,6 +984,11 @@ static void build_regions(orf_plan& P, const std::vector<zc>& epsInc, const std:
   DBuf dDinv;
   dDinv.alloc(sizeof(cplx) * orf_dinv_stride(n) * 2 * G);
   orf_zgetrf(st, n, dLU.as<cplx>(), n, n2, dip.as<int>(), nullptr, 2 * G, dDinv.as<cplx>());
+  if (P.reduced) {  // V0^-1 per group for the direct single-layer solve (finish_direct)
+    P.dV0inv.alloc(sizeof(cplx) * n2 * G);
+    eye_axpby_kernel<<<g1(n2, G), 256, 0, st>>>(n, ONE, ZERO, nullptr, 0, P.dV0inv.as<cplx>(), n2);
+    orf_zgetrs(st, 'L', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), dDinv.as<cplx>(), P.dV0inv.as<cplx>(), n, n2, G);
+  }
   dX.alloc(sizeof(cplx) * n2 * 2 * G);
   HIPCHECK(hipMemcpyAsync(dX.p, dV.as<cplx>() + n2 * G, sizeof(cplx) * n2 * 2 * G, hipMemcpyDeviceToDevice, st));
   orf_zgetrs(st, 'L', n, n, dLU.as<cplx>(), n, n2, dip.as<int>(), dDinv.as<cplx>(), dX.as<cplx>(), n, n2, 2 * G);
@@ -1374,7 +1406,7 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   int cp = (int)std::max<size_t>(1, budget / std::max<size_t>(perMode, 1));
   cp = std::min(cp, 32768 / P.nSec);  // grid-dimension limit of the page-batched launches
   P.layerParallel = P.nLayers >= 2 && !P.keepModes;
-  P.directSingle = P.nLayers == 1 && !P.keepModes && !P.reduced && P.layerAnyPat[0] && P.nr >= 16;
+  P.directSingle = P.nLayers == 1 && !P.keepModes && P.layerAnyPat[0] && P.nr >= (P.reduced ? 8 : 16);
   if (P.layerParallel) {
     // every layer of a page is in the chunk at once: cap the chunk at kLayerChunkSolves
     // solve pages (enough to fill the device) so that its workspace stays moderate
@@ -1543,7 +1575,7 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
       bulk_gemm(P, 'N', 'N', M, nr, M, ONE, w.Exx, M, M2, Wf + s * nfr, nf, nSec * nfr, ZERO,
                 w.T12 + s * nfr + M, nf, nSec * nfr, Cp);
     }
-    if (P.directSingle) {  // finish_direct forms Q W itself from W and T12
+    if (P.directSingle && !P.reduced) {  // finish_direct forms Q W itself from W and T12
       KCHECK();
       return;
     }
@@ -1571,10 +1603,11 @@ static void finish_chunk(orf_plan& P, Work& w, int p0, int Cp, cplx* const SA[4]
   hipStream_t st = P.st;
   const int nr = P.nr, nSec = P.nSec, Cs = Cp * nSec;
   const long long n2 = (long long)nr * nr, G = (long long)P.nCtx * nSec;
-  const bool direct = single && P.directSingle;  // reads the per-context tables itself
-  for (int k = 0; k < 4 && !direct; k += 2)
+  const bool direct = single && P.directSingle;
+  const bool gath = !direct || P.reduced;  // the full-basis direct solve reads the per-context tables itself
+  for (int k = 0; k < 4 && gath; k += 2)
     gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dStrn.as<cplx>() + k * n2 * G, n2, SB[k], n2);
-  if ((L || single) && !direct)
+  if ((L || single) && gath)
     for (int k = 0; k < 4; ++k)
       gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dSref.as<cplx>() + k * n2 * G, n2, SA[k], n2);
   const int ncol = P.reduced ? 1 : 2;
@@ -1586,7 +1619,7 @@ static void finish_chunk(orf_plan& P, Work& w, int p0, int Cp, cplx* const SA[4]
     }
     HIPCHECK(hipMemcpyAsync(w.colIdx, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, st));
   }
-  if (direct) finish_direct(P, w, Cs, P.dPagesByLayer[0]->as<DevPage>() + p0, ncol, w.colIdx, R, T);
+  if (direct) finish_direct(P, w, Cs, P.dPagesByLayer[0]->as<DevPage>() + p0, SA, SB, ncol, w.colIdx, R, T);
   else if (single) finish_single(P, w, Cs, SA, SB[0], SB[2], ncol, w.colIdx, R, T);
   else if (L) finish_stack(P, w, Cs, SA, L, SB[0], SB[2], ncol, w.colIdx, R, T);
   else smatrix_star_cols(P, w, Cs, SA, SB, ncol, w.colIdx, R, T);
@@ -1831,6 +1864,8 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
       HIPCHECK(hipMemcpyAsync(w.kL, kL.data(), sizeof(double) * Cs, hipMemcpyHostToDevice, st));
       if (!P.directSingle)
         gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dV0.as<cplx>(), n2, w.X2, n2);
+      else if (P.reduced)  // finish_direct: U = V0^-1 V
+        gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dV0inv.as<cplx>(), n2, w.X2, n2);
       if (P.keepModes) {
         auto* K = P.kept[j];
         K->W.alloc(sizeof(cplx) * n2);

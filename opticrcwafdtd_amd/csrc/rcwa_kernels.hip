@@ -546,6 +546,42 @@ __global__ void direct_vec_kernel(int n, int ncol, const cplx* __restrict__ lam,
   D[o + e2] = csub(xp, hm);
 }
 
+// [W + U, W - U] (n x 2n per page, stride sO) for the sector-basis form of the
+// single-layer system (the full basis forms its blocks in direct_blocks_kernel).
+__global__ void wpm_kernel(int n, const cplx* __restrict__ W, const cplx* __restrict__ U, long long s,
+                           cplx* __restrict__ out, long long sO) {
+  const int sp = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n) return;
+  const cplx w = W[(size_t)sp * s + e], u = U[(size_t)sp * s + e];
+  cplx* o = out + (size_t)sp * sO;
+  o[e] = cadd(w, u);
+  o[e + (size_t)n * n] = csub(w, u);
+}
+
+// The four blocks of the single-layer system from dense products (sector bases):
+// Wpm = [W+ W-] (stride 2 n^2), Pa = a22 [W+ W-] (stride 3 n^2), Pbp = b11 W+,
+// Pbm = b11 W- (stride s); X = exp(-lam kL) scales columns:
+//   A1 = W+ - a22 W-, B1 = (W- - a22 W+) X, B2 = (W- - b11 W+) X, A2 = W+ - b11 W-.
+__global__ void dense_blocks_kernel(int n, const cplx* __restrict__ lam, const double* __restrict__ kL,
+                                    const cplx* __restrict__ Wpm, const cplx* __restrict__ Pa,
+                                    const cplx* __restrict__ Pbp, const cplx* __restrict__ Pbm, long long s,
+                                    cplx* __restrict__ A1, cplx* __restrict__ B1, cplx* __restrict__ B2,
+                                    cplx* __restrict__ A2) {
+  const int sp = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n) return;
+  const size_t nn = (size_t)n * n, o = (size_t)sp * s + e;
+  const cplx x = cexp(cscale(lam[(size_t)sp * n + e / n], -kL[sp]));
+  const cplx* wpm = Wpm + (size_t)sp * 2 * nn;
+  const cplx* pa = Pa + (size_t)sp * 3 * nn;
+  const cplx wp = wpm[e], wm = wpm[e + nn];
+  A1[o] = csub(wp, pa[e + nn]);
+  B1[o] = cmul(x, csub(wm, pa[e]));
+  B2[o] = cmul(x, csub(wm, Pbp[o]));
+  A2[o] = csub(wp, Pbm[o]);
+}
+
 // h+ right-hand side a21 C: columns colIdx of the context's a21 (page grp[sp]).
 __global__ void direct_rhs_kernel(int n, int ncol, const int* __restrict__ colIdx, const int* __restrict__ grp,
                                   const cplx* __restrict__ a21, cplx* __restrict__ out, long long sO) {
