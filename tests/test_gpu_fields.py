@@ -58,3 +58,21 @@ def test_fields_slab_analytic_and_lossy_absorption():
     assert abs(f["flux"][0] - (1 - f["out"]["Rtot"][0])) < 1e-10
     assert abs(f["flux"][-1] - f["out"]["Ttot"][0]) < 1e-10
     assert np.all(np.diff(f["flux"][1:4]) < 0)
+
+
+@pytest.mark.parametrize("sym", ["none", "auto"])
+def test_single_layer_direct_solve_equals_smatrix_route(sym):
+    """One patterned layer is solved on its mode amplitudes without a layer
+    S-matrix; the kept-modes plan behind rcwaFields forms the layer S-matrix and
+    star products (smatrixLayer.m:6-18, smatrixStar.m:4-10).  Same R and T."""
+    dev = orf.makeStructure([0.4, 0.4], mc(1.45), mc(1),
+                            [orf.makeLayer(0.6, mc(1), [orf.shapeCircle(0.12, mc(3.6 + 0.01j))])])
+    for pol in ("s", "p"):
+        g = orf.rcwa2D(dev, 0.94, 0, 0, pol, Nx=5, Nsample=(256, 256), Symmetry=sym)
+        f = orf.rcwaFields(dev, 0.94, 0, 0, pol, np.zeros(1), np.zeros(1), np.array([0.3]), Nx=5,
+                           Nsample=(256, 256))["out"]
+        assert np.array_equal(np.ravel(g["m"]), np.ravel(f["m"])) and np.array_equal(np.ravel(g["n"]), np.ravel(f["n"]))
+        dT = np.max(np.abs(np.ravel(g["T"]) - np.ravel(f["T"])))
+        dR = np.max(np.abs(np.ravel(g["R"]) - np.ravel(f["R"])))
+        print(f"direct vs S-matrix route ({sym}, {pol}): max |dT| {dT:.2e}, max |dR| {dR:.2e}")
+        assert dT < 1e-11 and dR < 1e-11
