@@ -2740,19 +2740,21 @@ __global__ __launch_bounds__(256) void trevc_kernel(int n, const cplx* __restric
 
 // Blocked form of trevc_kernel: TV_EB eigenvectors of one page per workgroup.
 // Thread (e, g) = (t / 16, t % 16) holds rows g, g + 16, ... of eigenvector
-// k0 + e in registers.  Back substitution runs column by column from the
-// block's last one down to 0; each T column is staged in LDS once (fetched a
-// step ahead, three buffers) and serves every eigenvector of the block, so T
-// is read TV_EB times less often, and one barrier per column suffices.  Per
-// eigenvector the arithmetic and its order are those of trevc_kernel.
+// k0 + e in registers, so the 16 rows of each 16-row block of an eigenvector
+// sit in the 16 lanes of one group.  Back substitution runs block by block from
+// the top one down: the 16 x 16 diagonal block of T (staged in LDS) is solved
+// across the group's lanes with shuffles, then the rows above take the block's
+// 16 columns at once (T read through the cache, shared by the 16 groups): two
+// workgroup barriers per block instead of one per column.  Per eigenvector the
+// divisions and the small-pivot rule are those of trevc_kernel; the updates of
+// the rows above a block are summed over its columns.
 constexpr int TV_EB = 16;
 template <int RPT>
 __global__ __launch_bounds__(256) void trevc_blocked_kernel(int n, const cplx* __restrict__ T, int ldt,
                                                             long long sT, cplx* __restrict__ Y, int ldy,
                                                             long long sY) {
-  extern __shared__ cplx tvs[];
-  cplx* tcol = tvs;                    // 3 x n: staged columns
-  cplx* xjb = tvs + 3 * (size_t)n;     // 2 x TV_EB: x_j of every eigenvector of the block
+  __shared__ cplx tdiag[256];  // diagonal block, column-major 16 x 16
+  __shared__ cplx xsh[256];    // the block's solved values, 16 per eigenvector
   int page, kbk;
   xcd_page_tile2(page, kbk);
   const int k0 = kbk * TV_EB, t = threadIdx.x;
@@ -2771,38 +2773,58 @@ __global__ __launch_bounds__(256) void trevc_blocked_kernel(int n, const cplx* _
     x[sl] = (act && r < k) ? cneg(T[r + (size_t)k * ldt]) : cmk(act && r == k ? 1.0 : 0.0, 0.0);
   }
   const int kmax = min(k0 + TV_EB, n) - 1;
-  const int jtop = kmax - 1;
-  if (jtop >= 0)
-    for (int r = t; r <= jtop; r += 256) tcol[(size_t)(jtop % 3) * n + r] = T[r + (size_t)jtop * ldt];
-  __syncthreads();
-  for (int j = jtop; j >= 0; --j) {
-    const cplx* tc = tcol + (size_t)(j % 3) * n;
-    if (g == (j & 15)) {  // owner of row j
-      const int sj = j >> 4;
-      cplx xv = cmk(0.0, 0.0);
-#pragma unroll
-      for (int sl = 0; sl < RPT; ++sl)
-        if (sl == sj) xv = x[sl];
-      if (act && k > j) {
-        cplx d = csub(tc[j], tkk);
-        if (cabs1(d) < smin) d = cmk(smin, 0.0);
-        xv = cdiv(xv, d);
-#pragma unroll
-        for (int sl = 0; sl < RPT; ++sl)
-          if (sl == sj) x[sl] = xv;
-      } else {
-        xv = cmk(0.0, 0.0);
-      }
-      xjb[(j & 1) * TV_EB + e] = xv;
+  const int jtop = kmax - 1;  // columns jtop .. 0 carry the back substitution
+  for (int sb = jtop >= 0 ? (jtop >> 4) : -1; sb >= 0; --sb) {
+    const int j0 = 16 * sb, jhi = min(j0 + 15, jtop);
+    __syncthreads();  // the previous block's readers of tdiag are done
+    {
+      const int r = t & 15, c = t >> 4;
+      tdiag[t] = (j0 + r < n && j0 + c <= jhi) ? T[(j0 + r) + (size_t)(j0 + c) * ldt] : cmk(0.0, 0.0);
     }
-    if (j > 0)  // next column, rows 0..j-1 (its diagonal included)
-      for (int r = t; r < j; r += 256) tcol[(size_t)((j - 1) % 3) * n + r] = T[r + (size_t)(j - 1) * ldt];
     __syncthreads();
-    const cplx xj = xjb[(j & 1) * TV_EB + e];
+    cplx xb = cmk(0.0, 0.0);  // row j0 + g of this lane's eigenvector
+#pragma unroll
+    for (int sl = 0; sl < RPT; ++sl)
+      if (sl == sb) xb = x[sl];
+    for (int j = jhi; j >= j0; --j) {
+      const int gj = j - j0;
+      cplx xj = cmk(0.0, 0.0);
+      if (g == gj && act && k > j) {
+        cplx d = csub(tdiag[gj + 16 * gj], tkk);
+        if (cabs1(d) < smin) d = cmk(smin, 0.0);
+        xb = cdiv(xb, d);
+        xj = xb;
+      }
+      xj.x = __shfl(xj.x, gj, 16);
+      xj.y = __shfl(xj.y, gj, 16);
+      if (g < gj) xb = csub(xb, cmul(tdiag[g + 16 * gj], xj));
+    }
+#pragma unroll
+    for (int sl = 0; sl < RPT; ++sl)
+      if (sl == sb) x[sl] = xb;
+    if (sb == 0) continue;
+    // the block's solved values (zero where column j0 + c does not act on this
+    // eigenvector); the 16 lanes of a group are in one wave
+    xsh[t] = (act && k > j0 + g && j0 + g <= jhi) ? xb : cmk(0.0, 0.0);
+    wave_sync_lds();
+    const cplx* xs = xsh + 16 * e;
+    const int nc = jhi - j0 + 1;
 #pragma unroll
     for (int sl = 0; sl < RPT; ++sl) {
-      const int r = g + 16 * sl;
-      if (r < j) x[sl] = csub(x[sl], cmul(tc[r], xj));
+      if (sl < sb) {
+        const cplx* tr = T + (g + 16 * sl) + (size_t)j0 * ldt;
+        cplx acc = x[sl];
+#pragma unroll
+        for (int h = 0; h < 16; h += 8) {  // eight loads in flight
+          cplx tv[8];
+#pragma unroll
+          for (int c = 0; c < 8; ++c) tv[c] = tr[(size_t)min(h + c, nc - 1) * ldt];
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            if (h + c < nc) acc = csub(acc, cmul(tv[c], xs[h + c]));
+        }
+        x[sl] = acc;
+      }
     }
   }
   double mx = 0.0;
@@ -2954,12 +2976,11 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
   dim3 g2((n + 3) / 4, batch);
   tb("eig_vectors");
   {
-    const size_t lds = (3 * (size_t)n + 2 * TV_EB) * sizeof(cplx);
     const dim3 gb((n + TV_EB - 1) / TV_EB, batch);
     if (n <= 256)
-      trevc_blocked_kernel<16><<<gb, 256, lds, st>>>(n, A, lda, sA, Ywork, n, sZ);
+      trevc_blocked_kernel<16><<<gb, 256, 0, st>>>(n, A, lda, sA, Ywork, n, sZ);
     else if (n <= 512)
-      trevc_blocked_kernel<32><<<gb, 256, lds, st>>>(n, A, lda, sA, Ywork, n, sZ);
+      trevc_blocked_kernel<32><<<gb, 256, 0, st>>>(n, A, lda, sA, Ywork, n, sZ);
     else
       trevc_kernel<<<g2, 256, 0, st>>>(n, A, lda, sA, Ywork, n, sZ);
   }
