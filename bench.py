@@ -31,6 +31,7 @@ import numpy as np  # noqa: E402
 
 METRIC = "unit-cell solves/s at ±5 harm. (|ΔT|≤1e-6 vs FP64) + p50/p99 request latency"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBPS = 8000.0
 
 N_HARM = 5
 RADII = np.linspace(0.040, 0.180, 256)
@@ -46,7 +47,8 @@ def make_devs(orf, radii):
 
 # ---------------------------------------------------------------- CPU baseline (oracle)
 def _cpu_worker(args):
-    radii, lam, sym = args
+    idx, lam_i, sym = args
+    radii, lam = [float(RADII[i]) for i in idx], float(LAMBDAS[lam_i])
     import opticrcwafdtd_amd.structure as S
     from oracle import rcwa_oracle as O
 
@@ -57,28 +59,56 @@ def _cpu_worker(args):
         shapeCircle = staticmethod(S.shapeCircle)
     devs = make_devs(_NS, radii)
     t0 = time.perf_counter()
-    O.rcwa2d(devs, lam, 0.0, 0.0, "s", Nx=N_HARM, Nsample=(256, 256), Symmetry=sym)
-    return time.perf_counter() - t0, len(radii)
+    o = O.rcwa2d(devs, lam, 0.0, 0.0, "s", Nx=N_HARM, Nsample=(256, 256), Symmetry=sym)
+    return time.perf_counter() - t0, len(radii), (list(idx), lam_i, o["R"], o["T"], o["r"], o["t"])
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(sym, workers, per_worker=12):
     """The FP64 oracle (numpy + LAPACK, one thread per process) on a bounded
-    sample of the workload: workers x per_worker single-polarisation solves."""
+    sample of the workload: workers x per_worker TE solves.  Also returns the
+    oracle's results, compared with the GPU's pages afterwards."""
     jobs = []
     rng = np.random.default_rng(0)
     for w in range(workers):
-        rs = rng.choice(RADII, size=per_worker, replace=False)
-        jobs.append((list(rs), float(LAMBDAS[w % len(LAMBDAS)]), sym))
+        idx = sorted(int(i) for i in rng.choice(len(RADII), size=per_worker, replace=False))
+        jobs.append((idx, w % len(LAMBDAS), sym))
     ctx = mp.get_context("fork")
     t0 = time.perf_counter()
     with ctx.Pool(workers) as pool:
         res = pool.map(_cpu_worker, jobs)
     wall = time.perf_counter() - t0
     solves = sum(r[1] for r in res)
+    host = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     return {"value": solves / wall, "unit": "solves/s", "cores": workers, "kind": "port",
-            "sample": f"{solves} single-polarisation solves (Symmetry={sym}) of the same workload, "
+            "cpu_model": _cpu_model(), "host_cpus_visible": host,
+            "sample": f"{solves} TE solves (Symmetry={sym}) of the same workload, "
                       f"{per_worker} cells per process, {workers} processes x 1 BLAS thread, "
-                      f"{wall:.1f} s wall"}
+                      f"{wall:.1f} s wall"}, [r[2] for r in res]
+
+
+def parity_vs_oracle(samples, gpu):
+    """Max |difference| between the oracle's TE results of the CPU sample and the
+    GPU's pages of the same (radius, wavelength) in the timed plan."""
+    d = {"max_abs_dR": 0.0, "max_abs_dT": 0.0, "max_abs_dr": 0.0, "max_abs_dt": 0.0, "pages": 0}
+    for idx, l, R, T, r, t in samples:
+        for k, b in enumerate(idx):
+            d["max_abs_dR"] = max(d["max_abs_dR"], float(np.max(np.abs(gpu["R"][:, b, l, 0] - R[k]))))
+            d["max_abs_dT"] = max(d["max_abs_dT"], float(np.max(np.abs(gpu["T"][:, b, l, 0] - T[k]))))
+            d["max_abs_dr"] = max(d["max_abs_dr"], float(np.max(np.abs(gpu["r"][:, :, b, l, 0].T - r[k]))))
+            d["max_abs_dt"] = max(d["max_abs_dt"], float(np.max(np.abs(gpu["t"][:, :, b, l, 0].T - t[k]))))
+            d["pages"] += 1
+    d["within_1e-10"] = max(d["max_abs_dR"], d["max_abs_dT"], d["max_abs_dr"], d["max_abs_dt"]) <= 1e-10
+    return d
 
 
 # ---------------------------------------------------------------- algorithmic flops
@@ -119,10 +149,11 @@ def run(args, sym, precision="fp64", steps=None, warmup=None):
     t1 = time.perf_counter()
     plan.enable_timing(False)
     solves = len(RADII) * len(LAMBDAS) * 2
-    out = plan.fetch(amplitudes=False)
+    out = plan.fetch(amplitudes=True)
     plan.close()
     return {"elapsed": t1 - t0, "steps": steps, "stages": stages, "solves": solves, "n": plan.solved_size,
-            "symmetric": plan.symmetric, "pages": plan.solve_pages, "R": out["R"], "T": out["T"]}
+            "symmetric": plan.symmetric, "pages": plan.solve_pages, "R": out["R"], "T": out["T"],
+            "r": out["r"], "t": out["t"]}
 
 
 def extras(args, full):
@@ -214,7 +245,10 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=300)
-    ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
+    # the GPU host's CPU share is 16 cores per GPU; fewer where fewer are visible
+    ap.add_argument("--cpu-workers", type=int,
+                    default=min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                                else (os.cpu_count() or 1)))
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -224,7 +258,7 @@ def main():
               "their own share of the batch with no collective on the critical path; only --gpus 1 is "
               "supported.", file=sys.stderr)
         sys.exit(2)
-    cpu = None if args.no_cpu else cpu_baseline(args.symmetry, args.cpu_workers)
+    cpu, samples = (None, None) if args.no_cpu else cpu_baseline(args.symmetry, args.cpu_workers)
     r = run(args, args.symmetry)
     ms = 1e3 * r["elapsed"] / args.steps
     value = r["solves"] * args.steps / r["elapsed"]
@@ -238,6 +272,13 @@ def main():
     avg_s = tot_ms / launches / 1e3
     achieved = flops / avg_s / 1e12 if flops else None
     M = (2 * N_HARM + 1) ** 2
+    traffic = measured_traffic(dom, args.symmetry)
+    hbm = None
+    if traffic:
+        hbm = {"traffic_bytes_per_launch": traffic, "achieved_GBps": round(traffic / avg_s / 1e9, 1),
+               "peak_GBps": PEAK_HBM_GBPS, "frac": round(traffic / avg_s / 1e9 / PEAK_HBM_GBPS, 4),
+               "what": "measured HBM traffic of the dominant kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE, "
+                       "calibrated against a copy) over its launch time: the bandwidth it actually draws"}
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
@@ -249,13 +290,14 @@ def main():
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 4) if achieved else None,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 5) if achieved else None,
-                     "traffic": measured_traffic(dom, args.symmetry), "avg_launch_ms": round(tot_ms / launches, 3),
-                     "flops_per_launch": flops},
+                     "traffic": traffic, "avg_launch_ms": round(tot_ms / launches, 3),
+                     "flops_per_launch": flops, "hbm": hbm},
         "whole_solve": {"canonical_flops_per_page": solve_flops(n, M if args.symmetry == "none" else 0),
                         "achieved_tflops": round(solve_flops(n, M if args.symmetry == "none" else 0) *
                                                  pages / (ms / 1e3) / 1e12, 4)},
         "stages_ms_per_step": {k: round(v[0] / r["steps"], 3) for k, v in r["stages"].items()},
         "cpu_baseline": cpu,
+        "parity_vs_oracle": parity_vs_oracle(samples, r) if samples else None,
     }
     if not args.quick:
         line["extras"] = extras(args, r)

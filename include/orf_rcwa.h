@@ -139,7 +139,10 @@ int orf_plan_create(orf_ctx* ctx, const orf_problem* p, orf_plan** plan);
 int orf_plan_execute(orf_plan* plan, orf_progress_fn progress, void* user);
 int orf_plan_fetch(orf_plan* plan, orf_result* out);
 void orf_plan_destroy(orf_plan* plan);
-/* Solved matrix size (2M full, Mr reduced) and 1 if the symmetry reduction is used. */
+/* Solved matrix size (2M full, Mr reduced); symmetric = 0 for the full basis,
+ * 1 for the mirror-parity sectors (Symmetry "auto", symmetryBasis.m:2-15),
+ * 2 for the TE / TM split of a y-invariant grating lit in the x-z plane
+ * (Symmetry "auto", Ny = 0, every k_y = 0; M unknowns per polarisation). */
 int orf_plan_info(const orf_plan* plan, int32_t* solved_size, int32_t* symmetric,
                   int32_t* n_solve_pages);
 /* Per-stage device time of the last execute, measured with HIP events on the

@@ -5,10 +5,10 @@ builders mirror toolbox/+orf's model functions, and ``rcwa2D``, ``rcwa1D``,
 ``buildPhaseLibrary`` and ``convergeN`` run every solve through the HIP
 library liborf_rcwa.so (C ABI in include/orf_rcwa.h).
 """
-from .structure import (OrfError, alignedSampleCount, cellGrid, layerEps, makeLayer, makeStructure, material,
+from .structure import (OrfError, OrfWarning, alignedSampleCount, cellGrid, layerEps, makeLayer, makeStructure, material,
                         materialConst, shapeCircle, shapeCross, shapePolygon, shapeRect, slantedLayers)
 from .rcwa import Plan, Problem, buildPhaseLibrary, convergeN, rcwa1D, rcwa2D, rcwaFields
 
-__all__ = ["OrfError", "alignedSampleCount", "cellGrid", "layerEps", "makeLayer", "makeStructure", "material",
+__all__ = ["OrfError", "OrfWarning", "alignedSampleCount", "cellGrid", "layerEps", "makeLayer", "makeStructure", "material",
            "materialConst", "shapeCircle", "shapeCross", "shapePolygon", "shapeRect", "slantedLayers",
            "Plan", "Problem", "buildPhaseLibrary", "convergeN", "rcwa1D", "rcwa2D", "rcwaFields"]

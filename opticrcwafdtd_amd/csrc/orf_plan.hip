@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -227,7 +228,15 @@ struct DBuf {
   DBuf() {}
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
-  ~DBuf() { if (p) g_pool.put(p); }
+  // Blocks go back to the pool only when no kernel can still write them.  Every
+  // owner synchronises its stream before a normal return; while an exception
+  // unwinds (an allocation failing half-way through a stage), kernels already
+  // queued on the owner's stream may still run, so the device is drained first.
+  ~DBuf() {
+    if (!p) return;
+    if (std::uncaught_exceptions() > 0) (void)hipDeviceSynchronize();
+    g_pool.put(p);
+  }
   void alloc(size_t b) {
     if (p) { g_pool.put(p); p = nullptr; }
     bytes = b;
@@ -352,6 +361,29 @@ static HostSector build_sector(int Nx, int Ny, bool polP, int& nr) {
   return s;
 }
 
+// y-invariant grating lit in the x-z plane (Ny = 0, every k_y = 0): P Q is
+// block diagonal, blkdiag(-(I - Kx [eps]^-1 Kx) Exx, Kx^2 - Eyy), so E_x / H_y
+// (TM, p) and E_y / H_x (TE, s) are solved apart, M unknowns each.  The same
+// sparse-list form as the parity sectors, one entry per column:
+// p: UE = [Ex], UH = [Hy];  s: UE = [Ey], UH = [Hx].
+static HostSector build_sector_te_tm(int M, bool polP) {
+  const int nf = 2 * M;
+  HostSector s;
+  s.ue_row.assign(4 * M, -1); s.uh_row.assign(4 * M, -1);
+  s.ue_coef.assign(4 * M, 0); s.uh_coef.assign(4 * M, 0);
+  s.ue_rowmap.assign(nf, -1); s.ue_rowcoef.assign(nf, 0);
+  const int offE = polP ? 0 : M, offH = polP ? M : 0;
+  for (int a = 0; a < M; ++a) {
+    s.ue_row[4 * a] = offE + a;
+    s.ue_coef[4 * a] = 1.0;
+    s.uh_row[4 * a] = offH + a;
+    s.uh_coef[4 * a] = 1.0;
+    s.ue_rowmap[offE + a] = a;
+    s.ue_rowcoef[offE + a] = 1.0;
+  }
+  return s;
+}
+
 // ---------------------------------------------------------------- workspace set
 struct Work {
   // mode-page buffers (per chunk of Cp mode pages)
@@ -377,6 +409,14 @@ struct Work {
 };
 
 // ====================================================================== plan
+// Size limits: the eigensolver keeps a column of the solved matrix in 12
+// registers per lane (n <= 768); the convolution-matrix LU keeps a 4-wide panel
+// of M rows plus 4 rows of U in the 140 KiB of LDS it uses (M <= 1024).
+static const int kMaxSolved = 768;
+static const int kMaxHarmonics = 1024;
+// P.reduced: 0 full basis, 1 mirror-parity sectors, 2 TE / TM of a y-invariant grating
+static const int kReducedTeTm = 2;
+
 struct orf_plan {
   orf_ctx* ctx = nullptr;
   hipStream_t st = nullptr;
@@ -1057,7 +1097,10 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   const int Wx = 2 * P.Nx + 1, Wy = 2 * P.Ny + 1;
   P.M = Wx * Wy;
   P.nf = 2 * P.M;
-  if (P.nf > 768) fail(ORF_INVALID_ARG, "2M > 768 is not supported");
+  // the M x M convolution-matrix LU keeps a 4-column panel of M rows in LDS
+  if (P.M > kMaxHarmonics)
+    fail(ORF_INVALID_ARG, "M = " + std::to_string(P.M) + " harmonics: at most " + std::to_string(kMaxHarmonics) +
+                              " are supported");
   P.B = pr->nStruct;
   P.L = pr->nLambda;
   P.nPol = pr->nPol;
@@ -1278,6 +1321,11 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
     }
     P.reduced = sym;
   }
+  // otherwise a y-invariant grating lit in the x-z plane splits into TE and TM
+  // (exact, like the mirror reduction; same sources "s" / "p" only)
+  if (!P.reduced && pr->symmetry == ORF_SYM_AUTO && P.Ny == 0 && std::sin(P.theta) * std::sin(P.phi) == 0.0 &&
+      allSP && !P.keepModes)
+    P.reduced = kReducedTeTm;
   // sectors and sources
   std::vector<HostSector> hs;
   std::vector<int> srcSector(P.nPol, 0);
@@ -1286,8 +1334,9 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
     for (int i = 0; i < P.nPol; ++i) {
       const int k = pr->polKind[i];
       if (secOf[k] < 0) {
-        int nr = 0;
-        hs.push_back(build_sector(P.Nx, P.Ny, k == ORF_POL_P, nr));
+        int nr = P.M;
+        if (P.reduced == kReducedTeTm) hs.push_back(build_sector_te_tm(P.M, k == ORF_POL_P));
+        else hs.push_back(build_sector(P.Nx, P.Ny, k == ORF_POL_P, nr));
         secOf[k] = (int)hs.size() - 1;
         P.nr = nr;
       }
@@ -1298,6 +1347,10 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
     P.nr = P.nf;
     P.nSec = 1;
   }
+  if (P.nr > kMaxSolved)
+    fail(ORF_INVALID_ARG, "solved matrix size " + std::to_string(P.nr) + " (2M = " + std::to_string(P.nf) +
+                              (P.reduced ? ", reduced basis" : ", full basis") + "): at most " +
+                              std::to_string(kMaxSolved) + " is supported");
   {
     std::vector<DevSector> ds(std::max(1, P.nSec));
     for (int s = 0; s < P.nSec && P.reduced; ++s) {
@@ -1916,6 +1969,18 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
   }
   HIPCHECK(hipStreamSynchronize(st));
   P.timer.collect();
+  // a singular interface system (or non-finite input) shows as a non-finite total
+  {
+    const size_t nout = (size_t)P.nPages * P.nPol;
+    std::vector<double> rt(nout), tt(nout);
+    HIPCHECK(hipMemcpyAsync(rt.data(), P.dRtot.p, sizeof(double) * nout, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(tt.data(), P.dTtot.p, sizeof(double) * nout, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    for (size_t q = 0; q < nout; ++q)
+      if (!std::isfinite(rt[q]) || !std::isfinite(tt[q]))
+        fail(ORF_DEVICE_ERROR, "non-finite efficiencies on page " + std::to_string(q % P.nPages) + ", source " +
+                                   std::to_string(q / P.nPages) + " (singular interface system or non-finite input)");
+  }
   P.haveResult = true;
 }
 
@@ -2279,7 +2344,8 @@ int orf_plan_estimate(const orf_problem* p, int32_t reduced, int64_t* bytes) {
     P.nf = 2 * P.M;
     P.factorization = p->factorization;
     if (reduced) {
-      P.nr = (P.Nx + 1) * (P.Ny + 1) + P.Nx * P.Ny;
+      // mirror-parity sectors; a y-invariant grating at most its TE / TM split
+      P.nr = P.Ny == 0 ? P.M : (P.Nx + 1) * (P.Ny + 1) + P.Nx * P.Ny;
       bool s = false, pp = false;
       for (int i = 0; i < p->nPol; ++i) { s |= p->polKind[i] == ORF_POL_S; pp |= p->polKind[i] == ORF_POL_P; }
       P.nSec = (int)s + (int)pp;

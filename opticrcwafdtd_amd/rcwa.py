@@ -22,7 +22,7 @@ from typing import Callable, Optional, Sequence
 import numpy as np
 
 from . import _lib
-from .structure import OrfError, alignedSampleCount, default_sample_counts, layerEps, material_index
+from .structure import OrfError, OrfWarning, alignedSampleCount, default_sample_counts, layerEps, material_index
 
 _SHAPE_TYPES = {"circle": 0, "rect": 1, "polygon": 2}
 
@@ -231,7 +231,9 @@ class Plan:
         self.handle = h
         ns, sym, npg = C.c_int32(), C.c_int32(), C.c_int32()
         lib.orf_plan_info(h, C.byref(ns), C.byref(sym), C.byref(npg))
-        self.solved_size, self.symmetric, self.solve_pages = ns.value, bool(sym.value), npg.value
+        self.solved_size, self.solve_pages = ns.value, npg.value
+        self.symmetric = sym.value == 1                      # mirror-parity sectors
+        self.basis = ("full", "mirror", "te-tm")[sym.value]  # te-tm: y-invariant grating, k_y = 0
 
     def memory(self) -> int:
         """Device bytes this plan holds (tables, results, workspace)."""
@@ -274,7 +276,7 @@ class Plan:
                               _lib.dptr(R), _lib.dptr(T), _lib.dptr(Rt), _lib.dptr(Tt),
                               _lib.dptr(r) if amplitudes else None, _lib.dptr(t) if amplitudes else None)
         _raise(self._lib.orf_plan_fetch(self.handle, C.byref(res)))
-        return _pack(pr, m, n, R, T, Rt, Tt, r, t, self.symmetric)
+        return _pack(pr, m, n, R, T, Rt, Tt, r, t, self.symmetric, self.basis)
 
     def close(self):
         if getattr(self, "handle", None):
@@ -288,7 +290,7 @@ class Plan:
             pass
 
 
-def _pack(pr, m, n, R, T, Rt, Tt, r, t, symmetric):
+def _pack(pr, m, n, R, T, Rt, Tt, r, t, symmetric, basis="full"):
     """C-order (pol, lambda, B, M[, 3]) -> MATLAB order (M, B[, lambda][, pol])."""
     def fix(a, amp=False):
         if a is None:
@@ -305,7 +307,7 @@ def _pack(pr, m, n, R, T, Rt, Tt, r, t, symmetric):
             a = a[..., 0]
         return a
     out = {"m": m.astype(int), "n": n.astype(int), "R": fix(R), "T": fix(T), "Rtot": fix(Rt),
-           "Ttot": fix(Tt), "r": fix(r, True), "t": fix(t, True), "symmetry": symmetric,
+           "Ttot": fix(Tt), "r": fix(r, True), "t": fix(t, True), "symmetry": symmetric, "basis": basis,
            "Nsample": pr.Nsample}
     return out
 
@@ -424,12 +426,22 @@ def buildPhaseLibrary(makeCell: Callable, params: Sequence, lambda0, N=10, Pol="
 
 
 def convergeN(fun: Callable, Tol=1e-3, Ns=range(3, 26, 2), Progress=None):
-    """Raise N until two successive results differ by < Tol (convergeN.m:20-44)."""
+    """Raise N until two successive results differ by < Tol (convergeN.m:20-44).
+
+    Returns ``N`` (the first N that met Tol, or the last tried), ``value``
+    (fun(N)), ``converged`` and ``history``, the MATLAB table's columns
+    (N, change, seconds) as lists (convergeN.m:39-40).  An unconverged ladder
+    warns with identifier ``orf:convergeN:notConverged`` (convergeN.m:41-44).
+    """
     import time
     import warnings
-    Ns = list(Ns)
+    if not Tol > 0:
+        raise OrfError("orf:invalidArgument", "Tol must be positive")
+    Ns = [int(v) for v in Ns]
+    if not Ns or min(Ns) < 0:
+        raise OrfError("orf:invalidArgument", "Ns must be a non-empty list of non-negative integers")
     prev = None
-    hist = []
+    hist = {"N": [], "change": [], "seconds": []}
     res = {"converged": False}
     for k, N in enumerate(Ns):
         if Progress is not None:
@@ -437,8 +449,10 @@ def convergeN(fun: Callable, Tol=1e-3, Ns=range(3, 26, 2), Progress=None):
         t0 = time.perf_counter()
         q = np.atleast_1d(np.asarray(fun(N)))
         secs = time.perf_counter() - t0
-        change = float(np.max(np.abs(q - prev))) if prev is not None else float("nan")
-        hist.append({"N": N, "change": change, "seconds": secs})
+        change = float(np.max(np.abs(q.ravel() - prev.ravel()))) if prev is not None else float("nan")
+        hist["N"].append(N)
+        hist["change"].append(change)
+        hist["seconds"].append(secs)
         prev = q
         res["N"], res["value"] = N, q
         if change < Tol:
@@ -446,6 +460,7 @@ def convergeN(fun: Callable, Tol=1e-3, Ns=range(3, 26, 2), Progress=None):
             break
     res["history"] = hist
     if not res["converged"]:
-        warnings.warn(f"orf:convergeN:notConverged: Not converged to {Tol:g} by N = {res['N']} "
-                      f"(last change {hist[-1]['change']:.2g}).")
+        warnings.warn(OrfWarning("orf:convergeN:notConverged",
+                                 f"Not converged to {Tol:g} by N = {res['N']} "
+                                 f"(last change {hist['change'][-1]:.2g})."))
     return res

@@ -37,6 +37,15 @@ class OrfError(ValueError):
         self.identifier = ident
 
 
+class OrfWarning(UserWarning):
+    """Warning carrying the reference's MATLAB warning identifier
+    (e.g. ``orf:convergeN:notConverged``)."""
+
+    def __init__(self, ident: str, msg: str):
+        super().__init__(f"{ident}: {msg}")
+        self.identifier = ident
+
+
 # ---------------------------------------------------------------- materials
 def materialConst(n: Number, name: str = "const") -> dict:
     """Non-dispersive material with constant complex index (materialConst.m:9-11)."""

@@ -583,24 +583,48 @@ def build_phase_library(makeCell: Callable, params, lambda0, N=10, Pol="s", Chec
     return lib
 
 
+class OracleWarning(UserWarning):
+    """Warning carrying the reference's MATLAB warning identifier."""
+
+    def __init__(self, ident: str, msg: str):
+        super().__init__(f"{ident}: {msg}")
+        self.identifier = ident
+
+
 def converge_n(fun: Callable, Tol=1e-3, Ns=range(3, 26, 2), Progress=None):
-    """Raise N until successive results differ by < Tol (convergeN.m:20-44)."""
+    """Raise N until successive results differ by < Tol (convergeN.m:20-44).
+
+    ``history`` mirrors the MATLAB table (convergeN.m:40): columns N, change,
+    seconds over the N values tried; change is NaN for the first one.  A
+    ladder that ends unconverged warns 'orf:convergeN:notConverged'
+    (convergeN.m:41-44).
+    """
+    import time
+    import warnings
     Ns = list(Ns)
     prev = None
-    hist = []
+    hist = {"N": [], "change": [], "seconds": []}
     res = {"converged": False}
     for k, N in enumerate(Ns):
         if Progress is not None:
             Progress(k / len(Ns), f"N = {N}")
-        q = np.atleast_1d(fun(N))
-        change = float(np.max(np.abs(q - prev))) if prev is not None else float("nan")
-        hist.append((N, change))
+        t0 = time.perf_counter()
+        q = np.atleast_1d(np.asarray(fun(N)))
+        secs = time.perf_counter() - t0
+        change = float(np.max(np.abs(q.ravel() - prev.ravel()))) if prev is not None else float("nan")
+        hist["N"].append(N)
+        hist["change"].append(change)
+        hist["seconds"].append(secs)
         prev = q
         res["N"], res["value"] = N, q
         if change < Tol:
             res["converged"] = True
             break
     res["history"] = hist
+    if not res["converged"]:
+        warnings.warn(OracleWarning("orf:convergeN:notConverged",
+                                    f"Not converged to {Tol:g} by N = {res['N']} "
+                                    f"(last change {hist['change'][-1]:.2g})."))
     return res
 
 

@@ -84,6 +84,79 @@ def test_reticolo_phase_library(sym):
     assert np.max(np.abs(d)) < GOLD["phase_tolerance"]
 
 
+def _square_cell(a):
+    return S.makeStructure([0.4, 0.4], mc(1.45), mc(1), [S.makeLayer(0.5, mc(1), [S.shapeRect(a, a, mc(3.6))])])
+
+
+def test_build_phase_library_reticolo():
+    """tests/TestReticolo.m:63-78 through the phase-library caller (buildPhaseLibrary.m:2-17):
+    co-polarised zero-order T and the phase relative to the first cell."""
+    G = GOLD["phase_library"]
+    lib = O.build_phase_library(_square_cell, G["w"], 0.94, N=6, Pol="s")
+    np.testing.assert_allclose(lib["T"], G["T0"], atol=TOL, rtol=0)
+    ph = np.array(G["phase"])
+    assert lib["phase"][0] == 0.0
+    d = np.angle(np.exp(1j * lib["phase"])) - np.angle(np.exp(1j * (ph - ph[0])))
+    assert np.max(np.abs(d)) < GOLD["phase_tolerance"]
+    # unwrapped: no jump between neighbouring cells exceeds pi; span is the coverage
+    assert np.all(np.abs(np.diff(lib["phase"])) <= math.pi)
+    assert abs(lib["span"] - (lib["phase"].max() - lib["phase"].min())) == 0.0
+    assert lib["meta"]["N"] == 6 and lib["meta"]["pol"] == "s" and list(lib["meta"]["period"]) == [0.4, 0.4]
+
+
+def test_build_phase_library_checkn():
+    """CheckN (buildPhaseLibrary.m:7-9): the rerun at CheckN = N changes nothing, and at
+    another N the stored differences are those of the pinned rcwa2d at that N."""
+    w = [0.1, 0.2, 0.3]
+    same = O.build_phase_library(_square_cell, w, 0.94, N=4, CheckN=4)
+    assert same["meta"]["maxPhaseDiff"] == 0.0 and same["meta"]["maxTDiff"] == 0.0
+    lib = O.build_phase_library(_square_cell, w, 0.94, N=4, CheckN=5)
+    o5 = O.rcwa2d([_square_cell(a) for a in w], 0.94, 0, 0, "s", Nx=5)
+    k0 = _idx(o5, 0, 0)
+    np.testing.assert_allclose(lib["meta"]["maxTDiff"], np.max(np.abs(o5["T"][:, k0] - lib["T"])), atol=1e-15)
+    assert np.all(lib["meta"]["phaseDiff"] >= 0) and lib["meta"]["maxPhaseDiff"] < 0.2
+    # p-pol: the co-polarised component is t_x; the square pillar is polarisation independent
+    lp = O.build_phase_library(_square_cell, w, 0.94, N=4, Pol="p")
+    np.testing.assert_allclose(lp["T"], lib["T"], atol=1e-12)
+    np.testing.assert_allclose(lp["phase"], lib["phase"], atol=1e-11)
+
+
+def test_converge_n_ladder_logic():
+    """convergeN.m:20-44 on closed-form sequences: the first N whose change is below Tol,
+    NaN change for the first entry, Progress fractions (k-1)/numel(Ns)."""
+    log = []
+    r = O.converge_n(lambda N: 2.0 ** -N, Tol=0.1, Ns=[1, 2, 3, 4, 5, 6],
+                     Progress=lambda f, msg: log.append((f, msg)))
+    assert r["converged"] and r["N"] == 4 and r["value"][0] == 1 / 16
+    assert r["history"]["N"] == [1, 2, 3, 4]
+    ch = r["history"]["change"]
+    assert math.isnan(ch[0]) and ch[1:] == [0.25, 0.125, 0.0625]
+    assert [f for f, _ in log] == [0, 1 / 6, 2 / 6, 3 / 6] and log[2][1] == "N = 3"
+    # vector results: max abs over the elements, complex included
+    r = O.converge_n(lambda N: np.array([1.0, 1j / N]), Tol=0.3, Ns=[1, 2, 3])
+    assert r["N"] == 3 and abs(r["history"]["change"][1] - 0.5) < 1e-15
+
+
+def test_converge_n_warns_when_not_converged():
+    """tests/TestGUISupport.m:66-67: fun = 1/N, Tol = 1e-9, Ns = 1:3 warns
+    'orf:convergeN:notConverged' and reports the last N."""
+    with pytest.warns(O.OracleWarning) as rec:
+        r = O.converge_n(lambda N: 1 / N, Tol=1e-9, Ns=[1, 2, 3])
+    assert rec[0].message.identifier == "orf:convergeN:notConverged"
+    assert not r["converged"] and r["N"] == 3 and abs(r["value"][0] - 1 / 3) < 1e-16
+    assert r["history"]["N"] == [1, 2, 3]
+
+
+@pytest.mark.slow
+def test_converge_n_two_layer_cell():
+    """tests/TestGUISupport.m:60-65: Ttot of the two-layer cell converges to 1e-3 on
+    Ns = 1:2:15, the last change is below Tol, and the value equals a direct solve."""
+    dev = two_layers()
+    r = O.converge_n(lambda N: O.rcwa2d(dev, 1.0, 0, 0, "s", Nx=N)["Ttot"], Tol=1e-3, Ns=range(1, 16, 2))
+    assert r["converged"] and r["history"]["change"][-1] < 1e-3
+    np.testing.assert_array_equal(r["value"], O.rcwa2d(dev, 1.0, 0, 0, "s", Nx=r["N"])["Ttot"])
+
+
 # ------------------------------------------------------------------ TMM closed forms (tests/TestTMM.m)
 def test_tmm_fresnel_and_energy():
     n1, n2 = 1.0, 1.5
