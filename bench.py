@@ -38,6 +38,11 @@ RADII = np.linspace(0.040, 0.180, 256)
 LAMBDAS = np.linspace(0.900, 0.980, 8)
 
 
+def _progress(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def make_devs(orf, radii):
     mc = orf.materialConst
     sio2, air, asi = mc(1.45, "SiO2"), mc(1.0, "air"), mc(3.6 + 0.01j, "a-Si")
@@ -135,8 +140,10 @@ def run(args, sym, precision="fp64", steps=None, warmup=None):
     prob = orf.Problem(devs, list(LAMBDAS), 0.0, 0.0, ["s", "p"], Nx=N_HARM, Nsample=(256, 256), Symmetry=sym,
                        Precision=precision)
     plan = orf.Plan(prob)
+    _progress(f"plan created (Symmetry={sym}, precision={precision}, solved {plan.solved_size})")
     for _ in range(warmup):
         plan.execute()
+    _progress("warm-up done")
     plan.enable_timing(True)
     stages = {}  # stage -> [total ms, launches] over every timed step (the plan's timer covers one execution)
     t0 = time.perf_counter()
@@ -149,6 +156,7 @@ def run(args, sym, precision="fp64", steps=None, warmup=None):
     t1 = time.perf_counter()
     plan.enable_timing(False)
     solves = len(RADII) * len(LAMBDAS) * 2
+    _progress(f"{steps} timed steps done")
     out = plan.fetch(amplitudes=True)
     plan.close()
     return {"elapsed": t1 - t0, "steps": steps, "stages": stages, "solves": solves, "n": plan.solved_size,
@@ -258,7 +266,9 @@ def main():
               "their own share of the batch with no collective on the critical path; only --gpus 1 is "
               "supported.", file=sys.stderr)
         sys.exit(2)
+    _progress("CPU baseline (oracle sample)")
     cpu, samples = (None, None) if args.no_cpu else cpu_baseline(args.symmetry, args.cpu_workers)
+    _progress("headline plan")
     r = run(args, args.symmetry)
     ms = 1e3 * r["elapsed"] / args.steps
     value = r["solves"] * args.steps / r["elapsed"]
@@ -300,6 +310,7 @@ def main():
         "parity_vs_oracle": parity_vs_oracle(samples, r) if samples else None,
     }
     if not args.quick:
+        _progress("extras")
         line["extras"] = extras(args, r)
     print(json.dumps(line), flush=True)
 
