@@ -58,14 +58,12 @@ def _pillars(radii):
 
 
 # ------------------------------------------------------------------ the benchmark workload
-@pytest.mark.skip(reason="stayed silent for over 180 s on the GPU host (twice) with no cause found in the code; "
-                         "the same comparison runs in bench.py (parity_vs_oracle, 192 pages of the timed plan)")
 def test_phase_library_pm5_full_basis_at_benchmark_size():
     """256 radii 40-180 nm x 8 wavelengths 900-980 nm, +-5 harmonics in the full
     242 x 242 basis, eps 256 x 256, TE and TM, one resident plan (the batch-size
     dependent kernel choices and the per-distinct-page convolution stage are the
-    benchmark's); 16 sampled (radius, wavelength) pages x both polarisations
-    against the oracle."""
+    benchmark's); one sampled radius per wavelength x both polarisations against
+    the oracle, computed serially in this process."""
     radii = np.linspace(0.040, 0.180, 256)
     lams = np.linspace(0.900, 0.980, 8)
     prob = orf.Problem(_pillars(radii), list(lams), 0.0, 0.0, ["s", "p"], Nx=5, Nsample=(256, 256),
@@ -73,23 +71,26 @@ def test_phase_library_pm5_full_basis_at_benchmark_size():
     plan = orf.Plan(prob)
     try:
         assert plan.solved_size == 242 and not plan.symmetric and plan.solve_pages == 2048
+        _say("headline: plan created")
         plan.execute()
         g = plan.fetch()
     finally:
         plan.close()
+    _say("headline: GPU done")
     rng = np.random.default_rng(7)
-    picks = [(int(b), l) for l in range(8) for b in rng.choice(256, size=2, replace=False)]
-    jobs = [([_pillars([radii[b]])[0]], float(lams[l]), pol, {"Nx": 5, "Nsample": (256, 256), "Symmetry": "none"})
-            for b, l in picks for pol in ("s", "p")]
-    outs = _oracle_many(jobs)
     worst = 0.0
-    for k, o in enumerate(outs):
-        (b, l), q = picks[k // 2], k % 2
-        # g: R (M, B, L, P); r (3, M, B, L, P)
-        d = max(np.max(np.abs(g["R"][:, b, l, q] - o["R"][0])), np.max(np.abs(g["T"][:, b, l, q] - o["T"][0])),
-                abs(g["Rtot"][b, l, q] - o["Rtot"][0]), abs(g["Ttot"][b, l, q] - o["Ttot"][0]),
-                np.max(np.abs(g["r"][:, :, b, l, q].T - o["r"][0])), np.max(np.abs(g["t"][:, :, b, l, q].T - o["t"][0])))
-        worst = max(worst, d)
+    for l in range(8):
+        b = int(rng.integers(0, 256))
+        for q, pol in enumerate(("s", "p")):
+            o = _oracle_call(([_pillars([radii[b]])[0]], float(lams[l]), pol,
+                              {"Nx": 5, "Nsample": (256, 256), "Symmetry": "none"}))
+            # g: R (M, B, L, P); r (3, M, B, L, P)
+            d = max(np.max(np.abs(g["R"][:, b, l, q] - o["R"][0])), np.max(np.abs(g["T"][:, b, l, q] - o["T"][0])),
+                    abs(g["Rtot"][b, l, q] - o["Rtot"][0]), abs(g["Ttot"][b, l, q] - o["Ttot"][0]),
+                    np.max(np.abs(g["r"][:, :, b, l, q].T - o["r"][0])),
+                    np.max(np.abs(g["t"][:, :, b, l, q].T - o["t"][0])))
+            worst = max(worst, d)
+        _say(f"headline: wavelength {l} compared (max diff so far {worst:.2e})")
     assert worst < TOL, worst
 
 
