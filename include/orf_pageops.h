@@ -32,6 +32,18 @@ int orf_pagemtimes_f32(orf_ctx* ctx, char opA, char opB, int32_t m, int32_t n, i
 /* X(:,:,p) = A(:,:,p) \ B(:,:,p), A n x n, B n x nrhs (LU, partial pivoting). */
 int orf_pagemldivide(orf_ctx* ctx, int32_t n, int32_t nrhs, const double* A, int64_t strideA,
                      const double* B, double* X, int32_t batch);
+/* orf_pagemldivide on DEVICE buffers without allocating.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  dA (strideA 0 = one matrix for every page, or 2 n^2 doubles),
+ * dB and dX (n x nrhs per page, dX may equal dB) are device pointers.  The LU
+ * copies, pivots and diagonal-block inverses live in a fixed 64 MiB scratch
+ * array per GPU (orf_init sets it up): batch * n^2 * 16 B plus the pivots and
+ * inverses must fit, or ORF_INVALID_ARG.  stream NULL: runs on the context's
+ * stream and returns when done; otherwise it is only enqueued on `stream`.
+ * Calls that overlap on one GPU share the scratch and race. */
+int orf_pagemldivide_noalloc(orf_ctx* ctx, int32_t n, int32_t nrhs, const double* dA, int64_t strideA,
+                             const double* dB, double* dX, int32_t batch, void* stream);
 /* X(:,:,p) = B(:,:,p) / A(:,:,p), B nrows x n. */
 int orf_pagemrdivide(orf_ctx* ctx, int32_t n, int32_t nrows, const double* B, const double* A,
                      int64_t strideA, double* X, int32_t batch);

@@ -69,6 +69,7 @@ EXPORTS = [
     "orf_plan_stage_times", "orf_plan_enable_timing", "orf_plan_memory", "orf_plan_estimate",
     "orf_release_cached_memory", "orf_fields",
     "orf_pagemtimes", "orf_pagemtimes_f32", "orf_pagemldivide", "orf_pagemrdivide", "orf_pageeig",
+    "orf_pagemldivide_noalloc",
 ]
 
 _lib = None
@@ -129,6 +130,9 @@ def load_library():
     lib.orf_pagemldivide.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_double), C.c_int64,
                                      P(C.c_double), P(C.c_double), C.c_int32]
     lib.orf_pagemldivide.restype = C.c_int
+    lib.orf_pagemldivide_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p,
+                                             C.c_void_p, C.c_int32, C.c_void_p]
+    lib.orf_pagemldivide_noalloc.restype = C.c_int
     lib.orf_pagemrdivide.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_double), P(C.c_double),
                                      C.c_int64, P(C.c_double), C.c_int32]
     lib.orf_pagemrdivide.restype = C.c_int

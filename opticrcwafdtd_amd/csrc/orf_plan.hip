@@ -89,6 +89,7 @@ __global__ void scatter_cols_kernel(int, int, const int*, const cplx*, int, cplx
 __global__ void phase_kernel(int, int, int, const double*, const double*, const double*, const double*, double, cplx*);
 __global__ void flux_kernel(int, const cplx*, const cplx*, double, double*);
 __global__ void copy_cols_kernel(long long, const cplx*, cplx*);
+__global__ void page_copy_kernel(long long, const cplx*, long long, cplx*, long long);
 __global__ void scale_kernel(long long, cplx, cplx*);
 __global__ void output_kernel(int, int, int, int, int, const DevSolve*, const DevPage*, const DevSector*,
                               const DevSource*, const double*, const double*, const double*, const cplx*,
@@ -2225,6 +2226,36 @@ static void plan_fields(orf_plan& P, int nx, const double* x, int ny, const doub
   HIPCHECK(hipStreamSynchronize(st));
 }
 
+// ====================================================================== allocation-free solves
+// Device scratch of the allocation-free entry point: each GPU the library runs
+// on holds its own copy of this array until the process ends.  Host code never
+// passes `scratch` itself to a kernel or a copy (in host code the name is a
+// stand-in, not a device address): orf_init looks up each device's address
+// once, with that device current, and the entry point uses that address.
+static constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;  // [device] -> address of that device's scratch
+
+static void scratch_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (g_scratch_ptr[device]) return;
+  void* ptr = nullptr;
+  HIPCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) fail(ORF_DEVICE_ERROR, "no device address for the scratch array");
+  g_scratch_ptr[device] = ptr;
+}
+
+static void* scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= (int)g_scratch_ptr.size() || !g_scratch_ptr[device])
+    fail(ORF_INVALID_ARG, "context not initialised by orf_init");
+  return g_scratch_ptr[device];
+}
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
 // ====================================================================== C ABI
 #define ORF_TRY(...)                         \
   try {                                      \
@@ -2256,6 +2287,7 @@ int orf_init(int device, orf_ctx** out) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
       fail(ORF_DEVICE_ERROR, std::string("device is ") + prop.gcnArchName + ", the library is compiled for gfx950 only");
     HIPCHECK(hipSetDevice(device));
+    scratch_setup(device);
     orf_ctx* c = new orf_ctx();
     c->device = device;
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -2495,6 +2527,44 @@ int orf_pagemldivide(orf_ctx* ctx, int32_t n, int32_t nrhs, const double* A, int
     KCHECK();
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     HIPCHECK(hipMemcpy(X, b.p, (size_t)batch * n * nrhs * 16, hipMemcpyDeviceToHost));
+  })
+}
+
+int orf_pagemldivide_noalloc(orf_ctx* ctx, int32_t n, int32_t nrhs, const double* dA, int64_t strideA,
+                             const double* dB, double* dX, int32_t batch, void* stream) {
+  ORF_TRY({
+    if (!ctx || n <= 0 || nrhs <= 0 || batch <= 0) fail(ORF_INVALID_ARG, "bad sizes");
+    if (!dA || !dB || !dX) fail(ORF_INVALID_ARG, "device pointer is NULL");
+    if (strideA != 0 && strideA != 2 * (int64_t)n * n) fail(ORF_INVALID_ARG, "strideA must be 0 or 2 n^2 doubles");
+    const size_t n2 = (size_t)n * n;
+    // scratch layout: LU copies of A, diagonal-block inverses, pivots
+    const size_t luB = align256((size_t)batch * n2 * sizeof(cplx));
+    const size_t dvB = align256(sizeof(cplx) * (size_t)orf_dinv_stride(n) * batch);
+    const size_t ipB = align256(sizeof(int) * (size_t)n * batch);
+    if (luB + dvB + ipB > kScratchBytes)
+      fail(ORF_INVALID_ARG, "batch x n^2 too large for the allocation-free solve: needs " +
+                                std::to_string((luB + dvB + ipB) >> 10) + " KiB of scratch, " +
+                                std::to_string(kScratchBytes >> 10) + " KiB available");
+    HIPCHECK(hipSetDevice(ctx->device));
+    unsigned char* base = (unsigned char*)scratch_of(ctx->device);
+    cplx* lu = (cplx*)base;
+    cplx* dv = (cplx*)(base + luB);
+    int* ip = (int*)(base + luB + dvB);
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    // every scratch word the solve reads is written first (LU copy, pivots and
+    // inverses by the factorisation), so nothing needs clearing
+    for (int b0 = 0; b0 < batch; b0 += 65535) {
+      const int bb = std::min(65535, batch - b0);
+      page_copy_kernel<<<dim3((unsigned)((n2 + 255) / 256), bb), 256, 0, st>>>(
+          (long long)n2, (const cplx*)dA + (strideA ? (size_t)b0 * n2 : 0), strideA ? (long long)n2 : 0,
+          lu + (size_t)b0 * n2, (long long)n2);
+    }
+    if (dX != dB)
+      HIPCHECK(hipMemcpyAsync(dX, dB, (size_t)batch * n * nrhs * sizeof(cplx), hipMemcpyDeviceToDevice, st));
+    orf_zgetrf(st, n, lu, n, n2, ip, nullptr, batch, dv);
+    orf_zgetrs(st, 'L', n, nrhs, lu, n, n2, ip, dv, (cplx*)dX, n, (long long)n * nrhs, batch);
+    KCHECK();
+    if (!stream) HIPCHECK(hipStreamSynchronize(st));
   })
 }
 

@@ -657,6 +657,14 @@ __global__ void gather_pages_kernel(long long elems, const int* __restrict__ gro
   dst[(size_t)sp * sDst + e] = src[(size_t)groupOf[sp] * sSrc + e];
 }
 
+// dst page p = src page p (sSrc = 0: the one src matrix to every page), pages along blockIdx.y
+__global__ void page_copy_kernel(long long elems, const cplx* __restrict__ src, long long sSrc,
+                                 cplx* __restrict__ dst, long long sDst) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= elems) return;
+  dst[(size_t)blockIdx.y * sDst + e] = src[(size_t)blockIdx.y * sSrc + e];
+}
+
 // ------------------------------------------------------------------ outputs
 // One block per (solve page, source).  r_xy = S11 c, t_xy = S21 c (full basis
 // or UE (S c_r) in a parity sector); z from transversality; R, T per order;
