@@ -1028,18 +1028,12 @@ __global__ __launch_bounds__(256) void ztrsm_right_kernel(int n, int nrows, int 
 }  // namespace
 
 static int g_lds_limit = 65536;
-static int g_qr_bulges = 4;      // 1 = the per-rotation kernel at every size
-static int g_qr_small_n = 96;    // up to this size the per-rotation LDS kernel is used
-static int g_qr_aed = 2;         // early deflation: 0 off, 1 reordering, 2 bottom only, 3 whole blocks only
-
-void orf_set_qr_params(int bulges, int small_n) {
-  if (bulges > 0) g_qr_bulges = bulges;
-  if (small_n >= 0) g_qr_small_n = small_n;
-}
-
-void orf_set_qr_aed(int mode) {
-  if (mode >= 0 && mode <= 3) g_qr_aed = mode;
-}
+// QR kernel choice (fixed by measurement on the +-5 batch): multi-bulge passes of
+// MBQ_NB bulges; up to this size the per-rotation LDS kernel serves small batches;
+// early deflation from the bottom of the window
+static constexpr int g_qr_bulges = 4;      // 1 would select the per-rotation kernel at every size
+static constexpr int g_qr_small_n = 96;
+static constexpr int g_qr_aed = 2;         // 0 off, 1 reordering, 2 bottom only, 3 whole blocks only
 
 void orf_set_lds_limit(int bytes) { g_lds_limit = bytes; }
 int orf_lds_limit() { return g_lds_limit; }
@@ -2966,7 +2960,6 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
       const size_t stat = 64 * 1024;
       const size_t lim2 = lim + 20 * 1024 - stat;
       const int hIn = bytes <= lim2, zIn = 2 * bytes <= lim2;
-      // ORF_QR_AED: 0 off, 1 with reordering, 2 from the bottom (default), 3 whole blocks only
       const int mode = g_qr_aed == 1 ? 0 : (g_qr_aed == 2 ? 1 : (g_qr_aed == 3 ? 2 : 3));
       schur_qr_aed_kernel<MBQ_NB, MBQ_SEG, MBQ_OCC, MBQ_GRP, MBQ_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
           n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn, mode);

@@ -44,5 +44,3 @@ bool orf_eig_inverse(hipStream_t st, int n, const cplx* Z, const cplx* Y, const 
 // per-rotation LDS kernel is used.
 bool orf_small_inverse(hipStream_t st, int n, const cplx* A, long long sA, cplx* X, long long sX,
                        long long count);
-void orf_set_qr_params(int bulges, int small_n);
-void orf_set_qr_aed(int mode);

@@ -2294,9 +2294,6 @@ int orf_init(int device, orf_ctx** out) {
     // leave room for each kernel's static LDS
     const int lim = (int)std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024) - 20 * 1024;
     orf_set_lds_limit(std::max(lim, 32 * 1024));
-    if (const char* e = getenv("ORF_QR_BULGES")) orf_set_qr_params(atoi(e), -1);
-    if (const char* e = getenv("ORF_QR_SMALL_N")) orf_set_qr_params(0, atoi(e));
-    if (const char* e = getenv("ORF_QR_AED")) orf_set_qr_aed(atoi(e));
     *out = c;
   })
 }

@@ -25,7 +25,11 @@
 #include "orf_rcwa.h"
 
 static orf_ctx* g_ctx = NULL;
-static mxArray* g_exception = NULL;  /* MException thrown by the progress function */
+/* MException thrown by the progress function.  It is a temporary of the current
+ * MEX call (the callback runs inside orf_rcwa2d, inside mexFunction) and is
+ * rethrown before mexFunction returns, so it is not made persistent: MATLAB
+ * frees it with the call's other temporaries. */
+static mxArray* g_exception = NULL;
 
 static void release_ctx(void) {
   if (g_ctx) orf_release(g_ctx);
@@ -61,7 +65,6 @@ static int progress_cb(double fraction, const char* message, void* user) {
   mxDestroyArray(args[2]);
   if (err) {
     g_exception = err;
-    mexMakeArrayPersistent(g_exception);
     return 1;
   }
   return 0;

@@ -123,30 +123,81 @@ def shapeCross(armWidth: float, armLength: float, mat: dict,
 
 
 # ---------------------------------------------------------------- slanted walls
-def _polygon_offset_inward(xy: np.ndarray, d: float) -> np.ndarray:
-    """Move every edge of a simple polygon inward by d (miter joins)."""
-    xy = np.asarray(xy, float)
-    area = 0.5 * np.sum(xy[:, 0] * np.roll(xy[:, 1], -1) - np.roll(xy[:, 0], -1) * xy[:, 1])
-    if area < 0:                     # make counter-clockwise
-        xy = xy[::-1]
+def _signed_area(xy: np.ndarray) -> float:
+    return 0.5 * float(np.sum(xy[:, 0] * np.roll(xy[:, 1], -1) - np.roll(xy[:, 0], -1) * xy[:, 1]))
+
+
+def _segments_meet(p1, p2, q1, q2) -> bool:
+    """Closed segments p1-p2 and q1-q2 share a point (touching and collinear overlap included)."""
+    def orient(a, b, c):
+        return (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0])
+
+    def on(a, b, c):  # c on segment a-b, given collinear
+        return min(a[0], b[0]) <= c[0] <= max(a[0], b[0]) and min(a[1], b[1]) <= c[1] <= max(a[1], b[1])
+    d1, d2 = orient(q1, q2, p1), orient(q1, q2, p2)
+    d3, d4 = orient(p1, p2, q1), orient(p1, p2, q2)
+    if d1 * d2 < 0 and d3 * d4 < 0:
+        return True
+    return ((d1 == 0 and on(q1, q2, p1)) or (d2 == 0 and on(q1, q2, p2)) or
+            (d3 == 0 and on(p1, p2, q1)) or (d4 == 0 and on(p1, p2, q2)))
+
+
+def _is_simple(xy: np.ndarray) -> bool:
     k = xy.shape[0]
-    lines = []
+    for i in range(k):
+        for j in range(i + 2, k):
+            if i == 0 and j == k - 1:
+                continue  # neighbours through the closing edge
+            if _segments_meet(xy[i], xy[(i + 1) % k], xy[j], xy[(j + 1) % k]):
+                return False
+    return True
+
+
+def _polygon_offset_inward(xy: np.ndarray, d: float):
+    """Move every edge of a simple polygon inward by d with miter joins.
+
+    Edges whose offset runs backwards (they shrink to nothing before d) are
+    dropped and the neighbours re-joined, so a convex polygon gives exactly
+    the intersection of the inward-shifted half-planes.  Returns the vertices,
+    None when nothing is left, or raises orf:slantedLayers:split when the
+    offset outline crosses itself, i.e. the region pinches into pieces
+    (slantedLayers.m:54-57).
+    """
+    xy = np.asarray(xy, float)
+    if _signed_area(xy) < 0:         # make counter-clockwise
+        xy = xy[::-1]
+    edges = []                       # (point on the offset line, unit direction)
+    k = xy.shape[0]
     for i in range(k):
         p, q = xy[i], xy[(i + 1) % k]
         t = q - p
-        t = t / np.hypot(t[0], t[1])
-        nrm = np.array([-t[1], t[0]])          # inward normal for CCW
-        lines.append((p + d * nrm, t))
-    out = np.empty_like(xy)
-    for i in range(k):
-        (p1, t1), (p2, t2) = lines[i - 1], lines[i]
-        den = t1[0] * t2[1] - t1[1] * t2[0]
-        if abs(den) < 1e-15:
-            out[i] = p2
+        L = math.hypot(t[0], t[1])
+        if L == 0.0:
             continue
-        s = ((p2[0] - p1[0]) * t2[1] - (p2[1] - p1[1]) * t2[0]) / den
-        out[i] = p1 + s * t1
-    return out
+        t = t / L
+        edges.append((p + d * np.array([-t[1], t[0]]), t))   # inward normal for CCW
+    while len(edges) >= 3:
+        k = len(edges)
+        out = np.empty((k, 2))
+        for i in range(k):           # vertex i joins edge i-1 and edge i
+            (p1, t1), (p2, t2) = edges[i - 1], edges[i]
+            den = t1[0] * t2[1] - t1[1] * t2[0]
+            if abs(den) < 1e-15:
+                out[i] = p2
+                continue
+            s_ = ((p2[0] - p1[0]) * t2[1] - (p2[1] - p1[1]) * t2[0]) / den
+            out[i] = p1 + s_ * t1
+        inverted = [i for i in range(k) if float(np.dot(out[(i + 1) % k] - out[i], edges[i][1])) <= 0.0]
+        if not inverted:
+            if _signed_area(out) <= 0.0:
+                return None
+            if not _is_simple(out):
+                raise OrfError("orf:slantedLayers:split", "The polygon splits into pieces at this sidewall angle.")
+            return out
+        # drop the edge that vanished first: the one whose offset is most reversed
+        worst = min(inverted, key=lambda i: float(np.dot(out[(i + 1) % k] - out[i], edges[i][1])))
+        edges.pop(worst)
+    return None
 
 
 def _shrink(shape: dict, d: float) -> list:
@@ -159,14 +210,7 @@ def _shrink(shape: dict, d: float) -> list:
         return [shapeRect(sx, sy, shape["mat"], shape["center"])] if (sx > 0 and sy > 0) else []
     if t == "polygon":
         v = _polygon_offset_inward(shape["vertices"], d)
-        a = 0.5 * abs(np.sum(v[:, 0] * np.roll(v[:, 1], -1) - np.roll(v[:, 0], -1) * v[:, 1]))
-        # orientation flip means the polygon collapsed
-        a0 = 0.5 * np.sum(shape["vertices"][:, 0] * np.roll(shape["vertices"][:, 1], -1)
-                          - np.roll(shape["vertices"][:, 0], -1) * shape["vertices"][:, 1])
-        a1 = 0.5 * np.sum(v[:, 0] * np.roll(v[:, 1], -1) - np.roll(v[:, 0], -1) * v[:, 1])
-        if a > 0 and np.sign(a0) == np.sign(a1):
-            return [shapePolygon(v, shape["mat"], shape["center"])]
-        return []
+        return [] if v is None else [shapePolygon(v, shape["mat"], shape["center"])]
     raise OrfError("orf:slantedLayers:shape", f'Unsupported shape "{t}".')
 
 
