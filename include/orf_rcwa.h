@@ -150,6 +150,11 @@ int orf_plan_info(const orf_plan* plan, int32_t* solved_size, int32_t* symmetric
 int orf_plan_stage_times(const orf_plan* plan, int32_t max_stages, const char** names,
                          double* total_ms, int32_t* launches, int32_t* n_stages);
 void orf_plan_enable_timing(orf_plan* plan, int32_t on);
+/* Build extension: with on = 1, each batch chunk's stage chain is captured into a
+ * HIP graph on the next execute and replayed on the executes after it (one
+ * graph launch per chunk instead of one launch per kernel).  Executes with a
+ * progress callback or stage timing, and kept-mode (field) plans, run eagerly. */
+void orf_plan_enable_graphs(orf_plan* plan, int32_t on);
 
 /* ------------------------------------------------------------------ fields
  * orf.rcwaFields(dev, lambda0, theta, phi, pol, x, y, z) -- rcwaFields.m:1-21.

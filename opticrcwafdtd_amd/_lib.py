@@ -69,7 +69,7 @@ EXPORTS = [
     "orf_plan_stage_times", "orf_plan_enable_timing", "orf_plan_memory", "orf_plan_estimate",
     "orf_release_cached_memory", "orf_fields",
     "orf_pagemtimes", "orf_pagemtimes_f32", "orf_pagemldivide", "orf_pagemrdivide", "orf_pageeig",
-    "orf_pagemldivide_noalloc",
+    "orf_pagemldivide_noalloc", "orf_plan_enable_graphs",
 ]
 
 _lib = None
@@ -111,6 +111,8 @@ def load_library():
     lib.orf_plan_stage_times.restype = C.c_int
     lib.orf_plan_enable_timing.argtypes = [C.c_void_p, C.c_int32]
     lib.orf_plan_enable_timing.restype = None
+    lib.orf_plan_enable_graphs.argtypes = [C.c_void_p, C.c_int32]
+    lib.orf_plan_enable_graphs.restype = None
     lib.orf_plan_memory.argtypes = [C.c_void_p, P(C.c_int64)]
     lib.orf_plan_memory.restype = C.c_int
     lib.orf_plan_estimate.argtypes = [P(orf_problem), C.c_int32, P(C.c_int64)]

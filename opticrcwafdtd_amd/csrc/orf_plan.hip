@@ -476,8 +476,32 @@ struct orf_plan {
   double kzi0 = 1.0;
   zc epsInc0 = 1.0, epsSub0 = 1.0;  // half-space permittivities of context 0
   StageTimer timer;
+  // Index and parameter tables of the execution (solve-page lists, context maps,
+  // thicknesses, distinct-page maps, star-tree pairings): the same on every
+  // execute, so each is uploaded once (on the first execute) and the k-th table
+  // request of later executes returns the k-th device copy.  Execution then
+  // issues only kernels and device-to-device copies, which is what a captured
+  // graph replays.
+  struct Table {
+    DBuf* buf = nullptr;
+    std::vector<unsigned char> host;
+  };
+  std::vector<Table> tables;
+  size_t tableCursor = 0;
+  // eigensolver status words of the current chunk (one region per eig stage),
+  // checked after the chunk has run
+  int infoCursor = 0, infoCap = 0;
+  std::vector<std::pair<int, int>> pendingInfo;  // (offset, count)
+  // captured chunks (orf_plan_enable_graphs): one graph per chunk
+  bool useGraphs = false, capturing = false, graphsStale = false;
+  std::vector<hipGraphExec_t> graphExec;
+  std::vector<std::vector<std::pair<int, int>>> graphInfo;  // per chunk: its status regions
+  std::vector<int> chunkCursor;                               // per chunk: its first table
   ~orf_plan() {
     if (st) (void)hipStreamSynchronize(st);  // no block returns to the pool while in use
+    for (auto g : graphExec)
+      if (g) (void)hipGraphExecDestroy(g);
+    for (auto& t : tables) delete t.buf;
     for (auto* b : dPagesByLayer) delete b;
     for (auto* b : dSlotEps) delete b;
     for (auto* b : dSlotInv) delete b;
@@ -582,7 +606,9 @@ static Work make_work(orf_plan& P, int Cp) {
   w.Vv = (cplx*)get(cs * 8 * P.nr * Cs);
   w.dinv = (cplx*)get(cs * std::max((size_t)orf_dinv_stride(P.nr) * Cs, (size_t)orf_dinv_stride(P.M) * Cp));
   w.ipiv = (int*)get(sizeof(int) * (size_t)std::max(P.nr, P.M) * std::max(Cs, Cp));
-  w.info = (int*)get(sizeof(int) * (size_t)std::max(Cs, Cp) * 64);
+  // one status region per eigensolver stage of a chunk (one per layer at most)
+  P.infoCap = std::max(Cs, Cp) * std::max(64, P.nLayers + 1);
+  w.info = (int*)get(sizeof(int) * (size_t)P.infoCap);
   w.solves = (DevSolve*)get(sizeof(DevSolve) * Cs);
   w.groupOf = (int*)get(sizeof(int) * Cs);
   w.colIdx = (int*)get(sizeof(int) * 2 * Cs);
@@ -624,12 +650,45 @@ static void bulk_gemm(const orf_plan& P, char opA, char opB, int M, int N, int K
     orf_zgemm(P.st, opA, opB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
 }
 
-static void check_info(orf_plan& P, const int* dinfo, int count, const char* what) {
-  std::vector<int> h(count);
-  HIPCHECK(hipMemcpyAsync(h.data(), dinfo, sizeof(int) * count, hipMemcpyDeviceToHost, P.st));
-  HIPCHECK(hipStreamSynchronize(P.st));
-  for (int i = 0; i < count; ++i)
-    if (h[i] != 0) fail(ORF_DEVICE_ERROR, std::string(what) + " failed on page " + std::to_string(i));
+// Device copy of an execution table (see orf_plan::Table): uploaded on first use,
+// reused while the host contents are unchanged.
+template <class T>
+static T* table(orf_plan& P, const std::vector<T>& v) {
+  const size_t bytes = v.size() * sizeof(T);
+  const size_t k = P.tableCursor++;
+  if (k >= P.tables.size()) P.tables.emplace_back();
+  orf_plan::Table& e = P.tables[k];
+  if (e.buf && e.host.size() == bytes && (bytes == 0 || std::memcmp(e.host.data(), v.data(), bytes) == 0))
+    return e.buf->as<T>();
+  if (P.capturing) fail(ORF_DEVICE_ERROR, "execution tables changed while a graph was captured");
+  if (e.buf) P.graphsStale = true;  // a captured graph may read the old copy
+  if (!e.buf) e.buf = new DBuf();
+  e.buf->alloc(std::max<size_t>(bytes, 16));
+  e.host.assign((const unsigned char*)v.data(), (const unsigned char*)v.data() + bytes);
+  if (bytes) HIPCHECK(hipMemcpyAsync(e.buf->p, e.host.data(), bytes, hipMemcpyHostToDevice, P.st));
+  return e.buf->as<T>();
+}
+
+// Next status region of the chunk's eigensolver stages (checked by check_pending_info).
+static int* next_info(orf_plan& P, Work& w, int count) {
+  if (P.infoCursor + count > P.infoCap) fail(ORF_DEVICE_ERROR, "status buffer too small");
+  int* p = w.info + P.infoCursor;
+  P.pendingInfo.emplace_back(P.infoCursor, count);
+  P.infoCursor += count;
+  return p;
+}
+
+static void check_pending_info(orf_plan& P, Work& w) {
+  for (auto& pi : P.pendingInfo) {
+    std::vector<int> h(pi.second);
+    HIPCHECK(hipMemcpyAsync(h.data(), w.info + pi.first, sizeof(int) * pi.second, hipMemcpyDeviceToHost, P.st));
+    HIPCHECK(hipStreamSynchronize(P.st));
+    for (int i = 0; i < pi.second; ++i)
+      if (h[i] != 0)
+        fail(ORF_DEVICE_ERROR, "eigensolver (QR iteration did not converge) failed on solve page " + std::to_string(i));
+  }
+  P.pendingInfo.clear();
+  P.infoCursor = 0;
 }
 
 // Layer S-matrix (smatrixLayer.m:6-18): W, V, lam per solve page; W0 = I; V0 per
@@ -1600,13 +1659,13 @@ static void patterned_modes(orf_plan& P, Work& w, const DevPage* pages, const cp
   }
   {
     STAGE("eig");
-    orf_zgeev_timed(st, nr, w.A0, nr, n2, w.mu, w.Wm, nr, n2, w.Zw, w.Yw, w.Hv, w.info, Cs,
+    int* info = next_info(P, w, Cs);
+    orf_zgeev_timed(st, nr, w.A0, nr, n2, w.mu, w.Wm, nr, n2, w.Zw, w.Yw, w.Hv, info, Cs,
                     [&](const char* nm, int phase) {
                       if (phase == 0) P.timer.begin(nm, st); else P.timer.end(st);
                     }, w.vnorm);
     KCHECK();
   }
-  check_info(P, w.info, Cs, "eigensolver (QR iteration did not converge)");
   {
     STAGE("modes");
     lambda_kernel<<<g1((long long)nr * Cs), 256, 0, st>>>(nr, Cs, w.mu, w.lam);
@@ -1671,7 +1730,7 @@ static void finish_chunk(orf_plan& P, Work& w, int p0, int Cp, cplx* const SA[4]
       if (P.reduced) cols[sp] = P.sectorSrcCol[sp % nSec];
       else { cols[2 * sp] = P.i0; cols[2 * sp + 1] = P.M + P.i0; }
     }
-    HIPCHECK(hipMemcpyAsync(w.colIdx, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, st));
+    w.colIdx = table(P, cols);
   }
   if (direct) finish_direct(P, w, Cs, P.dPagesByLayer[0]->as<DevPage>() + p0, SA, SB, ncol, w.colIdx, R, T);
   else if (single) finish_single(P, w, Cs, SA, SB[0], SB[2], ncol, w.colIdx, R, T);
@@ -1735,20 +1794,20 @@ static void execute_chunk_layers(orf_plan& P, Work& w, int p0, int Cp) {
   };
   for (int a = 0; a < Lp; ++a) fill(patL[a], a, 0, true);
   for (int a = 0; a < Lh; ++a) fill(homL[a], a, CsP, false);
-  HIPCHECK(hipMemcpyAsync(w.solves, sv.data(), sizeof(DevSolve) * CsAll, hipMemcpyHostToDevice, st));
-  HIPCHECK(hipMemcpyAsync(w.groupOf, grp.data(), sizeof(int) * CsAll, hipMemcpyHostToDevice, st));
-  HIPCHECK(hipMemcpyAsync(w.kL, kL.data(), sizeof(double) * CsAll, hipMemcpyHostToDevice, st));
-  HIPCHECK(hipMemcpyAsync(w.idxA, unified.data(), sizeof(int) * unified.size(), hipMemcpyHostToDevice, st));
+  w.solves = table(P, sv);
+  w.groupOf = table(P, grp);
+  w.kL = table(P, kL);
+  w.idxA = table(P, unified);
   if (CpP) {
-    HIPCHECK(hipMemcpyAsync(w.pagesA, pgA.data(), sizeof(DevPage) * CpP, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(w.seA, seA.data(), sizeof(cplx) * CpP * ns, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(w.siA, siA.data(), sizeof(cplx) * CpP * ns, hipMemcpyHostToDevice, st));
+    w.pagesA = table(P, pgA);
+    w.seA = table(P, seA);
+    w.siA = table(P, siA);
     patterned_modes(P, w, w.pagesA, w.seA, w.siA, CpP, CsP, nullptr);
   }
   if (CsH) {
     STAGE("modes");
-    HIPCHECK(hipMemcpyAsync(w.pagesH, pgH.data(), sizeof(DevPage) * Lh * Cp, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(w.epsSolve, epsH.data(), sizeof(cplx) * CsH, hipMemcpyHostToDevice, st));
+    w.pagesH = table(P, pgH);
+    w.epsSolve = table(P, epsH);
     homog_modes_kernel<<<g1(n2, CsH), 256, 0, st>>>(M, nr, P.reduced, w.solves + CsP, w.pagesH,
                                                     P.dSectors.as<DevSector>(), P.dKxy.as<double>(), w.epsSolve,
                                                     w.Wm + (size_t)CsP * n2, w.Vm + (size_t)CsP * n2, n2,
@@ -1774,8 +1833,8 @@ static void execute_chunk_layers(orf_plan& P, Work& w, int p0, int Cp) {
       ev[q] = 2 * k * CpS + r;
       od[q] = (2 * k + 1) * CpS + r;
     }
-    HIPCHECK(hipMemcpyAsync(w.idxA, ev.data(), sizeof(int) * ev.size(), hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(w.idxB, od.data(), sizeof(int) * od.size(), hipMemcpyHostToDevice, st));
+    w.idxA = table(P, ev);
+    w.idxB = table(P, od);
     cplx* A[4];
     cplx* B[4];
     cplx* out[4];
@@ -1794,7 +1853,6 @@ static void execute_chunk_layers(orf_plan& P, Work& w, int p0, int Cp) {
     for (int k = 0; k < 4; ++k) cur[k] = out[k];
     inSa = !inSa;
     Kc = K + (Kc & 1);
-    HIPCHECK(hipStreamSynchronize(st));  // ev / od are rebuilt on the host for the next level
   }
   // standard solve order for the chunk's last stages
   std::vector<DevSolve> sv0(CpS);
@@ -1804,8 +1862,8 @@ static void execute_chunk_layers(orf_plan& P, Work& w, int p0, int Cp) {
       sv0[m * nSec + sc] = DevSolve{m, sc};
       grp0[m * nSec + sc] = P.pageCtx[p0 + m] * nSec + sc;
     }
-  HIPCHECK(hipMemcpyAsync(w.solves, sv0.data(), sizeof(DevSolve) * CpS, hipMemcpyHostToDevice, st));
-  HIPCHECK(hipMemcpyAsync(w.groupOf, grp0.data(), sizeof(int) * CpS, hipMemcpyHostToDevice, st));
+  w.solves = table(P, sv0);
+  w.groupOf = table(P, grp0);
   // result of S_ref * (S_1 * ... * S_L) * S_trn on the source columns; cur holds the
   // layers' composite (in one of the Sa / Sb sets), the regions go to Sn and the other set
   cplx* Ref[4];
@@ -1816,26 +1874,26 @@ static void execute_chunk_layers(orf_plan& P, Work& w, int p0, int Cp) {
   }
   (void)G;
   finish_chunk(P, w, p0, Cp, Ref, Trn, w.X1, w.X2, cur);
-  HIPCHECK(hipStreamSynchronize(st));
 }
 
 // ---------------------------------------------------------------- execution
-static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
+// The GPU work of one chunk: kernels and device-to-device copies only (tables
+// come from the plan's table cache), so that it can be captured into a graph.
+// With a progress callback it waits for the device after every layer and calls
+// back; a nonzero return cancels.
+static void execute_chunk(orf_plan& P, int ch, orf_progress_fn progress, void* user) {
   hipStream_t st = P.st;
   const int npages = P.nPages;
   const int Cmax = P.chunkPages;
   Work& w = P.work;
-  if (P.timer.on) P.timer.reset();
-  if (P.keepModes && P.kept.empty())
-    for (int j = 0; j < P.nLayers; ++j) P.kept.push_back(new orf_plan::Kept());
   const int nChunks = (npages + Cmax - 1) / Cmax;
   const int M = P.M, nr = P.nr, nSec = P.nSec;
   const long long n2 = (long long)nr * nr;
-  for (int ch = 0; ch < nChunks; ++ch) {
+  {
     const int p0 = ch * Cmax, Cp = std::min(Cmax, npages - p0), Cs = Cp * nSec;
     if (P.layerParallel && !progress) {
       execute_chunk_layers(P, w, p0, Cp);
-      continue;
+      return;
     }
     // solve pages: sp = mode * nSec + sector
     std::vector<DevSolve> sv(Cs);
@@ -1845,8 +1903,8 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         sv[m * nSec + s] = DevSolve{m, s};
         grp[m * nSec + s] = P.pageCtx[p0 + m] * nSec + s;
       }
-    HIPCHECK(hipMemcpyAsync(w.solves, sv.data(), sizeof(DevSolve) * Cs, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(w.groupOf, grp.data(), sizeof(int) * Cs, hipMemcpyHostToDevice, st));
+    w.solves = table(P, sv);
+    w.groupOf = table(P, grp);
     const long long G = (long long)P.nCtx * nSec;
     // with layers: L = S_1 * ... * S_L accumulates in Sa and the regions join at the
     // end (finish_stack); without: Sa = S_ref and the column star with S_trn
@@ -1889,10 +1947,10 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         ConvUnique cu;
         cu.Cu = (int)uPg.size();
         if (cu.Cu < Cp) {
-          HIPCHECK(hipMemcpyAsync(w.pagesA, uPg.data(), sizeof(DevPage) * uPg.size(), hipMemcpyHostToDevice, st));
-          HIPCHECK(hipMemcpyAsync(w.seA, uSe.data(), sizeof(cplx) * uSe.size(), hipMemcpyHostToDevice, st));
-          HIPCHECK(hipMemcpyAsync(w.siA, uSi.data(), sizeof(cplx) * uSi.size(), hipMemcpyHostToDevice, st));
-          HIPCHECK(hipMemcpyAsync(w.idxA, uMap.data(), sizeof(int) * Cp, hipMemcpyHostToDevice, st));
+          w.pagesA = table(P, uPg);
+          w.seA = table(P, uSe);
+          w.siA = table(P, uSi);
+          w.idxA = table(P, uMap);
           cu.uPages = w.pagesA;
           cu.uSe = w.seA;
           cu.uSi = w.siA;
@@ -1903,11 +1961,10 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         STAGE("modes");
         std::vector<cplx> eps(Cs);
         for (int sp = 0; sp < Cs; ++sp) eps[sp] = C(P.layerBgEps[j][p0 + sp / nSec]);
-        HIPCHECK(hipMemcpyAsync(w.epsSolve, eps.data(), sizeof(cplx) * Cs, hipMemcpyHostToDevice, st));
+        w.epsSolve = table(P, eps);
         homog_modes_kernel<<<g1(n2, Cs), 256, 0, st>>>(M, nr, P.reduced, w.solves, pages, P.dSectors.as<DevSector>(),
                                                        P.dKxy.as<double>(), w.epsSolve, w.Wm, w.Vm, n2, w.lam);
         KCHECK();
-        HIPCHECK(hipStreamSynchronize(st));
       }
       // layer S-matrix against the gap (W0 = I, V0 per context and sector)
       std::vector<double> kL(Cs);
@@ -1915,7 +1972,7 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
         const int pg = p0 + sp / nSec;
         kL[sp] = P.k0[P.pageCtx[pg]] * P.thick[(size_t)j * P.B + P.pageB[pg]];
       }
-      HIPCHECK(hipMemcpyAsync(w.kL, kL.data(), sizeof(double) * Cs, hipMemcpyHostToDevice, st));
+      w.kL = table(P, kL);
       if (!P.directSingle)
         gather_pages_kernel<<<g1(n2, Cs), 256, 0, st>>>(n2, w.groupOf, P.dV0.as<cplx>(), n2, w.X2, n2);
       else if (P.reduced)  // finish_direct: U = V0^-1 V
@@ -1950,8 +2007,8 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
           for (int k = 0; k < 4; ++k) std::swap(w.Sa[k], w.Sn[k]);
         }
       }
-      HIPCHECK(hipStreamSynchronize(st));
       if (progress) {
+        HIPCHECK(hipStreamSynchronize(st));
         const double frac = (double)(ch * P.nLayers + j + 1) / (double)(nChunks * P.nLayers);
         char msg[64];
         snprintf(msg, sizeof msg, "layer %d/%d", j + 1, P.nLayers);
@@ -1965,8 +2022,72 @@ static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
     else if (P.nLayers == 1 && !P.keepModes) finish_chunk(P, w, p0, Cp, w.Sn, w.Sb, w.X1, w.X2, nullptr, true);
     else finish_chunk(P, w, p0, Cp, w.Sn, w.Sb, w.X1, w.X2, w.Sa);
     if (progress && P.nLayers == 0) {
+      HIPCHECK(hipStreamSynchronize(st));
       if (progress((double)(ch + 1) / nChunks, "done", user)) fail(ORF_CANCELLED, "cancelled by the progress callback");
     }
+  }
+}
+
+static void plan_execute(orf_plan& P, orf_progress_fn progress, void* user) {
+  hipStream_t st = P.st;
+  const int nChunks = (P.nPages + P.chunkPages - 1) / P.chunkPages;
+  if (P.timer.on) P.timer.reset();
+  if (P.keepModes && P.kept.empty())
+    for (int j = 0; j < P.nLayers; ++j) P.kept.push_back(new orf_plan::Kept());
+  P.tableCursor = 0;
+  P.pendingInfo.clear();
+  P.infoCursor = 0;
+  // graphs: replay the captured chunks; capture them on the first execute after the
+  // tables exist (progress callbacks, stage timing and kept modes run eagerly)
+  if (P.graphsStale) {
+    HIPCHECK(hipStreamSynchronize(st));
+    for (auto& g : P.graphExec)
+      if (g) (void)hipGraphExecDestroy(g);
+    P.graphExec.clear();
+    P.graphInfo.clear();
+    P.graphsStale = false;
+  }
+  const bool graphs = P.useGraphs && !progress && !P.timer.on && !P.keepModes && !P.tables.empty();
+  if (graphs && (int)P.graphExec.size() < nChunks) {
+    P.graphExec.assign(nChunks, nullptr);
+    P.graphInfo.assign(nChunks, {});
+  }
+  if ((int)P.chunkCursor.size() < nChunks) P.chunkCursor.assign(nChunks, -1);
+  for (int ch = 0; ch < nChunks; ++ch) {
+    // each chunk walks its own run of the table cache
+    if (P.chunkCursor[ch] >= 0) P.tableCursor = (size_t)P.chunkCursor[ch];
+    else P.chunkCursor[ch] = (int)P.tableCursor;
+    if (graphs) {
+      if (!P.graphExec[ch]) {
+        hipGraph_t g = nullptr;
+        HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        P.capturing = true;
+        try {
+          execute_chunk(P, ch, nullptr, nullptr);
+        } catch (...) {
+          P.capturing = false;
+          hipGraph_t dead = nullptr;
+          (void)hipStreamEndCapture(st, &dead);
+          if (dead) (void)hipGraphDestroy(dead);
+          throw;
+        }
+        P.capturing = false;
+        HIPCHECK(hipStreamEndCapture(st, &g));
+        hipGraphExec_t ge = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHECK(e);
+        P.graphExec[ch] = ge;
+        P.graphInfo[ch] = P.pendingInfo;
+      } else {
+        P.pendingInfo = P.graphInfo[ch];  // status regions the captured chunk writes
+      }
+      HIPCHECK(hipGraphLaunch(P.graphExec[ch], st));
+    } else {
+      execute_chunk(P, ch, progress, user);
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    check_pending_info(P, P.work);
   }
   HIPCHECK(hipStreamSynchronize(st));
   P.timer.collect();
@@ -2430,6 +2551,10 @@ int orf_fields(orf_ctx* ctx, const orf_problem* p, int32_t nx, const double* x, 
     g_err = e.what();
     return ORF_DEVICE_ERROR;
   }
+}
+
+void orf_plan_enable_graphs(orf_plan* plan, int32_t on) {
+  if (plan) plan->useGraphs = on != 0;
 }
 
 void orf_plan_enable_timing(orf_plan* plan, int32_t on) {

@@ -246,6 +246,10 @@ class Plan:
         rc = self._lib.orf_plan_execute(self.handle, fn, None)
         _raise(rc, state["exc"])
 
+    def enable_graphs(self, on=True):
+        """Capture each chunk's stage chain into a HIP graph and replay it on later executes."""
+        self._lib.orf_plan_enable_graphs(self.handle, 1 if on else 0)
+
     def enable_timing(self, on=True):
         self._lib.orf_plan_enable_timing(self.handle, 1 if on else 0)
 

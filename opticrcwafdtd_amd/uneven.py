@@ -66,6 +66,7 @@ class UnevenBatch:
                            Factorization=Factorization, Pages=pages)
             self.classes.append({"key": key, "cells": idx, "problem": prob, "plan": None})
         self._lock = threading.Lock()
+        self.graphs = False
 
     @staticmethod
     def _work(key, count):
@@ -80,13 +81,25 @@ class UnevenBatch:
     def _run_class(self, c):
         if c["plan"] is None:
             c["plan"] = Plan(c["problem"])
+            if self.graphs:
+                c["plan"].enable_graphs(True)
         c["plan"].execute()
+
+    def enable_graphs(self, on=True):
+        """Capture every class plan's stage chain into HIP graphs and replay them
+        (orf_plan_enable_graphs) from the next execute on."""
+        self.graphs = bool(on)
+        for c in self.classes:
+            if c["plan"] is not None:
+                c["plan"].enable_graphs(self.graphs)
 
     def create(self):
         """Create every plan (uploads, rasterisation, half-space S-matrices, workspace)."""
         for c in self.classes:
             if c["plan"] is None:
                 c["plan"] = Plan(c["problem"])
+                if self.graphs:
+                    c["plan"].enable_graphs(True)
 
     def execute(self):
         """Run all classes, ``workers`` at a time, largest first."""

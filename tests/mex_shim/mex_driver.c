@@ -120,8 +120,15 @@ int main(int argc, char** argv) {
     print_arr("t", (const double*)mxGetComplexDoubles(t), 2 * (int)mxGetNumberOfElements(t));
     printf(", \"persistent\": %d}\n", fake_mex_persistent);
   } else {
+    char esc[1024];
+    size_t k = 0;
+    for (const char* c = fake_mex_err_msg; *c && k + 2 < sizeof esc; ++c) {  /* JSON string escape */
+      if (*c == '"' || *c == '\\') esc[k++] = '\\';
+      esc[k++] = *c;
+    }
+    esc[k] = 0;
     printf("{\"status\": \"error\", \"error_id\": \"%s\", \"message\": \"%s\", \"rethrown\": %s, \"persistent\": %d}\n",
-           fake_mex_err_id, fake_mex_err_msg, rc == 2 ? "true" : "false", fake_mex_persistent);
+           fake_mex_err_id, esc, rc == 2 ? "true" : "false", fake_mex_persistent);
   }
   fake_mex_unload();
   return 0;
