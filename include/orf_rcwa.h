@@ -167,6 +167,22 @@ void orf_plan_enable_graphs(orf_plan* plan, int32_t on);
 int orf_fields(orf_ctx* ctx, const orf_problem* p, int32_t nx, const double* x, int32_t ny, const double* y,
                int32_t nz, const double* z, double* E, double* H, double* flux, orf_result* out);
 
+/* ------------------------------------------------------------------ tuning (build extension)
+ * Kernel choices of the batched eigensolver per matrix size and batch class
+ * (pages >= 512 or fewer), as measured by opticrcwafdtd_amd/autotune.py and
+ * stored in opticrcwafdtd_amd/tuning_table.txt ("kind n large_batch value"
+ * per line).  Kinds: 0 Schur QR (0 per-rotation with H in LDS, 1 multishift),
+ * 1 Hessenberg reduction (0 rank-1 updates, 8 or 12 panel width), 2
+ * eigenvectors (16 or 32 rows per lane, 0 one wave per vector).  A lookup
+ * uses the entry of the largest tuned size <= n; choices a size cannot take
+ * (LDS or register limits) fall back to the built-in rule.  orf_tune_force
+ * (value -1 clears) overrides the table for every size (tuning runs).
+ * Process-wide, not per context. */
+int orf_tune_set(int32_t kind, int32_t n, int32_t large_batch, int32_t value);
+int orf_tune_force(int32_t kind, int32_t value);
+void orf_tune_clear(void);
+int orf_tune_load(const char* path);
+
 /* ------------------------------------------------------------------ memory (build extension)
  * Admission by device footprint for request loops.  orf_plan_memory: bytes a
  * created plan holds.  orf_plan_estimate: an upper bound before creation,

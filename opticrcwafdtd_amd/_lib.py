@@ -70,7 +70,9 @@ EXPORTS = [
     "orf_release_cached_memory", "orf_fields",
     "orf_pagemtimes", "orf_pagemtimes_f32", "orf_pagemldivide", "orf_pagemrdivide", "orf_pageeig",
     "orf_pagemldivide_noalloc", "orf_plan_enable_graphs",
+    "orf_tune_set", "orf_tune_force", "orf_tune_clear", "orf_tune_load",
 ]
+TUNING_TABLE = os.path.join(_HERE, "tuning_table.txt")
 
 _lib = None
 _ctx = None
@@ -111,6 +113,14 @@ def load_library():
     lib.orf_plan_stage_times.restype = C.c_int
     lib.orf_plan_enable_timing.argtypes = [C.c_void_p, C.c_int32]
     lib.orf_plan_enable_timing.restype = None
+    lib.orf_tune_set.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    lib.orf_tune_set.restype = C.c_int
+    lib.orf_tune_force.argtypes = [C.c_int32, C.c_int32]
+    lib.orf_tune_force.restype = C.c_int
+    lib.orf_tune_clear.argtypes = []
+    lib.orf_tune_clear.restype = None
+    lib.orf_tune_load.argtypes = [C.c_char_p]
+    lib.orf_tune_load.restype = C.c_int
     lib.orf_plan_enable_graphs.argtypes = [C.c_void_p, C.c_int32]
     lib.orf_plan_enable_graphs.restype = None
     lib.orf_plan_memory.argtypes = [C.c_void_p, P(C.c_int64)]
@@ -159,6 +169,9 @@ def context():
             rc = lib.orf_init(0, C.byref(h))
             if rc != ORF_OK:
                 raise RuntimeError(f"orf_init failed ({rc}): {last_error()}")
+            # the autotuned kernel choices (opticrcwafdtd_amd/autotune.py) when present
+            if os.path.exists(TUNING_TABLE) and lib.orf_tune_load(TUNING_TABLE.encode()) != ORF_OK:
+                raise RuntimeError(f"orf_tune_load failed: {last_error()}")
             _ctx = h
         return _ctx
 

@@ -23,6 +23,9 @@
 //                 the triangular factor by back substitution (LAPACK ztrevc's
 //                 small-pivot rule), back-transformed by one ZGEMM and
 //                 normalised to unit 2-norm.
+#include <mutex>
+#include <vector>
+
 #include "orf_common.h"
 #include "orf_internal.h"
 
@@ -1037,6 +1040,52 @@ static constexpr int g_qr_aed = 2;         // 0 off, 1 reordering, 2 bottom only
 
 void orf_set_lds_limit(int bytes) { g_lds_limit = bytes; }
 int orf_lds_limit() { return g_lds_limit; }
+
+// Tuning table of the eigensolver's kernel choices (written by the autotuner,
+// loaded by the host): per choice kind, entries (n, batch class, value); a
+// lookup takes the entry of the largest tuned n <= n in the batch class
+// (pages >= 512 or fewer), else the smallest tuned n; -1 = no entry, the
+// built-in rule applies.  A forced value overrides the table (tuning runs).
+namespace {
+struct TuneEntry {
+  int n, large, value;
+};
+std::mutex g_tune_mu;
+std::vector<TuneEntry> g_tune[ORF_TUNE_KINDS];
+int g_tune_force[ORF_TUNE_KINDS] = {-1, -1, -1};
+}  // namespace
+
+int orf_tune_value(int kind, int n, int batch) {
+  if (kind < 0 || kind >= ORF_TUNE_KINDS) return -1;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (g_tune_force[kind] >= 0) return g_tune_force[kind];
+  const int large = batch >= 512;
+  int below = -1, belowN = -1, smallest = -1, smallestN = 1 << 30;
+  for (const TuneEntry& e : g_tune[kind]) {
+    if (e.large != large) continue;
+    if (e.n <= n && e.n > belowN) { belowN = e.n; below = e.value; }
+    if (e.n < smallestN) { smallestN = e.n; smallest = e.value; }
+  }
+  return below >= 0 ? below : smallest;
+}
+
+void orf_tune_put(int kind, int n, int large, int value) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (TuneEntry& e : g_tune[kind])
+    if (e.n == n && e.large == large) { e.value = value; return; }
+  g_tune[kind].push_back(TuneEntry{n, large, value});
+}
+
+void orf_tune_reset() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (auto& v : g_tune) v.clear();
+  for (int& f : g_tune_force) f = -1;
+}
+
+void orf_tune_override(int kind, int value) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_force[kind] = value;
+}
 
 long long orf_dinv_stride(int n) { return 2LL * ((n + 15) / 16) * 256; }
 
@@ -2876,16 +2925,20 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
   // in LDS: one rank-1 update pair per column; otherwise panels of reflectors with
   // Level-3 trailing updates (panel reflectors and the current column in LDS, Y in Zwork)
   const size_t pan12 = (size_t)n * 13 * sizeof(cplx), pan8 = (size_t)n * 9 * sizeof(cplx);
-  if (use_lds || n < 32)
-    hessenberg_noq_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, Vh, tau, use_lds);
-  else if (pan12 <= (size_t)g_lds_limit)
+  const bool ok12 = n >= 32 && pan12 <= (size_t)g_lds_limit, ok8 = n >= 32 && pan8 <= (size_t)g_lds_limit;
+  // built-in rule: rank-1 updates in LDS when the page fits (or n < 32), else 12- or 8-wide panels
+  int blockedNB = (use_lds || n < 32) ? 0 : (ok12 ? 12 : (ok8 ? 8 : 0));
+  {
+    const int tv = orf_tune_value(ORF_TUNE_HESS, n, batch);
+    if (tv == 0 || (tv == 12 && ok12) || (tv == 8 && ok8)) blockedNB = tv;
+  }
+  if (blockedNB == 12)
     hessenberg_blocked_kernel<12><<<batch, 256, pan12, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
-  else if (pan8 <= (size_t)g_lds_limit)
+  else if (blockedNB == 8)
     hessenberg_blocked_kernel<8><<<batch, 256, pan8, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
   else
-    hessenberg_noq_kernel<<<batch, 256, 0, st>>>(n, A, lda, sA, Vh, tau, 0);
+    hessenberg_noq_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, Vh, tau, use_lds);
   // Z = Q of the reduction, so that the QR's accumulated Z is the full Schur basis
-  const int blockedNB = (use_lds || n < 32) ? 0 : (pan12 <= (size_t)g_lds_limit ? 12 : (pan8 <= (size_t)g_lds_limit ? 8 : 0));
   if (blockedNB && sZ >= (long long)n * n && (long long)3 * n * blockedNB + blockedNB * blockedNB <= (long long)n * n) {
     // Q = P_0 P_1 ... with P_p = I - V_p T_p V_p^H, applied to I from the last panel on;
     // Q stays the identity outside its trailing (n-k0-1) block, so each panel is three
@@ -2953,7 +3006,13 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
     const size_t lim = (size_t)g_lds_limit;
     const int nbul = g_qr_bulges;
     const bool few = batch < 512;
-    if (bytes <= lim && (nbul == 1 || n < 32 || (n <= g_qr_small_n && few))) {
+    bool perRotation = bytes <= lim && (nbul == 1 || n < 32 || (n <= g_qr_small_n && few));
+    {
+      const int tv = orf_tune_value(ORF_TUNE_QR, n, batch);  // 0 per-rotation (H in LDS), 1 multishift
+      if (tv == 1) perRotation = false;
+      else if (tv == 0 && bytes <= lim) perRotation = true;
+    }
+    if (perRotation) {
       const int zIn = 2 * bytes <= lim;
       schur_qr_kernel<<<batch, 256, (1 + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w, info, 1, zIn);
     } else {
@@ -2970,9 +3029,15 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
   tb("eig_vectors");
   {
     const dim3 gb((n + TV_EB - 1) / TV_EB, batch);
-    if (n <= 256)
+    // 16 or 32 rows per lane (n <= 256 / 512), or one wave per vector (n <= 768)
+    int rpt = n <= 256 ? 16 : (n <= 512 ? 32 : 0);
+    {
+      const int tv = orf_tune_value(ORF_TUNE_TREVC, n, batch);
+      if ((tv == 16 && n <= 256) || (tv == 32 && n <= 512) || tv == 0) rpt = tv;
+    }
+    if (rpt == 16)
       trevc_blocked_kernel<16><<<gb, 256, 0, st>>>(n, A, lda, sA, Ywork, n, sZ);
-    else if (n <= 512)
+    else if (rpt == 32)
       trevc_blocked_kernel<32><<<gb, 256, 0, st>>>(n, A, lda, sA, Ywork, n, sZ);
     else
       trevc_kernel<<<g2, 256, 0, st>>>(n, A, lda, sA, Ywork, n, sZ);

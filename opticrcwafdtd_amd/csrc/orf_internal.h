@@ -40,7 +40,16 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
 bool orf_eig_inverse_ok(int n);
 bool orf_eig_inverse(hipStream_t st, int n, const cplx* Z, const cplx* Y, const double* nrm, cplx* X,
                      cplx* dinv, int batch);
-// Schur QR dispatch knobs: bulges per multishift pass and the size below which the
-// per-rotation LDS kernel is used.
+// Inverses of many small matrices (n <= 22, one wave each); false (nothing
+// launched) when n is larger.
 bool orf_small_inverse(hipStream_t st, int n, const cplx* A, long long sA, cplx* X, long long sX,
                        long long count);
+
+// Eigensolver kernel choices the autotuner may set (values: QR 0 per-rotation,
+// 1 multishift; Hessenberg 0 rank-1 updates, 8 / 12 panel width; eigenvectors
+// 16 / 32 rows per lane, 0 one wave per vector).
+enum { ORF_TUNE_QR = 0, ORF_TUNE_HESS = 1, ORF_TUNE_TREVC = 2, ORF_TUNE_KINDS = 3 };
+int orf_tune_value(int kind, int n, int batch);
+void orf_tune_put(int kind, int n, int large, int value);
+void orf_tune_reset();
+void orf_tune_override(int kind, int value);

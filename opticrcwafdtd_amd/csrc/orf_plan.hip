@@ -2553,6 +2553,42 @@ int orf_fields(orf_ctx* ctx, const orf_problem* p, int32_t nx, const double* x, 
   }
 }
 
+int orf_tune_set(int32_t kind, int32_t n, int32_t large_batch, int32_t value) {
+  if (kind < 0 || kind >= ORF_TUNE_KINDS || n <= 0) return ORF_INVALID_ARG;
+  orf_tune_put(kind, n, large_batch ? 1 : 0, value);
+  return ORF_OK;
+}
+
+int orf_tune_force(int32_t kind, int32_t value) {
+  if (kind < 0 || kind >= ORF_TUNE_KINDS) return ORF_INVALID_ARG;
+  orf_tune_override(kind, value);
+  return ORF_OK;
+}
+
+void orf_tune_clear(void) { orf_tune_reset(); }
+
+int orf_tune_load(const char* path) {
+  ORF_TRY({
+    if (!path) fail(ORF_INVALID_ARG, "path is NULL");
+    FILE* f = std::fopen(path, "r");
+    if (!f) fail(ORF_INVALID_ARG, std::string("cannot open ") + path);
+    char line[256];
+    int lineNo = 0;
+    while (std::fgets(line, sizeof line, f)) {
+      ++lineNo;
+      if (line[0] == '#' || line[0] == '\n') continue;
+      int kind, n, large, value;
+      if (std::sscanf(line, "%d %d %d %d", &kind, &n, &large, &value) != 4 || kind < 0 || kind >= ORF_TUNE_KINDS ||
+          n <= 0) {
+        std::fclose(f);
+        fail(ORF_INVALID_ARG, std::string(path) + ": bad entry on line " + std::to_string(lineNo));
+      }
+      orf_tune_put(kind, n, large ? 1 : 0, value);
+    }
+    std::fclose(f);
+  })
+}
+
 void orf_plan_enable_graphs(orf_plan* plan, int32_t on) {
   if (plan) plan->useGraphs = on != 0;
 }

@@ -56,3 +56,19 @@ def test_invalid_pol_rejected_on_host():
     with pytest.raises(orf.OrfError) as e:
         orf.Problem(g, 1.0, 0, 0, "x", Nx=1)
     assert e.value.identifier == "rcwa:pol"
+
+
+def test_tuning_table_loads_and_rejects_bad_lines(tmp_path):
+    lib = _lib.load_library()
+    good = tmp_path / "t.txt"
+    good.write_text("# comment\n0 61 0 1\n1 242 1 12   # trailing note\n2 98 0 16\n")
+    assert lib.orf_tune_load(str(good).encode()) == _lib.ORF_OK
+    bad = tmp_path / "b.txt"
+    bad.write_text("0 61 0 1\nnonsense\n")
+    assert lib.orf_tune_load(str(bad).encode()) == _lib.ORF_INVALID_ARG
+    assert "line 2" in _lib.last_error()
+    assert lib.orf_tune_set(7, 10, 0, 1) == _lib.ORF_INVALID_ARG
+    lib.orf_tune_clear()
+    if os.path.exists(_lib.TUNING_TABLE):   # the committed table parses
+        assert lib.orf_tune_load(_lib.TUNING_TABLE.encode()) == _lib.ORF_OK
+        lib.orf_tune_clear()
