@@ -172,17 +172,20 @@ def extras(args, full):
     ex["symmetric_basis"] = {"solves_per_s": round(a["solves"] * a["steps"] / a["elapsed"], 1),
                              "ms_per_step": round(1e3 * a["elapsed"] / a["steps"], 3), "solved_matrix": a["n"],
                              "max_abs_dT_vs_full": float(np.max(np.abs(a["T"] - full["T"])))}
-    m = run(args, "none", precision="mixed", steps=args.steps, warmup=1)
-    dT, dR = np.abs(m["T"] - full["T"]), np.abs(m["R"] - full["R"])
-    ex["mixed_precision"] = {
-        "solves_per_s": round(m["solves"] * m["steps"] / m["elapsed"], 1),
-        "ms_per_step": round(1e3 * m["elapsed"] / m["steps"], 3),
-        "speedup_vs_fp64": round((full["elapsed"] / full["steps"]) / (m["elapsed"] / m["steps"]), 4),
-        "max_abs_dT": float(dT.max()), "mean_abs_dT": float(dT.mean()),
-        "max_abs_dR": float(dR.max()), "mean_abs_dR": float(dR.mean()),
-        "within_1e-6": bool(max(dT.max(), dR.max()) <= 1e-6),
-        "what": "FP32-MFMA convolution-matrix and per-layer star-product GEMMs; eigensolve, propagators, "
-                "every LU/solve and the last star product FP64"}
+    what = {"mixed": "FP32-MFMA convolution-matrix and star-product GEMMs (operands rounded to FP32, FP32 "
+                     "accumulation); eigensolve, propagators and every LU / solve FP64",
+            "split": "the same products as hi + lo FP32 split operands (three FP32 MFMA products, lo*lo dropped), "
+                     "FP64 accumulation across 4-deep MFMA steps; eigensolve, propagators and LU / solves FP64"}
+    for prec in ("mixed", "split"):
+        m = run(args, "none", precision=prec, steps=args.steps, warmup=1)
+        dT, dR = np.abs(m["T"] - full["T"]), np.abs(m["R"] - full["R"])
+        ex["mixed_precision" if prec == "mixed" else "split_precision"] = {
+            "solves_per_s": round(m["solves"] * m["steps"] / m["elapsed"], 1),
+            "ms_per_step": round(1e3 * m["elapsed"] / m["steps"], 3),
+            "speedup_vs_fp64": round((full["elapsed"] / full["steps"]) / (m["elapsed"] / m["steps"]), 4),
+            "max_abs_dT": float(dT.max()), "mean_abs_dT": float(dT.mean()),
+            "max_abs_dR": float(dR.max()), "mean_abs_dR": float(dR.mean()),
+            "within_1e-6": bool(max(dT.max(), dR.max()) <= 1e-6), "what": what[prec]}
     cells = uneven.random_cells(4096, seed=20261203)
     ub = uneven.UnevenBatch(cells, pol=["s", "p"], Nsample=(256, 256), Symmetry="auto")
     t0 = time.perf_counter()

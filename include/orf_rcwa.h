@@ -58,7 +58,10 @@ enum { ORF_POL_S = 0, ORF_POL_P = 1, ORF_POL_JONES = 2 };
 
 enum { ORF_FACT_LI = 0, ORF_FACT_LAURENT = 1 };          /* Factorization         */
 enum { ORF_SYM_AUTO = 0, ORF_SYM_NONE = 1 };             /* Symmetry              */
-enum { ORF_PREC_FP64 = 0, ORF_PREC_MIXED = 1 };          /* build extension       */
+/* build extension: the bulk products (convolution matrices, star products) in
+ * FP32 MFMA (MIXED), or as hi + lo FP32 split products with FP64 accumulation
+ * across K stages (SPLIT); eigensolver, propagators and LU solves stay FP64 */
+enum { ORF_PREC_FP64 = 0, ORF_PREC_MIXED = 1, ORF_PREC_SPLIT = 2 };
 
 /*
  * One batched solve: B structures sharing the period, the incident and

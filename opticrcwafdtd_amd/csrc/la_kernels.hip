@@ -194,8 +194,13 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(
 // 64x64 tile is accumulated in FP32, and alpha, beta and C stay FP64.  The
 // FP32 MFMA output layout differs from the FP64 one: lane l, register v holds
 // row 4 (l / 16) + v, column l % 16 of each 16x16 block.
+// SPLIT: each operand is split into hi = fl32(x) and lo = fl32(x - hi), the
+// product is Ah Bh + Ah Bl + Al Bh (three FP32 MFMA products, the lo * lo term
+// dropped), accumulated in FP32 over one 4-deep MFMA step and in FP64 across
+// steps: the operands' rounding to FP32 no longer enters, only the FP32
+// rounding of each step's products and sums.
 typedef float v4f __attribute__((ext_vector_type(4)));
-template <int OPA, int OPB>
+template <int OPA, int OPB, bool SPLIT>
 __global__ __launch_bounds__(256) void zgemm32_mfma_kernel(
     int M, int N, int K, cplx alpha, const cplx* __restrict__ A, int lda, long long sA,
     const cplx* __restrict__ B, int ldb, long long sB, cplx beta, cplx* __restrict__ C, int ldc,
@@ -203,6 +208,8 @@ __global__ __launch_bounds__(256) void zgemm32_mfma_kernel(
   constexpr int PAD = 4;
   __shared__ float Asr[GT_K][GT_M + PAD], Asi[GT_K][GT_M + PAD];
   __shared__ float Bsr[GT_K][GT_N + PAD], Bsi[GT_K][GT_N + PAD];
+  __shared__ float Alr[SPLIT ? GT_K : 1][GT_M + PAD], Ali[SPLIT ? GT_K : 1][GT_M + PAD];
+  __shared__ float Blr[SPLIT ? GT_K : 1][GT_N + PAD], Bli[SPLIT ? GT_K : 1][GT_N + PAD];
   int page, tile;
   xcd_page_tile((long long)blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z),
                 gridDim.x * gridDim.y, gridDim.z, page, tile);
@@ -213,12 +220,16 @@ __global__ __launch_bounds__(256) void zgemm32_mfma_kernel(
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
   v4f accR[2][2], accI[2][2];
+  double dR[SPLIT ? 2 : 1][SPLIT ? 2 : 1][4], dI[SPLIT ? 2 : 1][SPLIT ? 2 : 1][4];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       accR[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
       accI[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+      if (SPLIT)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dR[SPLIT ? i : 0][SPLIT ? j : 0][v] = dI[SPLIT ? i : 0][SPLIT ? j : 0][v] = 0.0;
     }
   for (int k0 = 0; k0 < K; k0 += GT_K) {
 #pragma unroll
@@ -230,6 +241,10 @@ __global__ __launch_bounds__(256) void zgemm32_mfma_kernel(
       cplx v = (gr < M && gk < K) ? load_op<OPA>(A, lda, gr, gk) : cmk(0.0, 0.0);
       Asr[k][r] = (float)v.x;
       Asi[k][r] = (float)v.y;
+      if (SPLIT) {
+        Alr[SPLIT ? k : 0][r] = (float)(v.x - (double)(float)v.x);
+        Ali[SPLIT ? k : 0][r] = (float)(v.y - (double)(float)v.y);
+      }
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -240,6 +255,10 @@ __global__ __launch_bounds__(256) void zgemm32_mfma_kernel(
       cplx v = (gk < K && gc < N) ? load_op<OPB>(B, ldb, gk, gc) : cmk(0.0, 0.0);
       Bsr[k][c] = (float)v.x;
       Bsi[k][c] = (float)v.y;
+      if (SPLIT) {
+        Blr[SPLIT ? k : 0][c] = (float)(v.x - (double)(float)v.x);
+        Bli[SPLIT ? k : 0][c] = (float)(v.y - (double)(float)v.y);
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -262,6 +281,41 @@ __global__ __launch_bounds__(256) void zgemm32_mfma_kernel(
           accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], bi[j], accI[i][j], 0, 0, 0);
           accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], br[j], accI[i][j], 0, 0, 0);
         }
+      if (SPLIT) {  // cross terms hi * lo + lo * hi
+        float alr[2], ali[2], blr[2], bli[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          alr[i] = Alr[SPLIT ? kr : 0][wm + 16 * i + rr];
+          ali[i] = Ali[SPLIT ? kr : 0][wm + 16 * i + rr];
+          blr[i] = Blr[SPLIT ? kr : 0][wn + 16 * i + rr];
+          bli[i] = Bli[SPLIT ? kr : 0][wn + 16 * i + rr];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], blr[j], accR[i][j], 0, 0, 0);
+            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(alr[i], br[j], accR[i][j], 0, 0, 0);
+            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[i], bli[j], accR[i][j], 0, 0, 0);
+            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ali[i], bi[j], accR[i][j], 0, 0, 0);
+            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], bli[j], accI[i][j], 0, 0, 0);
+            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(alr[i], bi[j], accI[i][j], 0, 0, 0);
+            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], blr[j], accI[i][j], 0, 0, 0);
+            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ali[i], br[j], accI[i][j], 0, 0, 0);
+          }
+        // this 4-deep step's FP32 sums into the FP64 accumulators
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              dR[SPLIT ? i : 0][SPLIT ? j : 0][v] += (double)accR[i][j][v];
+              dI[SPLIT ? i : 0][SPLIT ? j : 0][v] += (double)accI[i][j][v];
+              accR[i][j][v] = 0.f;
+              accI[i][j][v] = 0.f;
+            }
+      }
     }
     __syncthreads();
   }
@@ -275,7 +329,8 @@ __global__ __launch_bounds__(256) void zgemm32_mfma_kernel(
         int gr = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
         int gc = n0 + wn + 16 * j + (lane & 15);
         if (gr < M && gc < N) {
-          cplx acc = cmk((double)accR[i][j][v], (double)accI[i][j][v]);
+          cplx acc = SPLIT ? cmk(dR[SPLIT ? i : 0][SPLIT ? j : 0][v], dI[SPLIT ? i : 0][SPLIT ? j : 0][v])
+                           : cmk((double)accR[i][j][v], (double)accI[i][j][v]);
           cplx out = cmul(alpha, acc);
           size_t idx = (size_t)gr + (size_t)gc * ldc;
           if (!beta0) out = cadd(out, cmul(beta, C[idx]));
@@ -287,10 +342,10 @@ __global__ __launch_bounds__(256) void zgemm32_mfma_kernel(
 
 void orf_zgemm_f32(hipStream_t st, char opA, char opB, int M, int N, int K, cplx alpha, const cplx* A,
                    int lda, long long sA, const cplx* B, int ldb, long long sB, cplx beta, cplx* C,
-                   int ldc, long long sC, int batch) {
+                   int ldc, long long sC, int batch, bool split) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   while (batch > 65535) {  // grid z limit
-    orf_zgemm_f32(st, opA, opB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, 65535);
+    orf_zgemm_f32(st, opA, opB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, 65535, split);
     A += (size_t)sA * 65535;
     B += (size_t)sB * 65535;
     C += (size_t)sC * 65535;
@@ -299,11 +354,15 @@ void orf_zgemm_f32(hipStream_t st, char opA, char opB, int M, int N, int K, cplx
   dim3 grid((N + GT_N - 1) / GT_N, (M + GT_M - 1) / GT_M, batch);
   int oa = opA == 'N' ? 0 : (opA == 'T' ? 1 : 2);
   int ob = opB == 'N' ? 0 : (opB == 'T' ? 1 : 2);
-#define ORF_GEMM_CASE(a, b)                                                                          \
-  if (oa == a && ob == b) {                                                                          \
-    zgemm32_mfma_kernel<a, b><<<grid, 256, 0, st>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, \
-                                                    ldc, sC);                                        \
-    return;                                                                                          \
+#define ORF_GEMM_CASE(a, b)                                                                                 \
+  if (oa == a && ob == b) {                                                                                 \
+    if (split)                                                                                              \
+      zgemm32_mfma_kernel<a, b, true><<<grid, 256, 0, st>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, \
+                                                            ldc, sC);                                       \
+    else                                                                                                    \
+      zgemm32_mfma_kernel<a, b, false><<<grid, 256, 0, st>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,  \
+                                                             C, ldc, sC);                                   \
+    return;                                                                                                 \
   }
   ORF_GEMM_CASE(0, 0) ORF_GEMM_CASE(0, 1) ORF_GEMM_CASE(0, 2)
   ORF_GEMM_CASE(1, 0) ORF_GEMM_CASE(1, 1) ORF_GEMM_CASE(1, 2)

@@ -14,10 +14,11 @@ void orf_zgemm(hipStream_t st, char opA, char opB, int M, int N, int K, cplx alp
                int lda, long long sA, const cplx* B, int ldb, long long sB, cplx beta, cplx* C,
                int ldc, long long sC, int batch);
 // The same product with FP32 MFMA products and FP32 accumulation (operands rounded
-// to complex64 on load; alpha, beta and C in FP64).
+// to complex64 on load; alpha, beta and C in FP64); split: hi + lo FP32 operands,
+// three FP32 products, FP64 accumulation across 4-deep MFMA steps.
 void orf_zgemm_f32(hipStream_t st, char opA, char opB, int M, int N, int K, cplx alpha, const cplx* A,
                    int lda, long long sA, const cplx* B, int ldb, long long sB, cplx beta, cplx* C,
-                   int ldc, long long sC, int batch);
+                   int ldc, long long sC, int batch, bool split = false);
 // LU with partial pivoting, in place; ipiv [n x batch] 0-based; info [batch] (0 ok, k+1 zero pivot).
 // dinv (optional, orf_dinv_stride(n) complex per page) receives the inverses of the 16x16
 // diagonal blocks of L and U, which orf_zgetrs uses for its MFMA block solves.

@@ -644,8 +644,9 @@ static const cplx ONE = {1.0, 0.0}, ZERO = {0.0, 0.0}, MONE = {-1.0, 0.0};
 static void bulk_gemm(const orf_plan& P, char opA, char opB, int M, int N, int K, cplx alpha, const cplx* A,
                       int lda, long long sA, const cplx* B, int ldb, long long sB, cplx beta, cplx* C, int ldc,
                       long long sC, int batch) {
-  if (P.precision == ORF_PREC_MIXED)
-    orf_zgemm_f32(P.st, opA, opB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+  if (P.precision == ORF_PREC_MIXED || P.precision == ORF_PREC_SPLIT)
+    orf_zgemm_f32(P.st, opA, opB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
+                  P.precision == ORF_PREC_SPLIT);
   else
     orf_zgemm(P.st, opA, opB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
 }
@@ -1170,7 +1171,8 @@ static void plan_create(orf_ctx* ctx, const orf_problem* pr, orf_plan& P) {
   P.phi = pr->phi;
   P.factorization = pr->factorization;
   P.precision = pr->precision;
-  if (P.precision != ORF_PREC_FP64 && P.precision != ORF_PREC_MIXED) fail(ORF_INVALID_ARG, "unknown precision");
+  if (P.precision != ORF_PREC_FP64 && P.precision != ORF_PREC_MIXED && P.precision != ORF_PREC_SPLIT)
+    fail(ORF_INVALID_ARG, "unknown precision");
   if (pr->nMaterials <= 0 || !pr->matEps) fail(ORF_INVALID_ARG, "material table missing");
   auto matEps = [&](int mat, int l) -> zc {
     if (mat < 0 || mat >= pr->nMaterials) fail(ORF_INVALID_ARG, "material index out of range");

@@ -166,7 +166,9 @@ class Problem:
         pr.factorization = 0 if Factorization == "li" else 1
         pr.symmetry = 0 if Symmetry == "auto" else 1
         pr.gapEps = float(GapEps)
-        pr.precision = 0 if Precision == "fp64" else 1
+        if Precision not in ("fp64", "mixed", "split"):
+            raise OrfError("orf:invalidArgument", 'Precision must be "fp64", "mixed" or "split"')
+        pr.precision = {"fp64": 0, "mixed": 1, "split": 2}[Precision]
         pr.batchSize = int(BatchSize)
         if self.pages is not None:
             if self.pages.shape[0] == 0:
