@@ -7,7 +7,7 @@ three eigensolver stages on a real plan (a-Si pillar cells, 256 x 256
 permittivity grid, the plan's own stage timers):
 
 * Hessenberg reduction: rank-1 updates (0) or 8 / 12-wide panels;
-* Schur QR: per-rotation kernel with H in LDS (0, n <= 94) or multishift
+* Schur QR: per-rotation kernel with H in LDS (0, n <= 101) or multishift
   passes with early deflation (1);
 * eigenvectors: 16 / 32 rows per lane (n <= 256 / 512) or one wave per
   vector (0);
@@ -32,12 +32,12 @@ from .rcwa import Plan, Problem
 from .structure import makeLayer, makeStructure, materialConst, shapeCircle
 
 KINDS = {0: ("eig_schur_qr", "qr"), 1: ("eig_hessenberg", "hessenberg"), 2: ("eig_vectors", "vectors")}
-LDS_LIMIT = 143360          # bytes of LDS the library's kernels take (160 KiB less their static share)
+LDS_BYTES = 163840          # a workgroup's LDS (160 KiB); the per-rotation QR keeps H there
 
 
 def candidates(kind: int, n: int) -> list:
     if kind == 0:
-        return [0, 1] if n * n * 16 <= LDS_LIMIT else [1]
+        return [0, 1] if n * n * 16 + 64 <= LDS_BYTES else [1]
     if kind == 1:
         return [0] + ([12, 8] if n >= 32 else [])
     return ([16] if n <= 256 else []) + ([32] if n <= 512 else []) + [0]

@@ -1282,8 +1282,7 @@ constexpr int TREVC_SLOTS_H = 12;  // columns up to 768 rows in registers
 __global__ __launch_bounds__(256) void hessenberg_noq_kernel(int n, cplx* __restrict__ Ag, int lda,
                                                              long long sA, cplx* __restrict__ Vg,
                                                              cplx* __restrict__ taug, int use_lds) {
-  extern __shared__ cplx lds_mat[];
-  __shared__ cplx v[1024];
+  extern __shared__ cplx lds_mat[];  // [the matrix when use_lds] + the current reflector (n)
   __shared__ double red[4];
   const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
   const int lane = t & 63, w = t >> 6;
@@ -1292,6 +1291,7 @@ __global__ __launch_bounds__(256) void hessenberg_noq_kernel(int n, cplx* __rest
   cplx* tau = taug + (size_t)page * n;
   cplx* A = Ag;
   int ld = lda;
+  cplx* v = lds_mat + (use_lds ? (size_t)n * n : 0);
   if (use_lds) {
     A = lds_mat;
     ld = n;
@@ -2975,7 +2975,10 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
   auto te = [&](const char* nm) { if (hook) hook(nm, 1); };
   const long long sZ = (long long)n * n;
   const size_t bytes = (size_t)n * n * sizeof(cplx);
-  const int use_lds = bytes <= (size_t)g_lds_limit;
+  // the whole LDS of a workgroup; the one-page-per-workgroup kernels below keep at
+  // most a few dozen bytes of static LDS beside the page (64 reserved)
+  const size_t ldsMax = (size_t)g_lds_limit + 20 * 1024;
+  const int use_lds = bytes + (size_t)n * sizeof(cplx) + 64 <= ldsMax;  // matrix + reflector
   // Householder vectors and tau live in Vh (n*n + n per page), the first
   // n*n of which double as scratch for nothing else until Q is applied.
   cplx* Vh = Hwork;
@@ -2996,7 +2999,8 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
   else if (blockedNB == 8)
     hessenberg_blocked_kernel<8><<<batch, 256, pan8, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
   else
-    hessenberg_noq_kernel<<<batch, 256, use_lds ? bytes : 0, st>>>(n, A, lda, sA, Vh, tau, use_lds);
+    hessenberg_noq_kernel<<<batch, 256, (use_lds ? bytes : 0) + (size_t)n * sizeof(cplx), st>>>(n, A, lda, sA, Vh,
+                                                                                               tau, use_lds);
   // Z = Q of the reduction, so that the QR's accumulated Z is the full Schur basis
   if (blockedNB && sZ >= (long long)n * n && (long long)3 * n * blockedNB + blockedNB * blockedNB <= (long long)n * n) {
     // Q = P_0 P_1 ... with P_p = I - V_p T_p V_p^H, applied to I from the last panel on;
@@ -3065,14 +3069,15 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
     const size_t lim = (size_t)g_lds_limit;
     const int nbul = g_qr_bulges;
     const bool few = batch < 512;
-    bool perRotation = bytes <= lim && (nbul == 1 || n < 32 || (n <= g_qr_small_n && few));
+    const bool hFits = bytes + 64 <= ldsMax;  // the per-rotation kernel keeps 16 bytes of static LDS
+    bool perRotation = hFits && (nbul == 1 || n < 32 || (n <= g_qr_small_n && few));
     {
       const int tv = orf_tune_value(ORF_TUNE_QR, n, batch);  // 0 per-rotation (H in LDS), 1 multishift
       if (tv == 1) perRotation = false;
-      else if (tv == 0 && bytes <= lim) perRotation = true;
+      else if (tv == 0 && hFits) perRotation = true;
     }
     if (perRotation) {
-      const int zIn = 2 * bytes <= lim;
+      const int zIn = 2 * bytes + 64 <= ldsMax;
       schur_qr_kernel<<<batch, 256, (1 + zIn) * bytes, st>>>(n, A, lda, sA, Zwork, sZ, w, info, 1, zIn);
     } else {
       const size_t stat = 64 * 1024;
