@@ -106,6 +106,7 @@ static void fail(int code, const std::string& m) { throw OrfError{code, m}; }
   do {                                                                                  \
     hipError_t e_ = (x);                                                                \
     if (e_ != hipSuccess) {                                                             \
+      (void)hipGetLastError(); /* the error is reported here, not again by a later call */ \
       if (e_ == hipErrorOutOfMemory) fail(ORF_OOM, std::string("out of device memory: ") + #x); \
       fail(ORF_DEVICE_ERROR, std::string(hipGetErrorString(e_)) + " at " + #x);         \
     }                                                                                   \
@@ -196,6 +197,7 @@ struct DevicePool {
       trim();
       e = hipMalloc(&p, want);
     }
+    if (e != hipSuccess) (void)hipGetLastError();  // reported here, not again by the next call's checks
     if (e == hipErrorOutOfMemory) fail(ORF_OOM, "out of device memory (" + std::to_string(want >> 20) + " MiB)");
     if (e != hipSuccess) fail(ORF_DEVICE_ERROR, std::string(hipGetErrorString(e)) + " in hipMalloc");
     std::lock_guard<std::mutex> lk(mu);

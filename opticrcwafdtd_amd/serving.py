@@ -358,10 +358,26 @@ def run_load(server: Server, requests: Sequence[tuple], rate: Optional[float], s
             "by_kind": by_kind}
 
 
-def capacity(server: Server, count: int, seed: int, overload: float) -> dict:
-    """Service capacity under Poisson arrivals: offer ``overload`` requests/s
-    (more than the server sustains) and take the steady completion rate."""
-    return run_load(server, random_requests(count, seed), overload, seed=seed)
+def capacity(server: Server, rate0: float, seed: int, seconds: float = 4.0, steps: int = 5,
+             sustain: float = 0.98) -> dict:
+    """Largest sustained Poisson arrival rate: starting at ``rate0`` (e.g. the
+    burst rate), each step offers ``seconds`` worth of requests and raises the
+    rate by 15 % while the steady completion rate keeps up (>= ``sustain`` of
+    the offered one), else bisects towards the last sustained rate.  Returns
+    the sustained rate and the steps taken."""
+    lo, hi, rate, log = None, None, float(rate0), []
+    for k in range(steps):
+        n = int(max(200, rate * seconds))
+        r = run_load(server, random_requests(n, seed + k), rate, seed=seed + k)
+        ok = r["achieved_frac"] >= sustain
+        log.append({"offered": round(rate, 2), "achieved": round(r["steady_req_per_s"], 2), "sustained": bool(ok)})
+        if ok:
+            lo = rate
+            rate = rate * 1.15 if hi is None else 0.5 * (rate + hi)
+        else:
+            hi = rate
+            rate = 0.85 * rate if lo is None else 0.5 * (lo + rate)
+    return {"req_per_s": lo if lo is not None else rate, "search": log}
 
 
 __all__ = ["Server", "Request", "random_requests", "submit", "run_load", "capacity", "pillar", "sidewall_cell"]
