@@ -82,8 +82,10 @@ def test_request_mix_is_seeded_with_the_stated_shares():
     b = serving.random_requests(2000, seed=1)
     assert a == b
     kinds = [k for k, _ in a]
-    share = {k: kinds.count(k) / len(kinds) for k in ("lookup", "ladder", "sweep")}
-    assert abs(share["lookup"] - 0.7) < 0.04 and abs(share["ladder"] - 0.2) < 0.03 and abs(share["sweep"] - 0.1) < 0.03
+    for k0 in range(0, len(kinds), 10):  # exact shares in every block of ten
+        blk = kinds[k0:k0 + 10]
+        assert (blk.count("lookup"), blk.count("ladder"), blk.count("sweep")) == (7, 2, 1)
+    assert kinds[:10] != kinds[10:20] or kinds[10:20] != kinds[20:30]  # shuffled within blocks
     sw = next(x for k, x in a if k == "sweep")
     assert len(sw["angles"]) == 16 and sw["slices"] == 32
     assert len(serving.sidewall_cell(85.0, slices=32)["layers"]) == 32
