@@ -1935,11 +1935,14 @@ __global__ __launch_bounds__(256) void schur_qr_kernel(int n, cplx* __restrict__
 // waves per SIMD the register budget is cut for (two QR workgroups per CU hide
 // the latency of the dependent rotation chain), rotations between compiler
 // barriers in the strip updates, and the early-deflation window.  Measured on
-// the +-5 phase-library batch (n = 242): 4 bulges x 16 positions at 2
-// workgroups/CU beat 32 positions (longer chase) and 3-4 workgroups/CU
-// (spills); 8 bulges spill; windows of 8-12 beat 16-24 (the window's Schur
-// form is a serial rotation chain).
-constexpr int MBQ_NB = 4, MBQ_SEG = 16, MBQ_OCC = 2, MBQ_GRP = 16, MBQ_NW = 12;
+// the +-5 phase-library batch (n = 242, 2048 pages): 8 bulges (8 lanes per
+// bulge in the chase, strips swept in two groups of 4) with a 16-wide window:
+// 187 ms, against 208 ms for 4 bulges with a 12-wide window (half the passes,
+// early-deflation windows and chase steps per shift); windows of 12 / 20 / 24:
+// 199 / 187 / 198 ms; 8 / 24 / 32 positions per segment: 221 / 208 / 212 ms;
+// whole 4-bulge strips loaded at once (lookahead 16) instead of 8 steps ahead:
+// 172 vs 182 ms.
+constexpr int MBQ_NB = 8, MBQ_SEG = 16, MBQ_OCC = 2, MBQ_GRP = 16, MBQ_NW = 16;
 
 // Eigenvalues of a small m x m (m <= 8) complex matrix held in LDS (ld 8), by
 // single-shift QR without vectors; run by one thread.
@@ -2345,9 +2348,15 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
   constexpr int NIBBLE = (NW * 25) / 100;
   static_assert(NWX <= 32, "window lanes");
   constexpr int WINSZ = MB_SW * MB_SW > LDW * NWX ? MB_SW * MB_SW : LDW * NWX;
-  constexpr int LA = MB_NB >= 8 ? 4 : 8;         // strip fetch lookahead (steps)
-  constexpr int RING = 3 * MB_NB + LA;           // strip registers per task
-  constexpr int PLAST = MB_SEG + 3 * MB_NB - 2;  // last strip position a rotation touches
+  // The strips take a segment's rotations in sweeps of SNB bulges: the leading
+  // group of bulges first, then the next.  On any strip position a leading
+  // bulge's rotations all come before a trailing bulge's (bulge b + 1 reaches a
+  // position 3 steps after bulge b), so the group order is a valid reordering
+  // of the chase's rotations, and a sweep's register ring (3 SNB + LA) does not
+  // grow with the bulges per pass.
+  constexpr int SNB = MB_NB > 4 ? 4 : MB_NB;
+  static_assert(MB_NB % SNB == 0 && 64 % MB_NB == 0, "bulge groups");
+  constexpr int LA = 16;  // strip fetch lookahead (steps): a whole 4-bulge strip in flight
   extern __shared__ cplx lds_mat[];
   __shared__ cplx win[WINSZ];       // chase window, or the AED window T
   __shared__ cplx Uw[LDW * NWX];
@@ -2639,89 +2648,75 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
           // lockstep: at step j bulge b sits at position p0 - 3b + j; the rotations
           // of one step act on disjoint row and column pairs
           for (int j = 0; j < MB_SEG; ++j) {
-            // Lane e (and e + 64) takes bulge b = e / MB_SW at window column (row
-            // ops) or row (column ops) e % MB_SW, and computes b's rotation itself
-            // from the same two LDS words as the bulge's other lanes: no lane waits
-            // for a single computing lane, and the column ops reuse the rotation
-            // from registers.  All loads of a phase are issued before its stores
-            // (the pairs of one step are disjoint).
-            constexpr int EPL = (MB_NB * MB_SW + 63) / 64;
-            bool act[EPL];
-            int kr[EPL];
-            double cr[EPL];
-            cplx sr[EPL];
+            // LPB lanes per bulge: lane (b, sub) computes bulge b's rotation itself
+            // from the same two LDS words as the bulge's other lanes (no lane waits
+            // for a single computing lane, one rotation per lane whatever the bulge
+            // count) and applies it to window columns (row ops) or rows (column ops)
+            // sub, sub + LPB, ...; the column ops reuse the rotation from registers.
+            // All loads of a phase are issued before its stores (the pairs of one
+            // step are disjoint).
+            constexpr int LPB = 64 / MB_NB;
+            constexpr int EPB = (MB_SW + LPB - 1) / LPB;
+            const int b = lane / LPB, sub = lane % LPB;
+            const int k = p0 - 3 * b + j;
+            const bool act = b < nbul && k >= l && k <= i - 1;
+            const int kk = act ? k - a0 : 1;
+            double cr = 1.0;
+            cplx sr = cmk(0.0, 0.0);
             {
-              cplx xv[EPL], yv[EPL], h1v[EPL], h2v[EPL];
-              bool rowop[EPL];
+              const cplx xv = k == l ? WN(kk, kk) : WN(kk, kk - 1);
+              const cplx yv = k == l ? WN(kk + 1, kk) : WN(kk + 1, kk - 1);
+              cplx h1v[EPB], h2v[EPB];
+              bool rowop[EPB];
 #pragma unroll
-              for (int u = 0; u < EPL; ++u) {
-                const int e = lane + 64 * u;
-                const int b = e / MB_SW, jc = e % MB_SW;
-                const int k = p0 - 3 * b + j;
-                act[u] = e < MB_NB * MB_SW && b < nbul && k >= l && k <= i - 1;
-                const int kk = act[u] ? k - a0 : 1;
-                kr[u] = kk;
-                if (k == l) {
-                  xv[u] = WN(kk, kk);
-                  yv[u] = WN(kk + 1, kk);
-                } else {
-                  xv[u] = WN(kk, kk - 1);
-                  yv[u] = WN(kk + 1, kk - 1);
-                }
-                rowop[u] = act[u] && jc >= kk && a0 + jc < n;
+              for (int u = 0; u < EPB; ++u) {
+                const int jc = sub + LPB * u;
+                rowop[u] = act && jc < MB_SW && jc >= kk && a0 + jc < n;
                 const int jr = rowop[u] ? jc : kk;
                 h1v[u] = WN(kk, jr);
                 h2v[u] = WN(kk + 1, jr);
               }
+              cplx r = cmk(0.0, 0.0);
+              if (act) {
+                const cplx x = k == l ? csub(xv, shifts[b]) : xv;
+                givens(x, yv, cr, sr, r);
+              }
 #pragma unroll
-              for (int u = 0; u < EPL; ++u) {
-                const int e = lane + 64 * u;
-                const int b = e / MB_SW, jc = e % MB_SW;
-                const int k = p0 - 3 * b + j;
-                double c = 1.0;
-                cplx sn = cmk(0.0, 0.0), r = cmk(0.0, 0.0);
-                if (act[u]) {
-                  const cplx x = k == l ? csub(xv[u], shifts[b]) : xv[u];
-                  givens(x, yv[u], c, sn, r);
-                }
-                cr[u] = c;
-                sr[u] = sn;
-                const int kk = kr[u];
+              for (int u = 0; u < EPB; ++u) {
+                const int jc = sub + LPB * u;
                 if (rowop[u]) {
-                  WN(kk, jc) = cadd(cscale(h1v[u], c), cmul(sn, h2v[u]));
-                  WN(kk + 1, jc) = csub(cscale(h2v[u], c), cmul(cconj(sn), h1v[u]));
+                  WN(kk, jc) = cadd(cscale(h1v[u], cr), cmul(sr, h2v[u]));
+                  WN(kk + 1, jc) = csub(cscale(h2v[u], cr), cmul(cconj(sr), h1v[u]));
                 }
-                if (jc == 0 && e < MB_NB * MB_SW) {  // one lane per bulge: record and fix
-                  rcb[b * MB_SEG + j] = c;
-                  rsb[b * MB_SEG + j] = sn;
-                  if (act[u] && k > l) {
-                    WN(kk, kk - 1) = r;
-                    WN(kk + 1, kk - 1) = cmk(0.0, 0.0);
-                  }
+              }
+              if (sub == 0) {  // one lane per bulge: record and fix
+                rcb[b * MB_SEG + j] = cr;
+                rsb[b * MB_SEG + j] = sr;
+                if (act && k > l) {
+                  WN(kk, kk - 1) = r;
+                  WN(kk + 1, kk - 1) = cmk(0.0, 0.0);
                 }
               }
             }
             wave_sync_lds();
             // column ops of every bulge on window rows 0..min(kk+2, i-a0)
             {
-              cplx h1v[EPL], h2v[EPL];
-              bool colop[EPL];
+              cplx h1v[EPB], h2v[EPB];
+              bool colop[EPB];
 #pragma unroll
-              for (int u = 0; u < EPL; ++u) {
-                const int q = (lane + 64 * u) % MB_SW;
-                const int kk = kr[u];
-                colop[u] = act[u] && q <= min(kk + 2, i - a0) && a0 + q >= 0;
+              for (int u = 0; u < EPB; ++u) {
+                const int q = sub + LPB * u;
+                colop[u] = act && q < MB_SW && q <= min(kk + 2, i - a0) && a0 + q >= 0;
                 const int qr = colop[u] ? q : kk;
                 h1v[u] = WN(qr, kk);
                 h2v[u] = WN(qr, kk + 1);
               }
 #pragma unroll
-              for (int u = 0; u < EPL; ++u) {
-                const int q = (lane + 64 * u) % MB_SW;
-                const int kk = kr[u];
+              for (int u = 0; u < EPB; ++u) {
+                const int q = sub + LPB * u;
                 if (colop[u]) {
-                  WN(q, kk) = cadd(cscale(h1v[u], cr[u]), cmul(cconj(sr[u]), h2v[u]));
-                  WN(q, kk + 1) = csub(cscale(h2v[u], cr[u]), cmul(sr[u], h1v[u]));
+                  WN(q, kk) = cadd(cscale(h1v[u], cr), cmul(cconj(sr), h2v[u]));
+                  WN(q, kk + 1) = csub(cscale(h2v[u], cr), cmul(sr, h1v[u]));
                 }
               }
             }
@@ -2767,7 +2762,13 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
               }
               // one instance only: with a checked and an unchecked copy behind a branch
               // the compiler hoists every rotation's LDS loads above it and spills
-              strip_rotations<MB_NB, MB_SEG, GRP, LA, true>(base, (ptrdiff_t)stride, sa0, n, rowop, rcs, rss);
+#pragma unroll 1
+              for (int g = 0; g < MB_NB / SNB; ++g) {
+                const int off = 3 * SNB * (MB_NB / SNB - 1 - g);  // group g's first position
+                strip_rotations<SNB, MB_SEG, GRP, LA, true>(base + (ptrdiff_t)off * (ptrdiff_t)stride,
+                                                            (ptrdiff_t)stride, sa0 + off, n, rowop,
+                                                            rcs + g * SNB * MB_SEG, rss + g * SNB * MB_SEG);
+              }
             }
           }
         }
