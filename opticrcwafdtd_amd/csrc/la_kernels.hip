@@ -2081,23 +2081,24 @@ ORF_DEV int win_schur(int m, cplx* T, cplx* U, int lane) {
       double c;
       cplx sn, r;
       givens(a, b, c, sn, r);
-      // rows kk, kk+1 (columns kk..m-1) on lanes 0..31; U columns kk, kk+1 on lanes 32..63
-      if (lane < 32) {
-        const int j = kk + lane;
-        if (j < m) {
-          const cplx h1 = T[kk + LDW * j], h2 = T[(kk + 1) + LDW * j];
-          T[kk + LDW * j] = cadd(cscale(h1, c), cmul(sn, h2));
-          T[(kk + 1) + LDW * j] = csub(cscale(h2, c), cmulc(sn, h1));
+      // rows kk, kk+1 (columns kk..m-1) on lanes 0..31, U columns kk, kk+1 on lanes
+      // 32..63, as one instruction stream: x1 <- c x1 + g x2, x2 <- c x2 - conj(g) x1
+      // with g = s on T's rows and conj(s) on U's columns
+      {
+        const bool onU = lane >= 32;
+        const int jq = onU ? lane - 32 : kk + lane;
+        cplx* const p1 = onU ? U + jq + LDW * kk : T + kk + LDW * jq;
+        const int d = onU ? LDW : 1;
+        const cplx gs = onU ? cconj(sn) : sn;
+        if (jq < m) {
+          const cplx h1 = p1[0], h2 = p1[d];
+          p1[0] = cadd(cscale(h1, c), cmul(gs, h2));
+          p1[d] = csub(cscale(h2, c), cmulc(gs, h1));
         }
         if (lane == 0 && kk > lo) {
           T[kk + LDW * (kk - 1)] = r;
           T[(kk + 1) + LDW * (kk - 1)] = cmk(0.0, 0.0);
         }
-      } else if (lane - 32 < m) {
-        const int q = lane - 32;
-        const cplx u1 = U[q + LDW * kk], u2 = U[q + LDW * (kk + 1)];
-        U[q + LDW * kk] = cadd(cscale(u1, c), cmulc(sn, u2));
-        U[q + LDW * (kk + 1)] = csub(cscale(u2, c), cmul(sn, u1));
       }
       wave_sync_lds();
       // columns kk, kk+1 for rows 0..min(kk+2, hi)
@@ -2743,10 +2744,7 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
           const double* rcs = rc + (ss & 1) * MB_NB * MB_SEG;
           const cplx* rss = rs + (ss & 1) * MB_NB * MB_SEG;
           if (t0 >= 0) {
-            for (int task = tlo + t0; task < thi; task += tstep) {
-              cplx* base;
-              size_t stride;
-              bool rowop;
+            auto strip_of = [&](int task, cplx*& base, ptrdiff_t& stride, bool& rowop) {
               if (task < ncR) {
                 base = H + (size_t)(e0 + task) * ld + sa0;  // column e0+task, rows sa0..
                 stride = 1;
@@ -2760,14 +2758,19 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
                 stride = n;
                 rowop = false;
               }
+            };
+            for (int task = tlo + t0; task < thi; task += tstep) {
+              cplx* base;
+              ptrdiff_t stride;
+              bool rowop;
+              strip_of(task, base, stride, rowop);
               // one instance only: with a checked and an unchecked copy behind a branch
               // the compiler hoists every rotation's LDS loads above it and spills
 #pragma unroll 1
               for (int g = 0; g < MB_NB / SNB; ++g) {
                 const int off = 3 * SNB * (MB_NB / SNB - 1 - g);  // group g's first position
-                strip_rotations<SNB, MB_SEG, GRP, LA, true>(base + (ptrdiff_t)off * (ptrdiff_t)stride,
-                                                            (ptrdiff_t)stride, sa0 + off, n, rowop,
-                                                            rcs + g * SNB * MB_SEG, rss + g * SNB * MB_SEG);
+                strip_rotations<SNB, MB_SEG, GRP, LA, true>(base + (ptrdiff_t)off * stride, stride, sa0 + off, n,
+                                                            rowop, rcs + g * SNB * MB_SEG, rss + g * SNB * MB_SEG);
               }
             }
           }
