@@ -1090,10 +1090,9 @@ __global__ __launch_bounds__(256) void ztrsm_right_kernel(int n, int nrows, int 
 }  // namespace
 
 static int g_lds_limit = 65536;
-// QR kernel choice (fixed by measurement on the +-5 batch): multi-bulge passes of
-// MBQ_NB bulges; up to this size the per-rotation LDS kernel serves small batches;
-// early deflation from the bottom of the window
-static constexpr int g_qr_bulges = 4;      // 1 would select the per-rotation kernel at every size
+// QR kernel choice (fixed by measurement on the +-5 batch): up to this size the
+// per-rotation LDS kernel serves small batches; early deflation from the bottom
+// of the window
 static constexpr int g_qr_small_n = 96;
 static constexpr int g_qr_aed = 2;         // 0 off, 1 reordering, 2 bottom only, 3 whole blocks only
 
@@ -1943,6 +1942,10 @@ __global__ __launch_bounds__(256) void schur_qr_kernel(int n, cplx* __restrict__
 // whole 4-bulge strips loaded at once (lookahead 16) instead of 8 steps ahead:
 // 172 vs 182 ms.
 constexpr int MBQ_NB = 8, MBQ_SEG = 16, MBQ_OCC = 2, MBQ_GRP = 16, MBQ_NW = 16;
+// Small matrices: 4 bulges and a 12-wide window.  The 8-bulge kernel's static
+// LDS (40 KB) next to an H held in LDS leaves one workgroup per CU where this
+// one (20 KB) keeps two: 8,192 pages of n = 61 take 42 ms instead of 94.
+constexpr int MBS_NB = 4, MBS_NW = 12;
 
 // Eigenvalues of a small m x m (m <= 8) complex matrix held in LDS (ld 8), by
 // single-shift QR without vectors; run by one thread.
@@ -3071,14 +3074,21 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
     // decides: multi-bulge passes over a window, off-window work deferred
     // (measured at n = 61: 47 vs 69 ms for 8192 pages; 5.2 vs 4.2 ms for 4).
     const size_t lim = (size_t)g_lds_limit;
-    const int nbul = g_qr_bulges;
     const bool few = batch < 512;
     const bool hFits = bytes + 64 <= ldsMax;  // the per-rotation kernel keeps 16 bytes of static LDS
-    bool perRotation = hFits && (nbul == 1 || n < 32 || (n <= g_qr_small_n && few));
+    bool perRotation = hFits && (n < 32 || (n <= g_qr_small_n && few));
+    // multishift passes of 8 bulges with a 16-wide window from n = 128 (H no
+    // longer fits LDS there), else 4 bulges with a 12-wide window
+    bool wide = n >= 128;
     {
-      const int tv = orf_tune_value(ORF_TUNE_QR, n, batch);  // 0 per-rotation (H in LDS), 1 multishift
-      if (tv == 1) perRotation = false;
-      else if (tv == 0 && hFits) perRotation = true;
+      // 0 per-rotation (H in LDS), 1 multishift 4 bulges, 2 multishift 8 bulges
+      const int tv = orf_tune_value(ORF_TUNE_QR, n, batch);
+      if (tv == 1 || tv == 2) {
+        perRotation = false;
+        wide = tv == 2;
+      } else if (tv == 0 && hFits) {
+        perRotation = true;
+      }
     }
     if (perRotation) {
       const int zIn = 2 * bytes + 64 <= ldsMax;
@@ -3088,8 +3098,12 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
       const size_t lim2 = lim + 20 * 1024 - stat;
       const int hIn = bytes <= lim2, zIn = 2 * bytes <= lim2;
       const int mode = g_qr_aed == 1 ? 0 : (g_qr_aed == 2 ? 1 : (g_qr_aed == 3 ? 2 : 3));
-      schur_qr_aed_kernel<MBQ_NB, MBQ_SEG, MBQ_OCC, MBQ_GRP, MBQ_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
-          n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn, mode);
+      if (wide)
+        schur_qr_aed_kernel<MBQ_NB, MBQ_SEG, MBQ_OCC, MBQ_GRP, MBQ_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
+            n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn, mode);
+      else
+        schur_qr_aed_kernel<MBS_NB, MBQ_SEG, MBQ_OCC, MBQ_GRP, MBS_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
+            n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn, mode);
     }
   }
   te("eig_schur_qr");
