@@ -1935,17 +1935,21 @@ __global__ __launch_bounds__(256) void schur_qr_kernel(int n, cplx* __restrict__
 // the latency of the dependent rotation chain), rotations between compiler
 // barriers in the strip updates, and the early-deflation window.  Measured on
 // the +-5 phase-library batch (n = 242, 2048 pages): 8 bulges (8 lanes per
-// bulge in the chase, strips swept in two groups of 4) with a 16-wide window:
-// 187 ms, against 208 ms for 4 bulges with a 12-wide window (half the passes,
-// early-deflation windows and chase steps per shift); windows of 12 / 20 / 24:
-// 199 / 187 / 198 ms; 8 / 24 / 32 positions per segment: 221 / 208 / 212 ms;
-// whole 4-bulge strips loaded at once (lookahead 16) instead of 8 steps ahead:
-// 172 vs 182 ms.
-constexpr int MBQ_NB = 8, MBQ_SEG = 16, MBQ_OCC = 2, MBQ_GRP = 16, MBQ_NW = 16;
+// bulge in the chase) with a 16-wide window: 187 ms, against 208 ms for 4
+// bulges with a 12-wide window (half the passes, early-deflation windows and
+// chase steps per shift); windows of 12 / 20 / 24: 199 / 187 / 198 ms;
+// 8 / 24 / 32 positions per segment: 221 / 208 / 212 ms; whole strips loaded
+// at once (lookahead 16) instead of 8 steps ahead: 172 vs 182 ms.  The strips
+// take all 8 bulges' rotations in one sweep with 8 rotations between compiler
+// barriers: 172 ms, against 190 ms for two sweeps of 4 bulges (each strip
+// element loaded and stored twice per segment; same box, same run).
+// Deferring the rows of Z and above the windows over two segments (one sweep
+// of 32 steps) was 5 % slower: the larger strip code outweighs the traffic.
+constexpr int MBQ_NB = 8, MBQ_SEG = 16, MBQ_OCC = 2, MBQ_GRP = 8, MBQ_NW = 16;
 // Small matrices: 4 bulges and a 12-wide window.  The 8-bulge kernel's static
 // LDS (40 KB) next to an H held in LDS leaves one workgroup per CU where this
 // one (20 KB) keeps two: 8,192 pages of n = 61 take 42 ms instead of 94.
-constexpr int MBS_NB = 4, MBS_NW = 12;
+constexpr int MBS_NB = 4, MBS_NW = 12, MBS_GRP = 16;
 
 // Eigenvalues of a small m x m (m <= 8) complex matrix held in LDS (ld 8), by
 // single-shift QR without vectors; run by one thread.
@@ -2352,14 +2356,7 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
   constexpr int NIBBLE = (NW * 25) / 100;
   static_assert(NWX <= 32, "window lanes");
   constexpr int WINSZ = MB_SW * MB_SW > LDW * NWX ? MB_SW * MB_SW : LDW * NWX;
-  // The strips take a segment's rotations in sweeps of SNB bulges: the leading
-  // group of bulges first, then the next.  On any strip position a leading
-  // bulge's rotations all come before a trailing bulge's (bulge b + 1 reaches a
-  // position 3 steps after bulge b), so the group order is a valid reordering
-  // of the chase's rotations, and a sweep's register ring (3 SNB + LA) does not
-  // grow with the bulges per pass.
-  constexpr int SNB = MB_NB > 4 ? 4 : MB_NB;
-  static_assert(MB_NB % SNB == 0 && 64 % MB_NB == 0, "bulge groups");
+  static_assert(64 % MB_NB == 0, "lanes per bulge in the chase");
   constexpr int LA = 16;  // strip fetch lookahead (steps): a whole 4-bulge strip in flight
   extern __shared__ cplx lds_mat[];
   __shared__ cplx win[WINSZ];       // chase window, or the AED window T
@@ -2769,12 +2766,7 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
               strip_of(task, base, stride, rowop);
               // one instance only: with a checked and an unchecked copy behind a branch
               // the compiler hoists every rotation's LDS loads above it and spills
-#pragma unroll 1
-              for (int g = 0; g < MB_NB / SNB; ++g) {
-                const int off = 3 * SNB * (MB_NB / SNB - 1 - g);  // group g's first position
-                strip_rotations<SNB, MB_SEG, GRP, LA, true>(base + (ptrdiff_t)off * stride, stride, sa0 + off, n,
-                                                            rowop, rcs + g * SNB * MB_SEG, rss + g * SNB * MB_SEG);
-              }
+              strip_rotations<MB_NB, MB_SEG, GRP, LA, true>(base, stride, sa0, n, rowop, rcs, rss);
             }
           }
         }
@@ -3102,7 +3094,7 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
         schur_qr_aed_kernel<MBQ_NB, MBQ_SEG, MBQ_OCC, MBQ_GRP, MBQ_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
             n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn, mode);
       else
-        schur_qr_aed_kernel<MBS_NB, MBQ_SEG, MBQ_OCC, MBQ_GRP, MBS_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
+        schur_qr_aed_kernel<MBS_NB, MBQ_SEG, MBQ_OCC, MBS_GRP, MBS_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
             n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn, mode);
     }
   }
