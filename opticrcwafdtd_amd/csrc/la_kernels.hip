@@ -2368,6 +2368,7 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
   __shared__ cplx shifts[MB_NB];
   __shared__ int redk[4];
   __shared__ int aedres[2];
+  __shared__ int prid;  // segment sequence number whose priority strips are done
   const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
   const int lane = t & 63, wv = t >> 6, nw = nt >> 6;
   Hg += (size_t)page * sH;
@@ -2388,6 +2389,7 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
     Z = ldsNext;
     lds_fill(t, nt, n * n, [&](int e) { return Zp[e]; }, [&](int e) -> cplx& { return Z[e]; });
   }
+  if (t == 0) prid = 0;
   __syncthreads();
 #define HH(r, c) H[(size_t)(r) + (size_t)(c) * ld]
 #define WN(r, c) win[(r) + (c) * MB_SW]
@@ -2398,6 +2400,7 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
   int failed = 0;
   int i = n - 1;
   int its = 0;
+  int segseq = 0;  // segments started (the same count in every wave)
   while (i >= 0) {
     // ---- deflation search: largest k in (0, i] with negligible H[k, k-1]
     int kfound = 0;
@@ -2607,171 +2610,196 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
       continue;
     }
     // ---- one pass: segments of MB_SEG positions of bulge 0
-    // Segments are pipelined: while wave 0 loads and chases segment `it`, the
-    // other waves apply segment it-1's rotations to everything outside the next
-    // window (phase 0); then all threads write window `it` back and update the
-    // MB_SEG columns the next window reads (phase 1).  Rotations are double
-    // buffered.  Every region a phase touches is disjoint from what the other
-    // side of that phase touches, and successive updates of one element are
-    // separated by a barrier.
+    // Segments are pipelined, one barrier per segment: while wave 0 loads and
+    // chases segment `it`, writes its window back and gives the next window's
+    // first MB_SEG columns right segment it's rotations, the other waves apply
+    // segment it-1's rotations to everything else outside the window (the
+    // columns right beyond its first MB_SEG, the rows above its window, Z).  The
+    // one overlap, the first MB_SEG columns right of window it, which take
+    // segment it-1's rotations first, wave 1 does as its first tasks and then
+    // raises a flag that wave 0 waits for.  Rotations are double buffered.
     const int nseg = (i - 1 + 3 * (nbul - 1) - l) / MB_SEG + 1;
     for (int it = 0; it <= nseg; ++it) {
-      for (int ph = 0; ph < 2; ++ph) {
-        const int cs = ph == 0 ? it : -1;           // segment chased in this phase
-        const int ss = ph == 0 ? it - 1 : it;       // segment whose strips run in this phase
-        const int a0 = l + it * MB_SEG - 3 * (MB_NB - 1) - 1;  // window origin of segment it
-        if (ph == 0 && cs < nseg && wv == 0) {
-          double* rcb = rc + (cs & 1) * MB_NB * MB_SEG;
-          cplx* rsb = rs + (cs & 1) * MB_NB * MB_SEG;
-          const int p0 = l + cs * MB_SEG;
-          // the window's loads go out in batches of up to 13 per lane, all in flight
-          // together (a loop that stores each element before loading the next waits
-          // out one memory latency per trip)
-          constexpr int WLOADS = (MB_SW * MB_SW + 63) / 64;
-          constexpr int WB = WLOADS < 13 ? WLOADS : 13;
+      ++segseq;
+      const int cs = it;                                     // segment chased in this iteration
+      const int a0 = l + it * MB_SEG - 3 * (MB_NB - 1) - 1;  // window origin of segment it
+      const bool chasing = it < nseg;
+      if (chasing && wv == 0) {
+        double* rcb = rc + (cs & 1) * MB_NB * MB_SEG;
+        cplx* rsb = rs + (cs & 1) * MB_NB * MB_SEG;
+        const int p0 = l + cs * MB_SEG;
+        // the window's loads go out in batches of up to 13 per lane, all in flight
+        // together (a loop that stores each element before loading the next waits
+        // out one memory latency per trip)
+        constexpr int WLOADS = (MB_SW * MB_SW + 63) / 64;
+        constexpr int WB = WLOADS < 13 ? WLOADS : 13;
 #pragma unroll
-          for (int u0 = 0; u0 < WLOADS; u0 += WB) {
-            cplx wl[WB];
+        for (int u0 = 0; u0 < WLOADS; u0 += WB) {
+          cplx wl[WB];
 #pragma unroll
-            for (int u = 0; u < WB; ++u) {
-              int q = lane + 64 * (u0 + u);
-              asm volatile("" : "+v"(q));  // recomputed per segment, not held in registers
-              const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
-              const bool ok = q < MB_SW * MB_SW && r >= 0 && r < n && c >= 0 && c < n;
-              const cplx x = *(ok ? &HH(r, c) : g_strip_sink);
-              wl[u] = ok ? x : cmk(0.0, 0.0);
+          for (int u = 0; u < WB; ++u) {
+            int q = lane + 64 * (u0 + u);
+            asm volatile("" : "+v"(q));  // recomputed per segment, not held in registers
+            const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
+            const bool ok = q < MB_SW * MB_SW && r >= 0 && r < n && c >= 0 && c < n;
+            const cplx x = *(ok ? &HH(r, c) : g_strip_sink);
+            wl[u] = ok ? x : cmk(0.0, 0.0);
+          }
+#pragma unroll
+          for (int u = 0; u < WB; ++u)
+            if (lane + 64 * (u0 + u) < MB_SW * MB_SW) win[lane + 64 * (u0 + u)] = wl[u];
+        }
+        wave_sync_lds();
+        // lockstep: at step j bulge b sits at position p0 - 3b + j; the rotations
+        // of one step act on disjoint row and column pairs
+        for (int j = 0; j < MB_SEG; ++j) {
+          // LPB lanes per bulge: lane (b, sub) computes bulge b's rotation itself
+          // from the same two LDS words as the bulge's other lanes (no lane waits
+          // for a single computing lane, one rotation per lane whatever the bulge
+          // count) and applies it to window columns (row ops) or rows (column ops)
+          // sub, sub + LPB, ...; the column ops reuse the rotation from registers.
+          // All loads of a phase are issued before its stores (the pairs of one
+          // step are disjoint).
+          constexpr int LPB = 64 / MB_NB;
+          constexpr int EPB = (MB_SW + LPB - 1) / LPB;
+          const int b = lane / LPB, sub = lane % LPB;
+          const int k = p0 - 3 * b + j;
+          const bool act = b < nbul && k >= l && k <= i - 1;
+          const int kk = act ? k - a0 : 1;
+          double cr = 1.0;
+          cplx sr = cmk(0.0, 0.0);
+          {
+            const cplx xv = k == l ? WN(kk, kk) : WN(kk, kk - 1);
+            const cplx yv = k == l ? WN(kk + 1, kk) : WN(kk + 1, kk - 1);
+            cplx h1v[EPB], h2v[EPB];
+            bool rowop[EPB];
+#pragma unroll
+            for (int u = 0; u < EPB; ++u) {
+              const int jc = sub + LPB * u;
+              rowop[u] = act && jc < MB_SW && jc >= kk && a0 + jc < n;
+              const int jr = rowop[u] ? jc : kk;
+              h1v[u] = WN(kk, jr);
+              h2v[u] = WN(kk + 1, jr);
+            }
+            cplx r = cmk(0.0, 0.0);
+            if (act) {
+              const cplx x = k == l ? csub(xv, shifts[b]) : xv;
+              givens(x, yv, cr, sr, r);
             }
 #pragma unroll
-            for (int u = 0; u < WB; ++u)
-              if (lane + 64 * (u0 + u) < MB_SW * MB_SW) win[lane + 64 * (u0 + u)] = wl[u];
+            for (int u = 0; u < EPB; ++u) {
+              const int jc = sub + LPB * u;
+              if (rowop[u]) {
+                WN(kk, jc) = cadd(cscale(h1v[u], cr), cmul(sr, h2v[u]));
+                WN(kk + 1, jc) = csub(cscale(h2v[u], cr), cmul(cconj(sr), h1v[u]));
+              }
+            }
+            if (sub == 0) {  // one lane per bulge: record and fix
+              rcb[b * MB_SEG + j] = cr;
+              rsb[b * MB_SEG + j] = sr;
+              if (act && k > l) {
+                WN(kk, kk - 1) = r;
+                WN(kk + 1, kk - 1) = cmk(0.0, 0.0);
+              }
+            }
           }
           wave_sync_lds();
-          // lockstep: at step j bulge b sits at position p0 - 3b + j; the rotations
-          // of one step act on disjoint row and column pairs
-          for (int j = 0; j < MB_SEG; ++j) {
-            // LPB lanes per bulge: lane (b, sub) computes bulge b's rotation itself
-            // from the same two LDS words as the bulge's other lanes (no lane waits
-            // for a single computing lane, one rotation per lane whatever the bulge
-            // count) and applies it to window columns (row ops) or rows (column ops)
-            // sub, sub + LPB, ...; the column ops reuse the rotation from registers.
-            // All loads of a phase are issued before its stores (the pairs of one
-            // step are disjoint).
-            constexpr int LPB = 64 / MB_NB;
-            constexpr int EPB = (MB_SW + LPB - 1) / LPB;
-            const int b = lane / LPB, sub = lane % LPB;
-            const int k = p0 - 3 * b + j;
-            const bool act = b < nbul && k >= l && k <= i - 1;
-            const int kk = act ? k - a0 : 1;
-            double cr = 1.0;
-            cplx sr = cmk(0.0, 0.0);
-            {
-              const cplx xv = k == l ? WN(kk, kk) : WN(kk, kk - 1);
-              const cplx yv = k == l ? WN(kk + 1, kk) : WN(kk + 1, kk - 1);
-              cplx h1v[EPB], h2v[EPB];
-              bool rowop[EPB];
+          // column ops of every bulge on window rows 0..min(kk+2, i-a0)
+          {
+            cplx h1v[EPB], h2v[EPB];
+            bool colop[EPB];
 #pragma unroll
-              for (int u = 0; u < EPB; ++u) {
-                const int jc = sub + LPB * u;
-                rowop[u] = act && jc < MB_SW && jc >= kk && a0 + jc < n;
-                const int jr = rowop[u] ? jc : kk;
-                h1v[u] = WN(kk, jr);
-                h2v[u] = WN(kk + 1, jr);
-              }
-              cplx r = cmk(0.0, 0.0);
-              if (act) {
-                const cplx x = k == l ? csub(xv, shifts[b]) : xv;
-                givens(x, yv, cr, sr, r);
-              }
+            for (int u = 0; u < EPB; ++u) {
+              const int q = sub + LPB * u;
+              colop[u] = act && q < MB_SW && q <= min(kk + 2, i - a0) && a0 + q >= 0;
+              const int qr = colop[u] ? q : kk;
+              h1v[u] = WN(qr, kk);
+              h2v[u] = WN(qr, kk + 1);
+            }
 #pragma unroll
-              for (int u = 0; u < EPB; ++u) {
-                const int jc = sub + LPB * u;
-                if (rowop[u]) {
-                  WN(kk, jc) = cadd(cscale(h1v[u], cr), cmul(sr, h2v[u]));
-                  WN(kk + 1, jc) = csub(cscale(h2v[u], cr), cmul(cconj(sr), h1v[u]));
-                }
-              }
-              if (sub == 0) {  // one lane per bulge: record and fix
-                rcb[b * MB_SEG + j] = cr;
-                rsb[b * MB_SEG + j] = sr;
-                if (act && k > l) {
-                  WN(kk, kk - 1) = r;
-                  WN(kk + 1, kk - 1) = cmk(0.0, 0.0);
-                }
+            for (int u = 0; u < EPB; ++u) {
+              const int q = sub + LPB * u;
+              if (colop[u]) {
+                WN(q, kk) = cadd(cscale(h1v[u], cr), cmul(cconj(sr), h2v[u]));
+                WN(q, kk + 1) = csub(cscale(h2v[u], cr), cmul(sr, h1v[u]));
               }
             }
-            wave_sync_lds();
-            // column ops of every bulge on window rows 0..min(kk+2, i-a0)
-            {
-              cplx h1v[EPB], h2v[EPB];
-              bool colop[EPB];
-#pragma unroll
-              for (int u = 0; u < EPB; ++u) {
-                const int q = sub + LPB * u;
-                colop[u] = act && q < MB_SW && q <= min(kk + 2, i - a0) && a0 + q >= 0;
-                const int qr = colop[u] ? q : kk;
-                h1v[u] = WN(qr, kk);
-                h2v[u] = WN(qr, kk + 1);
-              }
-#pragma unroll
-              for (int u = 0; u < EPB; ++u) {
-                const int q = sub + LPB * u;
-                if (colop[u]) {
-                  WN(q, kk) = cadd(cscale(h1v[u], cr), cmul(cconj(sr), h2v[u]));
-                  WN(q, kk + 1) = csub(cscale(h2v[u], cr), cmul(sr, h1v[u]));
-                }
-              }
+          }
+          wave_sync_lds();
+        }
+        // window back to H: nothing else touches it in this iteration
+        for (int q = lane; q < MB_SW * MB_SW; q += 64) {
+          const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
+          if (r >= 0 && r < n && c >= 0 && c < n) HH(r, c) = win[q];
+        }
+        // the next window's first columns right have segment it-1's rotations once
+        // wave 1 has done its first tasks
+        // (wave 1 raises it on every path; the bound only keeps a broken invariant
+        // from hanging the device: the page is then reported as failed)
+        if (it > 0) {
+          int spins = 0;
+          while (__hip_atomic_load(&prid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < segseq) {
+            if (++spins > (1 << 22)) {
+              if (!failed) failed = i + 1;
+              break;
             }
-            wave_sync_lds();
+            __builtin_amdgcn_s_sleep(1);
           }
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         }
-        if (ph == 1 && it < nseg) {
-          for (int q = t; q < MB_SW * MB_SW; q += nt) {
-            const int r = a0 + q % MB_SW, c = a0 + q / MB_SW;
-            if (r >= 0 && r < n && c >= 0 && c < n) HH(r, c) = win[q];
-          }
-        }
+      }
+      {
+        // strips: wave 0 (after its chase) the first MB_SEG columns right of window it
+        // with segment it's rotations; the other threads (all of them after the last
+        // chase) segment it-1's rotations on the rest
+        const bool w0 = chasing && wv == 0;
+        const int ss = w0 ? it : it - 1;
         if (ss >= 0 && ss < nseg) {
-          // strips of segment ss: columns right of its window, rows above it, rows of Z;
-          // phase 1 takes the first MB_SEG columns right (the next window's), phase 0 the rest
           const int sa0 = l + ss * MB_SEG - 3 * (MB_NB - 1) - 1;
           const int e0 = sa0 + MB_SW;
           const int ncR = e0 < n ? n - e0 : 0;
           const int nrA = sa0 > 0 ? sa0 : 0;
           const int nfirst = min(MB_SEG, ncR);
-          const int tlo = ph == 1 ? 0 : nfirst, thi = ph == 1 ? nfirst : ncR + nrA + n;
-          const bool shared = ph == 0 && it < nseg;  // wave 0 is chasing
-          const int t0 = shared ? t - 64 : t, tstep = shared ? nt - 64 : nt;
+          const int tlo = w0 ? 0 : nfirst, thi = w0 ? nfirst : ncR + nrA + n;
+          const int t0 = w0 ? lane : (chasing ? t - 64 : t), tstep = w0 ? 64 : (chasing ? nt - 64 : nt);
           const double* rcs = rc + (ss & 1) * MB_NB * MB_SEG;
           const cplx* rss = rs + (ss & 1) * MB_NB * MB_SEG;
-          if (t0 >= 0) {
-            auto strip_of = [&](int task, cplx*& base, ptrdiff_t& stride, bool& rowop) {
-              if (task < ncR) {
-                base = H + (size_t)(e0 + task) * ld + sa0;  // column e0+task, rows sa0..
-                stride = 1;
-                rowop = true;
-              } else if (task < ncR + nrA) {
-                base = H + (task - ncR) + (size_t)sa0 * ld;  // row task-ncR, columns sa0..
-                stride = ld;
-                rowop = false;
-              } else {
-                base = Z + (task - ncR - nrA) + (ptrdiff_t)sa0 * n;
-                stride = n;
-                rowop = false;
-              }
-            };
-            for (int task = tlo + t0; task < thi; task += tstep) {
-              cplx* base;
-              ptrdiff_t stride;
-              bool rowop;
-              strip_of(task, base, stride, rowop);
-              // one instance only: with a checked and an unchecked copy behind a branch
-              // the compiler hoists every rotation's LDS loads above it and spills
-              strip_rotations<MB_NB, MB_SEG, GRP, LA, true>(base, stride, sa0, n, rowop, rcs, rss);
+          bool flag = chasing && wv == 1;  // wave 1 raises the flag after its first tasks
+          for (int task = tlo + t0; task < thi; task += tstep) {
+            cplx* base;
+            ptrdiff_t stride;
+            bool rowop;
+            if (task < ncR) {
+              base = H + (size_t)(e0 + task) * ld + sa0;  // column e0+task, rows sa0..
+              stride = 1;
+              rowop = true;
+            } else if (task < ncR + nrA) {
+              base = H + (task - ncR) + (size_t)sa0 * ld;  // row task-ncR, columns sa0..
+              stride = ld;
+              rowop = false;
+            } else {
+              base = Z + (task - ncR - nrA) + (ptrdiff_t)sa0 * n;
+              stride = n;
+              rowop = false;
+            }
+            // one instance only: with a checked and an unchecked copy behind a branch
+            // the compiler hoists every rotation's LDS loads above it and spills
+            strip_rotations<MB_NB, MB_SEG, GRP, LA, true>(base, stride, sa0, n, rowop, rcs, rss);
+            if (flag) {
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+              if (lane == 0) __hip_atomic_store(&prid, segseq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              flag = false;
             }
           }
+          if (flag) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (lane == 0) __hip_atomic_store(&prid, segseq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        } else if (chasing && wv == 1 && lane == 0) {
+          __hip_atomic_store(&prid, segseq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-        __syncthreads();
       }
+      __syncthreads();
     }
   }
 
