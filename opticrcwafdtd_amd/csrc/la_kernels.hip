@@ -2038,11 +2038,8 @@ ORF_DEV cplx win_wilkinson(const cplx* W, int i) {
 // Complex Schur form of the m x m Hessenberg window T by single-shift QR,
 // rotations accumulated into U (U <- U G^H).  0 on success, 1 when an
 // eigenvalue needs more than 30 sweeps (T and U then hold a partial, still
-// unitary, reduction).  ONEPHASE: one LDS phase per rotation (the 2 x 2 block
-// where the row and column rotations meet finished with a lane swap) instead
-// of a row phase and a column phase: 8 % faster QR on 16..20-wide windows,
-// 4 % slower on 12..16-wide ones.
-template <int LDW, bool ONEPHASE>
+// unitary, reduction).
+template <int LDW>
 ORF_DEV int win_schur(int m, cplx* T, cplx* U, int lane) {
   const double smlnum = SAFMIN * ((double)m / ULP);
   int hi = m - 1, its = 0;
@@ -2091,83 +2088,32 @@ ORF_DEV int win_schur(int m, cplx* T, cplx* U, int lane) {
       double c;
       cplx sn, r;
       givens(a, b, c, sn, r);
-      if constexpr (ONEPHASE) {
-        // One phase per rotation, one instruction stream for every lane: each lane
-        // rotates one pair, x1 <- c x1 + g x2, x2 <- c x2 - conj(g) x1: lanes 0..
-        // the row pair kk, kk+1 at columns kk.. (g = s), the next lanes the column
-        // pair kk, kk+1 at rows 0..kk-1 and kk+2 when inside the active block
-        // (g = conj(s)), lanes 32.. U's column pair (g = conj(s)).  The row and
-        // column pairs overlap in the 2 x 2 block: lanes 0 and 1 (its columns after
-        // the row rotation) swap values and apply the column rotation to its rows.
-        {
-          const int nrow = m - kk;  // row-pair columns kk..m-1 on lanes 0..nrow-1
-          cplx* p1 = T;
-          int d = 1;
-          bool act = false, conjg = true;
-          if (lane >= 32) {
-            act = lane - 32 < m;
-            p1 = U + (act ? lane - 32 : 0) + LDW * kk;
-            d = LDW;
-          } else if (lane < nrow) {
-            act = true;
-            conjg = false;
-            p1 = T + kk + LDW * (kk + lane);
-          } else {
-            const int v = lane - nrow;
-            const int q = v < kk ? v : (v == kk && kk + 2 <= hi ? kk + 2 : -1);
-            act = q >= 0;
-            p1 = T + (act ? q : 0) + LDW * kk;
-            d = LDW;
-          }
-          const cplx g = conjg ? cconj(sn) : sn;
+      // rows kk, kk+1 (columns kk..m-1) on lanes 0..31, U columns kk, kk+1 on lanes
+      // 32..63, as one instruction stream: x1 <- c x1 + g x2, x2 <- c x2 - conj(g) x1
+      // with g = s on T's rows and conj(s) on U's columns
+      {
+        const bool onU = lane >= 32;
+        const int jq = onU ? lane - 32 : kk + lane;
+        cplx* const p1 = onU ? U + jq + LDW * kk : T + kk + LDW * jq;
+        const int d = onU ? LDW : 1;
+        const cplx gs = onU ? cconj(sn) : sn;
+        if (jq < m) {
           const cplx h1 = p1[0], h2 = p1[d];
-          cplx x1 = cadd(cscale(h1, c), cmul(g, h2)), x2 = csub(cscale(h2, c), cmulc(g, h1));
-          // the 2 x 2 block: lane 0 holds its column kk, lane 1 its column kk+1
-          const cplx o1 = cmk(__shfl_xor(x1.x, 1, 64), __shfl_xor(x1.y, 1, 64));
-          const cplx o2 = cmk(__shfl_xor(x2.x, 1, 64), __shfl_xor(x2.y, 1, 64));
-          cplx* a1 = p1;
-          cplx* a2 = p1 + d;
-          if (lane < 2) {  // row kk (lane 0) or kk+1 (lane 1) of the block, column rotation
-            const cplx pp = lane == 0 ? x1 : o2, qq = lane == 0 ? o1 : x2;
-            x1 = cadd(cscale(pp, c), cmulc(sn, qq));
-            x2 = csub(cscale(qq, c), cmul(sn, pp));
-            a1 = T + (kk + lane) + LDW * kk;
-            a2 = a1 + LDW;
-          }
-          if (act) {
-            *a1 = x1;
-            *a2 = x2;
-          }
-          if (lane < 2 && kk > lo) T[(kk + lane) + LDW * (kk - 1)] = lane == 0 ? r : cmk(0.0, 0.0);
+          p1[0] = cadd(cscale(h1, c), cmul(gs, h2));
+          p1[d] = csub(cscale(h2, c), cmulc(gs, h1));
         }
-      } else {
-        // rows kk, kk+1 (columns kk..m-1) on lanes 0..31, U columns kk, kk+1 on lanes
-        // 32..63, as one instruction stream: x1 <- c x1 + g x2, x2 <- c x2 - conj(g) x1
-        // with g = s on T's rows and conj(s) on U's columns
-        {
-          const bool onU = lane >= 32;
-          const int jq = onU ? lane - 32 : kk + lane;
-          cplx* const p1 = onU ? U + jq + LDW * kk : T + kk + LDW * jq;
-          const int d = onU ? LDW : 1;
-          const cplx gs = onU ? cconj(sn) : sn;
-          if (jq < m) {
-            const cplx h1 = p1[0], h2 = p1[d];
-            p1[0] = cadd(cscale(h1, c), cmul(gs, h2));
-            p1[d] = csub(cscale(h2, c), cmulc(gs, h1));
-          }
-          if (lane == 0 && kk > lo) {
-            T[kk + LDW * (kk - 1)] = r;
-            T[(kk + 1) + LDW * (kk - 1)] = cmk(0.0, 0.0);
-          }
+        if (lane == 0 && kk > lo) {
+          T[kk + LDW * (kk - 1)] = r;
+          T[(kk + 1) + LDW * (kk - 1)] = cmk(0.0, 0.0);
         }
-        wave_sync_lds();
-        // columns kk, kk+1 for rows 0..min(kk+2, hi)
-        if (lane <= min(kk + 2, hi)) {
-          const int q = lane;
-          const cplx h1 = T[q + LDW * kk], h2 = T[q + LDW * (kk + 1)];
-          T[q + LDW * kk] = cadd(cscale(h1, c), cmulc(sn, h2));
-          T[q + LDW * (kk + 1)] = csub(cscale(h2, c), cmul(sn, h1));
-        }
+      }
+      wave_sync_lds();
+      // columns kk, kk+1 for rows 0..min(kk+2, hi)
+      if (lane <= min(kk + 2, hi)) {
+        const int q = lane;
+        const cplx h1 = T[q + LDW * kk], h2 = T[q + LDW * (kk + 1)];
+        T[q + LDW * kk] = cadd(cscale(h1, c), cmulc(sn, h2));
+        T[q + LDW * (kk + 1)] = csub(cscale(h2, c), cmul(sn, h1));
       }
       wave_sync_lds();
     }
@@ -2523,7 +2469,7 @@ __global__ __launch_bounds__(256, OCC) void schur_qr_aed_kernel(int n, cplx* __r
       __syncthreads();
       if (wv == 0) {
         int ns = -1;
-        if (win_schur<LDW, (MB_NB > 4)>(m, win, Uw, lane) == 0) {
+        if (win_schur<LDW>(m, win, Uw, lane) == 0) {
           ns = m;
           if (s.x == 0.0 && s.y == 0.0) {
             ns = 0;
