@@ -88,6 +88,9 @@ ORF_DEV cplx load_op(const cplx* __restrict__ X, int ld, int r, int c) {
   return v;
 }
 
+// OPB = 3: B used as is (like 0) and upper triangular -- entries below its diagonal
+// are taken as zero and never read, and a tile column stops its k loop at the
+// diagonal (about half the products of a square triangular factor).
 template <int OPA, int OPB>
 __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(
     int M, int N, int K, cplx alpha, const cplx* __restrict__ A, int lda, long long sA,
@@ -121,9 +124,11 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(
   for (int q = 0; q < 4; ++q) {
     if (OPA == 0) { ra_r[q] = t & 63; ra_k[q] = (t >> 6) + 4 * q; }
     else { ra_k[q] = t & 15; ra_r[q] = (t >> 4) + 16 * q; }
-    if (OPB == 0) { rb_k[q] = t & 15; rb_c[q] = (t >> 4) + 16 * q; }
+    if (OPB == 0 || OPB == 3) { rb_k[q] = t & 15; rb_c[q] = (t >> 4) + 16 * q; }
     else { rb_c[q] = t & 63; rb_k[q] = (t >> 6) + 4 * q; }
   }
+  constexpr int OB = OPB == 3 ? 0 : OPB;
+  const int Kend = OPB == 3 ? min(K, n0 + GT_N) : K;
   // the next stage is fetched into registers while the current one feeds the MFMAs
   cplx pa[4], pb[4];
   auto fetch = [&](int k0) {
@@ -132,11 +137,11 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(
       const int gr = m0 + ra_r[q], gk = k0 + ra_k[q];
       pa[q] = (gr < M && gk < K) ? load_op<OPA>(A, lda, gr, gk) : cmk(0.0, 0.0);
       const int gk2 = k0 + rb_k[q], gc = n0 + rb_c[q];
-      pb[q] = (gk2 < K && gc < N) ? load_op<OPB>(B, ldb, gk2, gc) : cmk(0.0, 0.0);
+      pb[q] = (gk2 < K && gc < N && (OPB != 3 || gk2 <= gc)) ? load_op<OB>(B, ldb, gk2, gc) : cmk(0.0, 0.0);
     }
   };
   fetch(0);
-  for (int k0 = 0; k0 < K; k0 += GT_K) {
+  for (int k0 = 0; k0 < Kend; k0 += GT_K) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       Asr[ra_k[q]][ra_r[q]] = pa[q].x;
@@ -145,7 +150,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(
       Bsi[rb_k[q]][rb_c[q]] = pb[q].y;
     }
     __syncthreads();
-    if (k0 + GT_K < K) fetch(k0 + GT_K);
+    if (k0 + GT_K < Kend) fetch(k0 + GT_K);
 #pragma unroll
     for (int kk = 0; kk < GT_K; kk += 4) {
       const int kr = kk + (lane >> 4), rr = lane & 15;
@@ -383,7 +388,7 @@ void orf_zgemm(hipStream_t st, char opA, char opB, int M, int N, int K, cplx alp
   }
   dim3 grid((N + GT_N - 1) / GT_N, (M + GT_M - 1) / GT_M, batch);
   int oa = opA == 'N' ? 0 : (opA == 'T' ? 1 : 2);
-  int ob = opB == 'N' ? 0 : (opB == 'T' ? 1 : 2);
+  int ob = opB == 'N' ? 0 : (opB == 'T' ? 1 : (opB == 'U' ? 3 : 2));  // 'U': B upper triangular
 #define ORF_GEMM_CASE(a, b)                                                                       \
   if (oa == a && ob == b) {                                                                       \
     zgemm_mfma_kernel<a, b><<<grid, 256, 0, st>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, \
@@ -393,6 +398,7 @@ void orf_zgemm(hipStream_t st, char opA, char opB, int M, int N, int K, cplx alp
   ORF_GEMM_CASE(0, 0) ORF_GEMM_CASE(0, 1) ORF_GEMM_CASE(0, 2)
   ORF_GEMM_CASE(1, 0) ORF_GEMM_CASE(1, 1) ORF_GEMM_CASE(1, 2)
   ORF_GEMM_CASE(2, 0) ORF_GEMM_CASE(2, 1) ORF_GEMM_CASE(2, 2)
+  ORF_GEMM_CASE(0, 3) ORF_GEMM_CASE(1, 3) ORF_GEMM_CASE(2, 3)
 #undef ORF_GEMM_CASE
 }
 
@@ -3144,7 +3150,8 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
     else
       trevc_kernel<<<g2, 256, 0, st>>>(n, A, lda, sA, Ywork, n, sZ);
   }
-  orf_zgemm(st, 'N', 'N', n, n, n, cmk(1.0, 0.0), Zwork, n, sZ, Ywork, n, sZ, cmk(0.0, 0.0), V,
+  // V = Z Y with Y upper triangular (the eigenvectors of T)
+  orf_zgemm(st, 'N', 'U', n, n, n, cmk(1.0, 0.0), Zwork, n, sZ, Ywork, n, sZ, cmk(0.0, 0.0), V,
             ldv, sV, batch);
   colnorm_kernel<<<g2, 256, 0, st>>>(n, V, ldv, sV, vnorm);
   te("eig_vectors");

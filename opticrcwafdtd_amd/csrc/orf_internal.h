@@ -9,7 +9,8 @@ typedef std::function<void(const char*, int)> OrfTimeHook;  // (name, 0 = begin 
 void orf_set_lds_limit(int bytes);
 int orf_lds_limit();
 
-// C = alpha op(A) op(B) + beta C, batched; op in {'N','T','C'}; stride 0 broadcasts.
+// C = alpha op(A) op(B) + beta C, batched; op in {'N','T','C'} (opB also 'U': B as is and
+// upper triangular, its lower part neither read nor used); stride 0 broadcasts.
 void orf_zgemm(hipStream_t st, char opA, char opB, int M, int N, int K, cplx alpha, const cplx* A,
                int lda, long long sA, const cplx* B, int ldb, long long sB, cplx beta, cplx* C,
                int ldc, long long sC, int batch);

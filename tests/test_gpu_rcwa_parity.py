@@ -41,7 +41,9 @@ def pillars(radii, n_pillar=3.6 + 0.01j, h=0.6, period=0.4):
 @pytest.mark.parametrize("sym", ["none", "auto"])
 @pytest.mark.parametrize("pol", ["s", "p"])
 def test_pillar_sweep_pm3(sym, pol):
-    devs = pillars(np.linspace(0.04, 0.18, 5))
+    """The +-3 pillar sweep at its full size: 64 radii 40-180 nm, lambda 0.94 um, 256 x 256
+    grid (98 x 98 eigenproblems in the full basis), every result against the oracle."""
+    devs = pillars(np.linspace(0.04, 0.18, 64))
     g = orf.rcwa2D(devs, 0.94, 0, 0, pol, Nx=3, Nsample=(256, 256), Symmetry=sym)
     o = O.rcwa2d(devs, 0.94, 0, 0, pol, Nx=3, Nsample=(256, 256), Symmetry=sym)
     assert g["symmetry"] == (sym == "auto") == o["symmetry"]
