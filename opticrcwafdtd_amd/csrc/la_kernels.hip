@@ -1947,10 +1947,9 @@ __global__ __launch_bounds__(256) void schur_qr_kernel(int n, cplx* __restrict__
 // 8 / 24 / 32 positions per segment: 221 / 208 / 212 ms; whole strips loaded
 // at once (lookahead 16) instead of 8 steps ahead: 172 vs 182 ms.  The strips
 // take all 8 bulges' rotations in one sweep with 8 rotations between compiler
-// barriers: 172 ms, against 190 ms for two sweeps of 4 bulges (each strip
-// element loaded and stored twice per segment; same box, same run).
-// Deferring the rows of Z and above the windows over two segments (one sweep
-// of 32 steps) was 5 % slower: the larger strip code outweighs the traffic.
+// barriers: 170-172 ms, against 177 ms for two sweeps of 4 bulges (each strip
+// element loaded and stored twice per segment).  Deferring the rows of Z and
+// above the windows over two segments (one sweep of 32 steps) was 5 % slower.
 constexpr int MBQ_NB = 8, MBQ_SEG = 16, MBQ_OCC = 2, MBQ_GRP = 8, MBQ_NW = 16;
 // Small matrices: 4 bulges and a 12-wide window.  The 8-bulge kernel's static
 // LDS (40 KB) next to an H held in LDS leaves one workgroup per CU where this
@@ -2335,9 +2334,11 @@ ORF_DEV void strip_rotations(cplx* base, ptrdiff_t stride, int a0, int n, bool r
 //    window by one wave (bulge b trails bulge b-1 by three positions, so the
 //    rotations of one step act on disjoint pairs), MB_SEG positions per
 //    segment; the rows above the window, the columns right of it and Z take
-//    the segment's rotations afterwards, streamed through a register ring by
-//    all threads (strip_rotations): one load and one store per element per
-//    segment serve every shift of the pass.
+//    the segment's rotations afterwards, streamed through a register ring
+//    (strip_rotations): one load and one store per element per segment serve
+//    every shift of the pass.  Wave 0 gives the next window's first MB_SEG
+//    columns right their rotations after its chase, the other waves do the
+//    rest one segment behind, with one barrier per segment.
 //  * Early deflation (aedMode 0 and 1): before each pass the trailing NW x NW
 //    window is brought to Schur form in LDS (win_schur); eigenvalues whose
 //    spike entry is negligible deflate at once, from the bottom (mode 1) or
