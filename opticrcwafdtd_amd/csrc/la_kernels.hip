@@ -1676,6 +1676,132 @@ __global__ void panel_pair_t_kernel(int n, int ka, int iba, int ibb, const cplx*
   T2[(size_t)page * sT2 + q] = v;
 }
 
+// One pass of the GEMM formation of Q: Q22 := (I - V T V^H) Q22 (Q22 m x m, ld ldq;
+// V m x ib unit lower trapezoid, ld ldv; T ib x ib upper, ld ldt; ib <= 32) with one
+// read and one write of Q22.  A workgroup takes a 16-column block; wave w holds its
+// rows 16 (RTW w + i) + 0..15, i < RTW, in registers.  X = V^H Q_blk by MFMA over each
+// wave's rows, the four waves' parts summed in LDS, Y = T X, then Q_blk -= V Y by
+// MFMA.  The lane that feeds Q(r, c) to the X products as B operand (row 4 v + lane/16
+// of a 16-row tile, column lane % 16) holds the same element in the update's output
+// layout, so the block stays in registers between the two products.
+template <int RTW, int OCC>
+__global__ __launch_bounds__(256, OCC) void hess_q_pass_kernel(int m, int ib, const cplx* __restrict__ Vg, int ldv,
+                                                             long long sV, const cplx* __restrict__ Tg, int ldt,
+                                                             long long sT, cplx* __restrict__ Qg, int ldq,
+                                                             long long sQ, int pages) {
+  // per-wave parts of X (j, c) at j + XL c (odd stride: the 16 columns of one row fall
+  // in different banks); their sum goes to Xw[0], Y = T X to Xw[1]
+  constexpr int XL = 33;
+  __shared__ cplx Xw[4][XL * 16];
+  __shared__ cplx Ts[32 * 32];  // T (j, l) at j + 32 l, zero outside ib x ib
+  cplx* Ys = Xw[1];
+  const int nblk = (m + 15) / 16;
+  int page, blk;
+  xcd_page_tile(blockIdx.x, nblk, pages, page, blk);
+  const cplx* V = Vg + (size_t)page * sV;
+  const cplx* T = Tg + (size_t)page * sT;
+  cplx* Q = Qg + (size_t)page * sQ;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int cl = lane & 15, rq = lane >> 4;
+  const int col = blk * 16 + cl;
+  const int colc = min(col, m - 1);
+  for (int e = t; e < 32 * 32; e += 256) {
+    const int j = e & 31, l = e >> 5;
+    Ts[e] = (j <= l && l < ib) ? T[(size_t)j + (size_t)l * ldt] : cmk(0.0, 0.0);
+  }
+  cplx q[RTW][4];
+#pragma unroll
+  for (int i = 0; i < RTW; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 16 * (RTW * w + i) + 4 * v + rq;
+      const cplx x = Q[(size_t)min(r, m - 1) + (size_t)colc * ldq];
+      q[i][v] = (r < m && col < m) ? x : cmk(0.0, 0.0);
+    }
+  // ---- X = V^H Q_blk over this wave's rows (three real products per complex one)
+  const int njt = ib > 16 ? 2 : 1;
+  v4d a1[2], a2[2], a3[2];
+#pragma unroll
+  for (int jt = 0; jt < 2; ++jt) {
+    a1[jt] = (v4d){0.0, 0.0, 0.0, 0.0};
+    a2[jt] = (v4d){0.0, 0.0, 0.0, 0.0};
+    a3[jt] = (v4d){0.0, 0.0, 0.0, 0.0};
+  }
+#pragma unroll
+  for (int i = 0; i < RTW; ++i) {
+    if (16 * (RTW * w + i) >= m) break;
+    asm volatile("" ::: "memory");  // one tile's loads in flight at a time (registers)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 16 * (RTW * w + i) + 4 * v + rq;
+      const double br = q[i][v].x, bi = q[i][v].y;
+#pragma unroll
+      for (int jt = 0; jt < 2; ++jt) {
+        if (jt >= njt) break;
+        const int j = 16 * jt + cl;
+        const cplx x = V[(size_t)min(r, m - 1) + (size_t)min(j, ib - 1) * ldv];
+        const bool ok = r < m && j < ib;
+        const double ar = ok ? x.x : 0.0, ai = ok ? -x.y : 0.0;  // conj(V)
+        a1[jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, a1[jt], 0, 0, 0);
+        a2[jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, a2[jt], 0, 0, 0);
+        a3[jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, br + bi, a3[jt], 0, 0, 0);
+      }
+    }
+  }
+  // output layout: lane holds X (16 jt + lane/16 + 4 u, lane % 16)
+#pragma unroll
+  for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double p1 = a1[jt][u], p2 = a2[jt][u];
+      Xw[w][16 * jt + rq + 4 * u + XL * cl] = cmk(p1 - p2, a3[jt][u] - p1 - p2);
+    }
+  __syncthreads();
+  for (int e = t; e < XL * 16; e += 256) Xw[0][e] = cadd(cadd(Xw[0][e], Xw[1][e]), cadd(Xw[2][e], Xw[3][e]));
+  __syncthreads();
+  // ---- Y = T X (T upper triangular; Ts and the rows of X from ib on are zero, so all
+  // 32 terms go in unconditionally and the LDS loads pipeline)
+  for (int e = t; e < 32 * 16; e += 256) {
+    const int j = e & 31, c = e >> 5;
+    cplx y = cmk(0.0, 0.0);
+#pragma unroll 8
+    for (int l = 0; l < 32; ++l) y = cadd(y, cmul(Ts[j + 32 * l], Xw[0][l + XL * c]));
+    Ys[j + XL * c] = y;
+  }
+  __syncthreads();
+  // ---- Q_blk -= V Y; output layout: lane holds row 16 tile + lane/16 + 4 v, column lane % 16
+#pragma unroll
+  for (int i = 0; i < RTW; ++i) {
+    const int rb = 16 * (RTW * w + i);
+    if (rb >= m) break;
+    asm volatile("" ::: "memory");
+    v4d u1 = (v4d){0.0, 0.0, 0.0, 0.0}, u2 = u1, u3 = u1;
+    const int r = rb + cl;
+    // all eight 4-deep steps with clamped loads, issued together (Y is zero from row
+    // ib on): a loop bounded by ib waits out one memory latency per step
+    cplx vx[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) vx[s] = V[(size_t)min(r, m - 1) + (size_t)min(4 * s + rq, ib - 1) * ldv];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int k = 4 * s + rq;
+      const bool ok = r < m && k < ib;
+      const double ar = ok ? vx[s].x : 0.0, ai = ok ? vx[s].y : 0.0;
+      const cplx y = Ys[k + XL * cl];
+      u1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, y.x, u1, 0, 0, 0);
+      u2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, y.y, u2, 0, 0, 0);
+      u3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, y.x + y.y, u3, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int rr = rb + rq + 4 * v;
+      const double p1 = u1[v], p2 = u2[v];
+      if (rr < m && col < m)
+        Q[(size_t)rr + (size_t)col * ldq] = csub(q[i][v], cmk(p1 - p2, u3[v] - p1 - p2));
+    }
+  }
+}
+
 // W := Q W with Q = H_0 H_1 ... H_{n-3} the Hessenberg reduction's reflectors,
 // 16 columns per workgroup held in LDS.  form = 1 writes Q itself (W = I on
 // entry is implied): column j of Q only needs H_0 .. H_{j-1}, a third of the
@@ -3080,9 +3206,14 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
                   Tp + (size_t)iba * ib, ib, sY, batch);
       }
       cplx* Q22 = Zwork + (size_t)(ka + 1) * n + (ka + 1);
-      orf_zgemm(st, 'C', 'N', ib, m, m, cmk(1.0, 0.0), Vp, m, sY, Q22, n, sZ, cmk(0.0, 0.0), Xp, ib, sY, batch);
-      orf_zgemm(st, 'N', 'N', ib, m, ib, cmk(1.0, 0.0), Tp, ib, sY, Xp, ib, sY, cmk(0.0, 0.0), Ypp, ib, sY, batch);
-      orf_zgemm(st, 'N', 'N', m, m, ib, cmk(-1.0, 0.0), Vp, m, sY, Ypp, ib, sY, cmk(1.0, 0.0), Q22, n, sZ, batch);
+      if (m <= 256 && ib <= 32) {  // one read and one write of Q22 (block in registers)
+        hess_q_pass_kernel<4, 3><<<(unsigned)((long long)((m + 15) / 16) * batch), 256, 0, st>>>(
+            m, ib, Vp, m, sY, Tp, ib, sY, Q22, n, sZ, batch);
+      } else {
+        orf_zgemm(st, 'C', 'N', ib, m, m, cmk(1.0, 0.0), Vp, m, sY, Q22, n, sZ, cmk(0.0, 0.0), Xp, ib, sY, batch);
+        orf_zgemm(st, 'N', 'N', ib, m, ib, cmk(1.0, 0.0), Tp, ib, sY, Xp, ib, sY, cmk(0.0, 0.0), Ypp, ib, sY, batch);
+        orf_zgemm(st, 'N', 'N', m, m, ib, cmk(-1.0, 0.0), Vp, m, sY, Ypp, ib, sY, cmk(1.0, 0.0), Q22, n, sZ, batch);
+      }
       k0 = ka - NB;
     }
   } else {
