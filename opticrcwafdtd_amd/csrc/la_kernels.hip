@@ -881,7 +881,7 @@ __global__ __launch_bounds__(256) void diag_inv_kernel(int n, const cplx* __rest
 // Left solve A X = B with A = P L U: per 16-row block, X_blk = D^-1 X_blk then
 // the rows below (above) are updated; both steps are MFMA tile products on an
 // LDS block of CB right-hand sides.
-__global__ __launch_bounds__(256) void ztrsm_left_kernel(int n, int nrhs, int CB, const cplx* __restrict__ LU,
+__global__ __launch_bounds__(256, 2) void ztrsm_left_kernel(int n, int nrhs, int CB, const cplx* __restrict__ LU,
                                                          int lda, long long sA, const int* __restrict__ ipiv,
                                                          const cplx* __restrict__ dinv, long long sD,
                                                          cplx* __restrict__ Bg, int ldb, long long sB,
@@ -923,20 +923,62 @@ __global__ __launch_bounds__(256) void ztrsm_left_kernel(int n, int nrhs, int CB
   }
   __syncthreads();
   const int tcn = (nc + 15) / 16;
-  for (int pass = upper_only ? 1 : 0; pass < 2; ++pass) {  // 0: forward with L, 1: backward with U
-    for (int bi = 0; bi < nblk; ++bi) {
+  const int fr = lane & 15, kq = lane >> 4;
+  // Block steps: forward with L (pass 0), then backward with U (pass 1), 16 rows each.
+  // A step's inputs that do not depend on X -- the inverse of its diagonal block and
+  // the LU rows of this wave's first four update tiles -- are fetched one step ahead,
+  // while the step before runs (each step would otherwise wait out two memory
+  // latencies: most of the solve's time).
+  const int p0 = upper_only ? 1 : 0;
+  const int nsteps = (2 - p0) * nblk;
+  cplx dnx[4], anx[4][4];
+  auto fetch = [&](int step) {
+    const int pass = p0 + step / nblk, bi = step % nblk;
+    const int bk = pass == 0 ? bi : nblk - 1 - bi;
+    const int kb = bk * 16, nb = min(16, n - kb);
+    const cplx* D = (pass == 0 ? Linv : Uinv) + (size_t)bk * 256;
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) dnx[s4] = D[fr + 16 * (4 * s4 + kq)];
+    const int rlo = pass == 0 ? kb + nb : 0, rhi = pass == 0 ? n : kb;
+    const int trn = rhi > rlo ? (rhi - rlo + 15) / 16 : 1, ntile = rhi > rlo ? trn * tcn : 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int tile = wv + q * nw;
+      const int r0 = rlo + (tile % trn) * 16;
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const int k = 4 * s4 + kq;
+        const bool ok = tile < ntile && r0 + fr < rhi && k < nb;
+        const cplx a = LU[(size_t)min(r0 + fr, n - 1) + (size_t)min(kb + k, n - 1) * lda];
+        anx[q][s4] = ok ? a : cmk(0.0, 0.0);
+      }
+    }
+  };
+  if (nsteps > 0) fetch(0);
+  for (int step = 0; step < nsteps; ++step) {
+    {
+      const int pass = p0 + step / nblk, bi = step % nblk;
       const int bk = pass == 0 ? bi : nblk - 1 - bi;
       const int kb = bk * 16, nb = min(16, n - kb);
-      const cplx* D = (pass == 0 ? Linv : Uinv) + (size_t)bk * 256;
+      cplx dcur[4], acur[4][4];
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        dcur[s4] = dnx[s4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acur[q][s4] = anx[q][s4];
+      }
+      if (step + 1 < nsteps) fetch(step + 1);
       // X_blk = D X_blk (in place: each wave owns whole column tiles)
       for (int ct = wv; ct < tcn; ct += nw) {
         const int cc0 = ct * 16;
+        cplx bv[4];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const int k = 4 * s4 + kq;
+          bv[s4] = (k < nb && cc0 + fr < nc) ? X[kb + k + (cc0 + fr) * n] : cmk(0.0, 0.0);
+        }
         v4d aR = {0, 0, 0, 0}, aI = {0, 0, 0, 0};
-        ztile_mfma<4>([&](int r, int k) -> cplx { return D[r + 16 * k]; },
-                      [&](int k, int c) -> cplx {
-                        return (k < nb && cc0 + c < nc) ? X[kb + k + (cc0 + c) * n] : cmk(0.0, 0.0);
-                      },
-                      aR, aI);
+        ztile_mfma_frag(dcur, bv, aR, aI);
         const int cc = cc0 + (lane & 15);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -949,30 +991,35 @@ __global__ __launch_bounds__(256) void ztrsm_left_kernel(int n, int nrhs, int CB
       const int rlo = pass == 0 ? kb + nb : 0, rhi = pass == 0 ? n : kb;
       const int rows = rhi - rlo;
       if (rows > 0) {
-        // a wave's tiles four at a time, every LU fragment fetched before the first MFMA
+        // a wave's tiles four at a time (the first four with the fragments fetched
+        // ahead, any further ones -- column blocks wider than 16 -- fetched here)
         const int trn = (rows + 15) / 16, ntile = trn * tcn;
-        const int fr = lane & 15, kq = lane >> 4;
         for (int tb = wv; tb < ntile; tb += 4 * nw) {
-          cplx af[4][4], bf[4][4];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int tile = tb + q * nw;
-            const int r0 = rlo + (tile % trn) * 16, cc0 = (tile / trn) * 16;
-#pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) {
-              const int k = 4 * s4 + kq;
-              af[q][s4] = (tile < ntile && r0 + fr < rhi && k < nb) ? LU[(size_t)(r0 + fr) + (size_t)(kb + k) * lda]
-                                                                    : cmk(0.0, 0.0);
-              bf[q][s4] = (tile < ntile && k < nb && cc0 + fr < nc) ? X[kb + k + (cc0 + fr) * n] : cmk(0.0, 0.0);
-            }
-          }
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int tile = tb + q * nw;
             if (tile >= ntile) break;
             const int r0 = rlo + (tile % trn) * 16, cc0 = (tile / trn) * 16;
+            // LU rows: fetched ahead for the first four tiles, here for any further ones;
+            // the block's rows of X per tile (the updates write other rows)
+            cplx aq[4], bq[4];
+            if (tb == wv) {
+#pragma unroll
+              for (int s4 = 0; s4 < 4; ++s4) aq[s4] = acur[q][s4];
+            } else {
+#pragma unroll
+              for (int s4 = 0; s4 < 4; ++s4) {
+                const int k = 4 * s4 + kq;
+                aq[s4] = (r0 + fr < rhi && k < nb) ? LU[(size_t)(r0 + fr) + (size_t)(kb + k) * lda] : cmk(0.0, 0.0);
+              }
+            }
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+              const int k = 4 * s4 + kq;
+              bq[s4] = (k < nb && cc0 + fr < nc) ? X[kb + k + (cc0 + fr) * n] : cmk(0.0, 0.0);
+            }
             v4d aR = {0, 0, 0, 0}, aI = {0, 0, 0, 0};
-            ztile_mfma_frag(af[q], bf[q], aR, aI);
+            ztile_mfma_frag(aq, bq, aR, aI);
             const int cc = cc0 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
