@@ -1723,9 +1723,76 @@ __global__ void panel_pair_t_kernel(int n, int ka, int iba, int ibb, const cplx*
   T2[(size_t)page * sT2 + q] = v;
 }
 
+// T2 (ib x ib, ld ib, ib = iba + ibb) of the block reflector of panel a (iba
+// reflectors from column ka) and panel b (ibb from ka + iba; ibb = 0: panel a alone):
+// T2 = [Ta, -Ta (Va^H Vb) Tb; 0, Tb], one workgroup per page.  Ta and Tb come from the
+// rows above the panels' reflectors, where the reduction keeps them, Va and Vb from
+// its reflector storage (unit lower trapezoids from row ka + 1).
+__global__ __launch_bounds__(256) void hess_pair_t_kernel(int n, int ka, int iba, int ibb, const cplx* __restrict__ Vg,
+                                                          cplx* __restrict__ T2g, long long sT2) {
+  __shared__ cplx Ts[32 * 32];  // T2 (l, j) at l + 32 j
+  __shared__ cplx Ss[16 * 16];  // S = Va^H Vb, then W = S Tb, (i, j) at i + 16 j
+  const int page = blockIdx.x, t = threadIdx.x, ib = iba + ibb, m = n - ka - 1;
+  const cplx* V = Vg + (size_t)page * n * n;
+  cplx* T2 = T2g + (size_t)page * sT2;
+  for (int e = t; e < 32 * 32; e += 256) {
+    const int l = e & 31, j = e >> 5;
+    cplx v = cmk(0.0, 0.0);
+    if (j < iba && l <= j) v = V[(size_t)(ka + j) * n + l];
+    else if (j >= iba && j < ib && l >= iba && l <= j) v = V[(size_t)(ka + j) * n + (l - iba)];
+    Ts[e] = v;
+  }
+  if (ibb > 0) {
+    // S(i, j) = sum_r conj(Va(r, i)) Vb(r, j): Vb is zero above row iba + j and one
+    // there, where Va is below its diagonal (i < iba)
+    if (t < iba * ibb) {
+      const int i = t % iba, j = t / iba;
+      const cplx* va = V + (size_t)(ka + i) * n + ka + 1;
+      const cplx* vb = V + (size_t)(ka + iba + j) * n + ka + 1;
+      const int r0 = iba + j;
+      cplx acc = cconj(va[r0]);
+      cplx acc8[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc8[u] = cmk(0.0, 0.0);
+      for (int r = r0 + 1; r < m; r += 8) {
+        cplx a8[8], b8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          a8[u] = va[min(r + u, m - 1)];
+          b8[u] = vb[min(r + u, m - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (r + u < m) acc8[u] = cadd(acc8[u], cmulc(a8[u], b8[u]));
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = cadd(acc, acc8[u]);
+      Ss[i + 16 * j] = acc;
+    }
+    __syncthreads();
+    // W = S Tb
+    cplx w = cmk(0.0, 0.0);
+    const int i = t % 16, j = t / 16;
+    if (i < iba && j < ibb)
+      for (int l = 0; l <= j; ++l) w = cadd(w, cmul(Ss[i + 16 * l], Ts[(iba + l) + 32 * (iba + j)]));
+    __syncthreads();
+    if (i < iba && j < ibb) Ss[i + 16 * j] = w;
+    __syncthreads();
+    // T2(0:iba, iba:) = -Ta W
+    if (i < iba && j < ibb) {
+      cplx x = cmk(0.0, 0.0);
+      for (int l = i; l < iba; ++l) x = cadd(x, cmul(Ts[i + 32 * l], Ss[l + 16 * j]));
+      Ts[i + 32 * (iba + j)] = cneg(x);
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < ib * ib; e += 256) T2[e] = Ts[(e % ib) + 32 * (e / ib)];
+}
+
 // One pass of the GEMM formation of Q: Q22 := (I - V T V^H) Q22 (Q22 m x m, ld ldq;
-// V m x ib unit lower trapezoid, ld ldv; T ib x ib upper, ld ldt; ib <= 32) with one
-// read and one write of Q22.  A workgroup takes a 16-column block; wave w holds its
+// V m x ib unit lower trapezoid, ld ldv, read from the rows below its diagonal only --
+// the reduction's own storage of the reflectors serves; T ib x ib upper, ld ldt;
+// ib <= 32) with one read and one write of Q22.  A workgroup takes a 16-column block; wave w holds its
 // rows 16 (RTW w + i) + 0..15, i < RTW, in registers.  X = V^H Q_blk by MFMA over each
 // wave's rows, the four waves' parts summed in LDS, Y = T X, then Q_blk -= V Y by
 // MFMA.  The lane that feeds Q(r, c) to the X products as B operand (row 4 v + lane/16
@@ -1787,8 +1854,9 @@ __global__ __launch_bounds__(256, OCC) void hess_q_pass_kernel(int m, int ib, co
         if (jt >= njt) break;
         const int j = 16 * jt + cl;
         const cplx x = V[(size_t)min(r, m - 1) + (size_t)min(j, ib - 1) * ldv];
-        const bool ok = r < m && j < ib;
-        const double ar = ok ? x.x : 0.0, ai = ok ? -x.y : 0.0;  // conj(V)
+        const bool ok = r < m && j < ib && r >= j;
+        const cplx xv = r == j ? cmk(1.0, 0.0) : x;
+        const double ar = ok ? xv.x : 0.0, ai = ok ? -xv.y : 0.0;  // conj(V)
         a1[jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, a1[jt], 0, 0, 0);
         a2[jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, a2[jt], 0, 0, 0);
         a3[jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, br + bi, a3[jt], 0, 0, 0);
@@ -1832,8 +1900,9 @@ __global__ __launch_bounds__(256, OCC) void hess_q_pass_kernel(int m, int ib, co
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
       const int k = 4 * s + rq;
-      const bool ok = r < m && k < ib;
-      const double ar = ok ? vx[s].x : 0.0, ai = ok ? vx[s].y : 0.0;
+      const bool ok = r < m && k < ib && r >= k;
+      const cplx xv = r == k ? cmk(1.0, 0.0) : vx[s];
+      const double ar = ok ? xv.x : 0.0, ai = ok ? xv.y : 0.0;
       const cplx y = Ys[k + XL * cl];
       u1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, y.x, u1, 0, 0, 0);
       u2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, y.y, u2, 0, 0, 0);
@@ -3235,6 +3304,16 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
       cplx* Tp = Ywork + (size_t)3 * n * W;  // ib x ib
       cplx* Sp = Tp + W * W;                 // NB x NB
       cplx* Wp = Sp + NB * NB;
+      cplx* Q22 = Zwork + (size_t)(ka + 1) * n + (ka + 1);
+      if (m <= 256 && ib <= 32 && iba <= 16 && ibb <= 16) {
+        // T2 by one small kernel per page, then the pass with V read in place and the
+        // block of Q22 held in registers (one read and one write of Q22)
+        hess_pair_t_kernel<<<batch, 256, 0, st>>>(n, ka, iba, ibb, Vh, Tp, sY);
+        hess_q_pass_kernel<4, 3><<<(unsigned)((long long)((m + 15) / 16) * batch), 256, 0, st>>>(
+            m, ib, Vh + (size_t)ka * n + ka + 1, n, (long long)n * n, Tp, ib, sY, Q22, n, sZ, batch);
+        k0 = ka - NB;
+        continue;
+      }
       for (int b0 = 0; b0 < batch; b0 += 65535) {
         const int bb = batch - b0 < 65535 ? batch - b0 : 65535;
         panel_extract_kernel<<<dim3((unsigned)(((long long)m * ib + ib * ib + 255) / 256), bb), 256, 0, st>>>(
@@ -3252,15 +3331,9 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
         orf_zgemm(st, 'N', 'N', iba, ibb, iba, cmk(-1.0, 0.0), Tp, ib, sY, Wp, iba, sY, cmk(0.0, 0.0),
                   Tp + (size_t)iba * ib, ib, sY, batch);
       }
-      cplx* Q22 = Zwork + (size_t)(ka + 1) * n + (ka + 1);
-      if (m <= 256 && ib <= 32) {  // one read and one write of Q22 (block in registers)
-        hess_q_pass_kernel<4, 3><<<(unsigned)((long long)((m + 15) / 16) * batch), 256, 0, st>>>(
-            m, ib, Vp, m, sY, Tp, ib, sY, Q22, n, sZ, batch);
-      } else {
-        orf_zgemm(st, 'C', 'N', ib, m, m, cmk(1.0, 0.0), Vp, m, sY, Q22, n, sZ, cmk(0.0, 0.0), Xp, ib, sY, batch);
-        orf_zgemm(st, 'N', 'N', ib, m, ib, cmk(1.0, 0.0), Tp, ib, sY, Xp, ib, sY, cmk(0.0, 0.0), Ypp, ib, sY, batch);
-        orf_zgemm(st, 'N', 'N', m, m, ib, cmk(-1.0, 0.0), Vp, m, sY, Ypp, ib, sY, cmk(1.0, 0.0), Q22, n, sZ, batch);
-      }
+      orf_zgemm(st, 'C', 'N', ib, m, m, cmk(1.0, 0.0), Vp, m, sY, Q22, n, sZ, cmk(0.0, 0.0), Xp, ib, sY, batch);
+      orf_zgemm(st, 'N', 'N', ib, m, ib, cmk(1.0, 0.0), Tp, ib, sY, Xp, ib, sY, cmk(0.0, 0.0), Ypp, ib, sY, batch);
+      orf_zgemm(st, 'N', 'N', m, m, ib, cmk(-1.0, 0.0), Vp, m, sY, Ypp, ib, sY, cmk(1.0, 0.0), Q22, n, sZ, batch);
       k0 = ka - NB;
     }
   } else {
