@@ -375,6 +375,51 @@ void orf_zgemm_f32(hipStream_t st, char opA, char opB, int M, int N, int K, cplx
 #undef ORF_GEMM_CASE
 }
 
+// C = alpha A B + beta C for a few columns (N <= NC): one thread per row of A streams
+// the row against B held in LDS (K x NC), sixteen loads in flight; A is read once.
+// The 64 x 64 MFMA tiles would leave most of each tile empty and wait out a memory
+// latency per 16-deep step.
+namespace {
+template <int NC>
+__global__ __launch_bounds__(256) void zgemm_skinny_kernel(int M, int N, int K, cplx alpha, const cplx* __restrict__ A,
+                                                           int lda, long long sA, const cplx* __restrict__ B,
+                                                           int ldb, long long sB, cplx beta, cplx* __restrict__ C,
+                                                           int ldc, long long sC) {
+  extern __shared__ cplx Bs[];  // B (k, c) at k NC + c, zero for c >= N
+  const int page = blockIdx.y, t = threadIdx.x, r = blockIdx.x * 256 + t;
+  A += (size_t)page * sA;
+  B += (size_t)page * sB;
+  C += (size_t)page * sC;
+  lds_fill(t, 256, K * NC,
+           [&](int e) { return (e % NC) < N ? B[(size_t)(e / NC) + (size_t)(e % NC) * ldb] : cmk(0.0, 0.0); },
+           [&](int e) -> cplx& { return Bs[e]; });
+  __syncthreads();
+  if (r >= M) return;
+  cplx acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = cmk(0.0, 0.0);
+  const cplx* ar = A + r;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    cplx a16[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) a16[u] = ar[(size_t)min(k0 + u, K - 1) * lda];
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (k0 + u < K)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] = cadd(acc[c], cmul(a16[u], Bs[(k0 + u) * NC + c]));
+  }
+  const bool beta0 = beta.x == 0.0 && beta.y == 0.0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    if (c < N) {
+      cplx out = cmul(alpha, acc[c]);
+      if (!beta0) out = cadd(out, cmul(beta, C[(size_t)r + (size_t)c * ldc]));
+      C[(size_t)r + (size_t)c * ldc] = out;
+    }
+}
+}  // namespace
+
 void orf_zgemm(hipStream_t st, char opA, char opB, int M, int N, int K, cplx alpha, const cplx* A,
                int lda, long long sA, const cplx* B, int ldb, long long sB, cplx beta, cplx* C,
                int ldc, long long sC, int batch) {
@@ -385,6 +430,16 @@ void orf_zgemm(hipStream_t st, char opA, char opB, int M, int N, int K, cplx alp
     B += (size_t)sB * 65535;
     C += (size_t)sC * 65535;
     batch -= 65535;
+  }
+  if (opA == 'N' && opB == 'N' && N <= 8 && K >= 1 && (size_t)K * 8 * sizeof(cplx) <= 65536) {
+    dim3 g((M + 255) / 256, batch);
+    if (N <= 4)
+      zgemm_skinny_kernel<4><<<g, 256, (size_t)K * 4 * sizeof(cplx), st>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB,
+                                                                           beta, C, ldc, sC);
+    else
+      zgemm_skinny_kernel<8><<<g, 256, (size_t)K * 8 * sizeof(cplx), st>>>(M, N, K, alpha, A, lda, sA, B, ldb, sB,
+                                                                           beta, C, ldc, sC);
+    return;
   }
   dim3 grid((N + GT_N - 1) / GT_N, (M + GT_M - 1) / GT_M, batch);
   int oa = opA == 'N' ? 0 : (opA == 'T' ? 1 : 2);
