@@ -3183,7 +3183,7 @@ __global__ __launch_bounds__(256) void trevc_kernel(int n, const cplx* __restric
 // the rows above a block are summed over its columns.
 constexpr int TV_EB = 16;
 template <int RPT>
-__global__ __launch_bounds__(256) void trevc_blocked_kernel(int n, const cplx* __restrict__ T, int ldt,
+__global__ __launch_bounds__(256, RPT == 16 ? 3 : 1) void trevc_blocked_kernel(int n, const cplx* __restrict__ T, int ldt,
                                                             long long sT, cplx* __restrict__ Y, int ldy,
                                                             long long sY) {
   __shared__ cplx tdiag[256];  // diagonal block, column-major 16 x 16
@@ -3219,13 +3219,25 @@ __global__ __launch_bounds__(256) void trevc_blocked_kernel(int n, const cplx* _
 #pragma unroll
     for (int sl = 0; sl < RPT; ++sl)
       if (sl == sb) xb = x[sl];
+    // the pivot of this lane's row, T(j0+g, j0+g) - T(k, k) (small-pivot rule applied),
+    // as Smith's quotient x / d = (x.x + x.y r, x.y - x.x r) / (d.x + d.y r) (|d.x| >= |d.y|,
+    // else mirrored) with r and the reciprocal of the denominator formed here, off the
+    // serial chain below, which then multiplies instead of dividing
+    bool swp;
+    double sr, sid;
+    {
+      cplx d = csub(tdiag[g + 16 * g], tkk);
+      if (cabs1(d) < smin) d = cmk(smin, 0.0);
+      swp = fabs(d.x) < fabs(d.y);
+      sr = swp ? d.x / d.y : d.y / d.x;
+      sid = 1.0 / (swp ? d.y + d.x * sr : d.x + d.y * sr);
+    }
     for (int j = jhi; j >= j0; --j) {
       const int gj = j - j0;
       cplx xj = cmk(0.0, 0.0);
       if (g == gj && act && k > j) {
-        cplx d = csub(tdiag[gj + 16 * gj], tkk);
-        if (cabs1(d) < smin) d = cmk(smin, 0.0);
-        xb = cdiv(xb, d);
+        xb = swp ? cmk((xb.x * sr + xb.y) * sid, (xb.y * sr - xb.x) * sid)
+                 : cmk((xb.x + xb.y * sr) * sid, (xb.y - xb.x * sr) * sid);
         xj = xb;
       }
       xj.x = __shfl(xj.x, gj, 16);
