@@ -1504,7 +1504,7 @@ ORF_DEV cplx ydot_sub(const cplx* Y, int n, int r, int i, cplx x, const cplx* sw
 // three within the CU for n up to 249): measured 4 % faster than 16-wide
 // panels at two per CU, whose extra resident pages hide more of the HBM latency
 // of the per-reflector mat-vec.
-template <int NB>
+template <int NB, int SL>  // SL: 64-row slots of a column in the left update (n <= 64 SL + 1)
 __global__ __launch_bounds__(256, 3) void hessenberg_blocked_kernel(int n, cplx* __restrict__ Ag, int lda,
                                                                  long long sA, cplx* __restrict__ Vg,
                                                                  cplx* __restrict__ taug,
@@ -1690,41 +1690,48 @@ __global__ __launch_bounds__(256, 3) void hessenberg_blocked_kernel(int n, cplx*
       AA(r, cc) = csub(AA(r, cc), s);
     }
     __syncthreads();
-    // (h) trailing columns from the left, rows k0+1..: A -= V T^H (V^H A), a wave per column
-    for (int q = q0 + w; q < n; q += 4) {
-      cplx x[TREVC_SLOTS_H];
-      cplx part[NB];
+    // (h) trailing columns from the left, rows k0+1..: A -= V T^H (V^H A), a wave per
+    // column held in SL 64-row slots (SL sized to n: idle slots cost registers)
+    {
+      cplx x[SL];
+      for (int q = q0 + w; q < n; q += 4) {
 #pragma unroll
-      for (int j = 0; j < NB; ++j) part[j] = cmk(0.0, 0.0);
-#pragma unroll
-      for (int sl = 0; sl < TREVC_SLOTS_H; ++sl) {
-        const int r = k0 + 1 + lane + 64 * sl;
-        x[sl] = (r < n) ? AA(r, q) : cmk(0.0, 0.0);
-        if (r < n) {
-#pragma unroll
-          for (int j = 0; j < NB; ++j)
-            if (j < ib && r >= k0 + j + 1) part[j] = cadd(part[j], cmulc(VS(r, j), x[sl]));
+        for (int sl = 0; sl < SL; ++sl) {
+          const int r = k0 + 1 + lane + 64 * sl;
+          x[sl] = (r < n) ? AA(r, q) : cmk(0.0, 0.0);
         }
-      }
+        cplx part[NB];
 #pragma unroll
-      for (int j = 0; j < NB; ++j) part[j] = cmk(wave_sum(part[j].x), wave_sum(part[j].y));
-      // W := T^H W in place, last entry first (entry j needs W_0..W_j)
+        for (int j = 0; j < NB; ++j) part[j] = cmk(0.0, 0.0);
 #pragma unroll
-      for (int j = NB - 1; j >= 0; --j) {
-        cplx s = cmk(0.0, 0.0);
+        for (int sl = 0; sl < SL; ++sl) {
+          const int r = k0 + 1 + lane + 64 * sl;
+          if (r < n) {
 #pragma unroll
-        for (int l = 0; l <= j; ++l) s = cadd(s, cmulc(Ts[l + NB * j], part[l]));
-        part[j] = j < ib ? s : cmk(0.0, 0.0);
-      }
+            for (int j = 0; j < NB; ++j)
+              if (j < ib && r >= k0 + j + 1) part[j] = cadd(part[j], cmulc(VS(r, j), x[sl]));
+          }
+        }
 #pragma unroll
-      for (int sl = 0; sl < TREVC_SLOTS_H; ++sl) {
-        const int r = k0 + 1 + lane + 64 * sl;
-        if (r < n) {
+        for (int j = 0; j < NB; ++j) part[j] = cmk(wave_sum(part[j].x), wave_sum(part[j].y));
+        // W := T^H W in place, last entry first (entry j needs W_0..W_j)
+#pragma unroll
+        for (int j = NB - 1; j >= 0; --j) {
           cplx s = cmk(0.0, 0.0);
 #pragma unroll
-          for (int j = 0; j < NB; ++j)
-            if (j < ib && r >= k0 + j + 1) s = cadd(s, cmul(VS(r, j), part[j]));
-          AA(r, q) = csub(x[sl], s);
+          for (int l = 0; l <= j; ++l) s = cadd(s, cmulc(Ts[l + NB * j], part[l]));
+          part[j] = j < ib ? s : cmk(0.0, 0.0);
+        }
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) {
+          const int r = k0 + 1 + lane + 64 * sl;
+          if (r < n) {
+            cplx s = cmk(0.0, 0.0);
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+              if (j < ib && r >= k0 + j + 1) s = cadd(s, cmul(VS(r, j), part[j]));
+            AA(r, q) = csub(x[sl], s);
+          }
         }
       }
     }
@@ -3337,10 +3344,12 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
     const int tv = orf_tune_value(ORF_TUNE_HESS, n, batch);
     if (tv == 0 || (tv == 12 && ok12) || (tv == 8 && ok8)) blockedNB = tv;
   }
-  if (blockedNB == 12)
-    hessenberg_blocked_kernel<12><<<batch, 256, pan12, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
+  if (blockedNB == 12 && n <= 257)
+    hessenberg_blocked_kernel<12, 4><<<batch, 256, pan12, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
+  else if (blockedNB == 12)
+    hessenberg_blocked_kernel<12, 12><<<batch, 256, pan12, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
   else if (blockedNB == 8)
-    hessenberg_blocked_kernel<8><<<batch, 256, pan8, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
+    hessenberg_blocked_kernel<8, 12><<<batch, 256, pan8, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
   else
     hessenberg_noq_kernel<<<batch, 256, (use_lds ? bytes : 0) + (size_t)n * sizeof(cplx), st>>>(n, A, lda, sA, Vh,
                                                                                                tau, use_lds);

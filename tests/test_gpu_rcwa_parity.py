@@ -251,8 +251,11 @@ def test_mixed_precision_error_is_bounded():
     """The reduced-precision forms on the +-5 phase-library batch (256 radii x 8 wavelengths,
     full 242 x 242 basis, TE + TM): Precision="mixed" (FP32-MFMA convolution and star products)
     and "split" (hi + lo FP32 split products, FP64 accumulation), every efficiency against the
-    FP64 path at the bounds measured on this batch (mixed 2.6e-5, split 1.6e-6; neither meets
-    1e-6), and sampled pages of the FP64 path against the oracle."""
+    FP64 path, and sampled pages of the FP64 path against the oracle.  Measured on this batch:
+    max |dT| mixed 2.6-2.8e-5, split 1.2-2.1e-6 (neither meets 1e-6); mean |dT| mixed 5.8e-9,
+    split 3.5e-10.  The max over the 4096 solves follows the worst-conditioned page and moves
+    by up to 2x between builds whose FP64 path differs in the last bits (1e-12 in T), so the
+    max bounds carry that spread and the means, which do not move, are bounded tightly."""
     radii = np.linspace(0.040, 0.180, 256)
     lams = list(np.linspace(0.900, 0.980, 8))
     devs = pillars(radii)
@@ -266,10 +269,12 @@ def test_mixed_precision_error_is_bounded():
         finally:
             plan.close()
     f = out["fp64"]
-    for prec, bound in (("mixed", 3e-5), ("split", 2e-6)):
+    for prec, bound, mbound in (("mixed", 5e-5, 1e-8), ("split", 4e-6, 1e-9)):
         dT = np.max(np.abs(out[prec]["T"] - f["T"]))
         dR = np.max(np.abs(out[prec]["R"] - f["R"]))
         assert 1e-12 < max(dT, dR) <= bound, (prec, dT, dR)  # the FP32 products are really used
+        mT = np.mean(np.abs(out[prec]["T"] - f["T"]))
+        assert mT <= mbound, (prec, mT)
     assert np.max(np.abs(out["split"]["T"] - f["T"])) < np.max(np.abs(out["mixed"]["T"] - f["T"])) / 4
     o = O.rcwa2d(pillars([radii[200]]), lams[3], 0, 0, "p", Nx=5, Nsample=(256, 256), Symmetry="none")
     np.testing.assert_allclose(f["T"][:, 200, 3, 1], o["T"][0], atol=TOL, rtol=0)
