@@ -1504,6 +1504,36 @@ ORF_DEV cplx ydot_sub(const cplx* Y, int n, int r, int i, cplx x, const cplx* sw
 // three within the CU for n up to 249): measured 4 % faster than 16-wide
 // panels at two per CU, whose extra resident pages hide more of the HBM latency
 // of the per-reflector mat-vec.
+// 32 sums over a wave at once: each butterfly step a lane keeps half of the values it
+// carries and adds its partner's copy of that half (CNT values left, partner lane
+// l ^ O); once one value is left, plain xor steps.  The pairs added, and their order,
+// are those of wave_sum, so each sum equals wave_sum's; lane l ends with value l >> 1.
+template <int CNT, int O>
+ORF_DEV void wave_sum_32(double (&v)[32], int lane) {
+  if constexpr (O > 0) {
+    if constexpr (CNT > 1) {
+      constexpr int H = CNT / 2;
+      const bool hi = (lane & O) != 0;
+#pragma unroll
+      for (int i = 0; i < H; ++i) {
+        const double keep = hi ? v[H + i] : v[i], send = hi ? v[i] : v[H + i];
+        v[i] = keep + __shfl_xor(send, O, 64);
+      }
+      wave_sum_32<H, O / 2>(v, lane);
+    } else {
+      v[0] += __shfl_xor(v[0], O, 64);
+      wave_sum_32<1, O / 2>(v, lane);
+    }
+  }
+}
+// x of lane ln (ln uniform), as a uniform value
+ORF_DEV double lane_value(double x, int ln) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, ln);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), ln);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 template <int NB, int SL>  // SL: 64-row slots of a column in the left update (n <= 64 SL + 1)
 __global__ __launch_bounds__(256, 3) void hessenberg_blocked_kernel(int n, cplx* __restrict__ Ag, int lda,
                                                                  long long sA, cplx* __restrict__ Vg,
@@ -1712,8 +1742,19 @@ __global__ __launch_bounds__(256, 3) void hessenberg_blocked_kernel(int n, cplx*
               if (j < ib && r >= k0 + j + 1) part[j] = cadd(part[j], cmulc(VS(r, j), x[sl]));
           }
         }
+        // the 2 NB sums over the wave by a butterfly (wave_sum_32): 62 shuffles for
+        // all of them instead of 6 per value, the same pairs added in the same order
+        {
+          double v[32];
 #pragma unroll
-        for (int j = 0; j < NB; ++j) part[j] = cmk(wave_sum(part[j].x), wave_sum(part[j].y));
+          for (int j = 0; j < 16; ++j) {
+            v[2 * j] = j < NB ? part[j].x : 0.0;
+            v[2 * j + 1] = j < NB ? part[j].y : 0.0;
+          }
+          wave_sum_32<32, 32>(v, lane);
+#pragma unroll
+          for (int j = 0; j < NB; ++j) part[j] = cmk(lane_value(v[0], 4 * j), lane_value(v[0], 4 * j + 2));
+        }
         // W := T^H W in place, last entry first (entry j needs W_0..W_j)
 #pragma unroll
         for (int j = NB - 1; j >= 0; --j) {
