@@ -1755,13 +1755,17 @@ __global__ __launch_bounds__(256, 3) void hessenberg_blocked_kernel(int n, cplx*
 #pragma unroll
           for (int j = 0; j < NB; ++j) part[j] = cmk(lane_value(v[0], 4 * j), lane_value(v[0], 4 * j + 2));
         }
-        // W := T^H W in place, last entry first (entry j needs W_0..W_j)
-#pragma unroll
-        for (int j = NB - 1; j >= 0; --j) {
+        // W := T^H W: lane j < NB forms entry j (sum over l <= j, l ascending), then the
+        // entries are read out uniform again
+        {
+          const int jl = lane < NB ? lane : NB - 1;
           cplx s = cmk(0.0, 0.0);
 #pragma unroll
-          for (int l = 0; l <= j; ++l) s = cadd(s, cmulc(Ts[l + NB * j], part[l]));
-          part[j] = j < ib ? s : cmk(0.0, 0.0);
+          for (int l = 0; l < NB; ++l)
+            if (l <= jl) s = cadd(s, cmulc(Ts[l + NB * jl], part[l]));
+          if (jl >= ib) s = cmk(0.0, 0.0);
+#pragma unroll
+          for (int j = 0; j < NB; ++j) part[j] = cmk(lane_value(s.x, j), lane_value(s.y, j));
         }
 #pragma unroll
         for (int sl = 0; sl < SL; ++sl) {
