@@ -1661,22 +1661,33 @@ __global__ __launch_bounds__(256, 3) void hessenberg_blocked_kernel(int n, cplx*
       const int l = e % ib, j = e / ib;
       if (l <= j) V[(size_t)(k0 + j) * n + l] = Ts[l + NB * j];
     }
-    // (e) rows 0..k0 of Y: P = A(0:k0, k0+1:) V in Y's columns NB.., then Y = P T
-    for (int e = t; e < (k0 + 1) * ib; e += nt) {
-      const int r = e % (k0 + 1), j = e / (k0 + 1);
-      cplx p4[4];
+    // (e) rows 0..k0 of Y: P = A(0:k0, k0+1:) V in Y's columns NB.., then Y = P T; a
+    // work item takes a row and three reflectors, so the block of A is read ib / 3
+    // times per panel instead of ib
+    {
+      constexpr int JG = 3;
+      const int ngr = (ib + JG - 1) / JG;
+      for (int e = t; e < (k0 + 1) * ngr; e += nt) {
+        const int r = e % (k0 + 1), jb = (e / (k0 + 1)) * JG;
+        cplx acc[JG];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) p4[u] = cmk(0.0, 0.0);
-      int q = k0 + j + 1;
-      for (; q + 3 < n; q += 4) {
-        cplx a4[4];
+        for (int jj = 0; jj < JG; ++jj) acc[jj] = cmk(0.0, 0.0);
+        for (int q = k0 + jb + 1; q < n; q += 8) {
+          cplx a8[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) a4[u] = AA(r, q + u);
+          for (int u = 0; u < 8; ++u) a8[u] = AA(r, min(q + u, n - 1));
 #pragma unroll
-        for (int u = 0; u < 4; ++u) p4[u] = cadd(p4[u], cmul(a4[u], VS(q + u, j)));
+          for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int jj = 0; jj < JG; ++jj) {
+              const int j = jb + jj;
+              if (q + u < n && j < ib && q + u >= k0 + j + 1) acc[jj] = cadd(acc[jj], cmul(a8[u], VS(q + u, j)));
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < JG; ++jj)
+          if (jb + jj < ib) YY(r, NB + jb + jj) = acc[jj];
       }
-      for (; q < n; ++q) p4[0] = cadd(p4[0], cmul(AA(r, q), VS(q, j)));
-      YY(r, NB + j) = cadd(cadd(p4[0], p4[1]), cadd(p4[2], p4[3]));
     }
     __syncthreads();
     for (int e = t; e < (k0 + 1) * ib; e += nt) {
