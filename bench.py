@@ -228,7 +228,7 @@ def extras(args, full):
                                    "p50_ms": round(sat["p50_ms"], 2), "p99_ms": round(sat["p99_ms"], 2)},
                          "capacity": {"req_per_s": round(C, 2), "search": cap["search"],
                                       "what": "largest Poisson arrival rate that the service kept up with "
-                                              "(achieved_frac >= 0.99 over the same arrival window as the loads), searched from "
+                                              "(achieved_frac >= 0.99 over the same arrival window as the loads, by two request streams), searched from "
                                               "the burst rate: the loads below are fractions of it"},
                          "poisson": loads, "achieved_rate": "over the 10th-90th percentile arrival window: "
                                                             "achieved_frac = service / arrival rate from the "

@@ -391,15 +391,24 @@ def capacity(server: Server, rate0: float, seed: int, seconds: float = 5.0, step
     """Largest sustained Poisson arrival rate: starting at ``rate0`` (e.g. the
     burst rate), each step offers ``seconds`` worth of requests and raises the
     rate by 15 % while the service keeps up with the arrivals over the
-    steady window (``achieved_frac`` of :func:`run_load` >= ``sustain``), else bisects towards the last sustained rate.  Returns
-    the sustained rate and the steps taken."""
+    steady window (``achieved_frac`` of :func:`run_load` >= ``sustain``), else
+    bisects towards the last sustained rate.  A rate counts as sustained only
+    when a second request stream (another seed: the mix of ladder and sweep
+    sizes, and so the work per request, varies by about 10 % between streams)
+    keeps up too.  Returns the sustained rate and the probes taken."""
     lo, hi, rate, log = None, None, float(rate0), []
     for k in range(steps):
-        n = int(max(200, rate * seconds))
-        r = run_load(server, random_requests(n, seed + k), rate, seed=seed + k)
-        ok = r["achieved_frac"] >= sustain
-        log.append({"offered": round(rate, 2), "arrived": round(r["arrived_req_per_s"], 2),
-                    "achieved": round(r["steady_req_per_s"], 2), "sustained": bool(ok)})
+        ok = True
+        for rep in range(2):
+            sd = seed + 2 * k + rep
+            n = int(max(200, rate * seconds))
+            r = run_load(server, random_requests(n, sd), rate, seed=sd)
+            ok_r = r["achieved_frac"] >= sustain
+            log.append({"offered": round(rate, 2), "seed": sd, "arrived": round(r["arrived_req_per_s"], 2),
+                        "achieved": round(r["steady_req_per_s"], 2), "sustained": bool(ok_r)})
+            if not ok_r:
+                ok = False
+                break
         if ok:
             lo = rate
             rate = rate * 1.15 if hi is None else 0.5 * (rate + hi)
