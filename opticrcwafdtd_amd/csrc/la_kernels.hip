@@ -983,7 +983,7 @@ __global__ __launch_bounds__(256, 2) void ztrsm_left_kernel(int n, int nrhs, int
   // A step's inputs that do not depend on X -- the inverse of its diagonal block and
   // the LU rows of this wave's first four update tiles -- are fetched one step ahead,
   // while the step before runs (each step would otherwise wait out two memory
-  // latencies: most of the solve's time).
+  // latencies before its products).
   const int p0 = upper_only ? 1 : 0;
   const int nsteps = (2 - p0) * nblk;
   cplx dnx[4], anx[4][4];
@@ -1910,8 +1910,8 @@ __global__ __launch_bounds__(256) void hess_pair_t_kernel(int n, int ka, int iba
 // One pass of the GEMM formation of Q: Q22 := (I - V T V^H) Q22 (Q22 m x m, ld ldq;
 // V m x ib unit lower trapezoid, ld ldv, read from the rows below its diagonal only --
 // the reduction's own storage of the reflectors serves; T ib x ib upper, ld ldt;
-// ib <= 32) with one read and one write of Q22.  A workgroup takes a 16-column block; wave w holds its
-// rows 16 (RTW w + i) + 0..15, i < RTW, in registers.  X = V^H Q_blk by MFMA over each
+// ib <= 32) with one read and one write of Q22.  A workgroup takes a 16-column
+// block; wave w holds its rows 16 (RTW w + i) + 0..15, i < RTW, in registers.  X = V^H Q_blk by MFMA over each
 // wave's rows, the four waves' parts summed in LDS, Y = T X, then Q_blk -= V Y by
 // MFMA.  The lane that feeds Q(r, c) to the X products as B operand (row 4 v + lane/16
 // of a 16-row tile, column lane % 16) holds the same element in the update's output

@@ -21,7 +21,10 @@ def _f(a):
 
 @pytest.mark.parametrize("opA,opB,m,n,k,batch", [
     ("N", "N", 17, 23, 9, 3), ("N", "N", 98, 98, 98, 2), ("T", "N", 64, 130, 70, 2),
-    ("N", "C", 242, 61, 100, 2), ("C", "T", 33, 65, 129, 1)])
+    ("N", "C", 242, 61, 100, 2), ("C", "T", 33, 65, 129, 1),
+    # few columns: the row-streaming product (4 and 8 column variants, rows past 256),
+    # and a long inner dimension that goes back to the MFMA tiles
+    ("N", "N", 242, 3, 242, 3), ("N", "N", 300, 7, 61, 2), ("N", "N", 50, 2, 600, 1)])
 def test_pagemtimes(opA, opB, m, n, k, batch):
     rng = np.random.default_rng(1)
     A = _crand(rng, batch, m if opA == "N" else k, k if opA == "N" else m)
