@@ -393,9 +393,10 @@ def capacity(server: Server, rate0: float, seed: int, seconds: float = 5.0, step
     rate by 15 % while the service keeps up with the arrivals over the
     steady window (``achieved_frac`` of :func:`run_load` >= ``sustain``), else
     bisects towards the last sustained rate.  A rate counts as sustained only
-    when a second request stream (another seed: the mix of ladder and sweep
-    sizes, and so the work per request, varies by about 10 % between streams)
-    keeps up too.  Returns the sustained rate and the probes taken."""
+    when a second request stream (another seed) keeps up too: at one offered
+    rate the achieved rate of single streams scatters by 5-10 % (arrival
+    sequence, request parameters, host timing), so one lucky stream would set
+    the capacity too high.  Returns the sustained rate and the probes taken."""
     lo, hi, rate, log = None, None, float(rate0), []
     for k in range(steps):
         ok = True

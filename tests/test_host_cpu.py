@@ -134,3 +134,29 @@ def test_slanted_polygon_split_raises():
         with pytest.raises(orf.OrfError) as e:
             orf.slantedLayers(0.3, mc(1), shp, 80, NSlices=4)   # largest inset 0.0463 > 0.02
         assert e.value.identifier == "orf:slantedLayers:split"
+
+
+def test_capacity_needs_two_streams_to_keep_up(monkeypatch):
+    """The capacity search counts a rate as sustained only when two request streams of
+    different seeds keep up; a stream that keeps up alone does not raise it."""
+    from opticrcwafdtd_amd import serving
+    limit = {5: 400.0, 6: 300.0}  # per seed: the rate that stream keeps up with (others: 350)
+    seen = []
+
+    def fake_run_load(server, requests, rate, seed=0):
+        seen.append((round(rate, 3), seed))
+        ok = rate <= limit.get(seed, 350.0)
+        return {"achieved_frac": 1.0 if ok else 0.9, "arrived_req_per_s": rate, "steady_req_per_s": rate}
+
+    monkeypatch.setattr(serving, "run_load", fake_run_load)
+    monkeypatch.setattr(serving, "random_requests", lambda n, seed: [None] * n)
+    cap = serving.capacity(None, 320.0, seed=5, seconds=1.0, steps=6)
+    # 320: seed 5 keeps up, seed 6 does not -> not sustained, even though seed 5 alone would be
+    assert cap["search"][0]["sustained"] and not cap["search"][1]["sustained"]
+    assert cap["req_per_s"] <= 350.0
+    assert all(p["sustained"] for p in cap["search"] if p["offered"] <= 300.0)
+    # every sustained rate was kept up with by two seeds
+    rates = {}
+    for p in cap["search"]:
+        rates.setdefault(p["offered"], []).append(p["sustained"])
+    assert all(len(v) == 2 and all(v) for r, v in rates.items() if r <= cap["req_per_s"])
