@@ -3404,6 +3404,8 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
     hessenberg_blocked_kernel<12, 4><<<batch, 256, pan12, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
   else if (blockedNB == 12)
     hessenberg_blocked_kernel<12, 12><<<batch, 256, pan12, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
+  else if (blockedNB == 8 && n <= 257)
+    hessenberg_blocked_kernel<8, 4><<<batch, 256, pan8, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
   else if (blockedNB == 8)
     hessenberg_blocked_kernel<8, 12><<<batch, 256, pan8, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
   else
