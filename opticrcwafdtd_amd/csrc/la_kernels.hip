@@ -2446,13 +2446,10 @@ ORF_DEV int win_schur(int m, cplx* T, cplx* U, int lane) {
     else if (its == 20) sig = cadd(cmk(0.75 * cabs1(T[hi + LDW * (hi - 1)]), 0.0), T[hi + LDW * hi]);
     else sig = win_wilkinson<LDW>(T, hi);
     if (!(isfinite(sig.x) && isfinite(sig.y))) return 1;
+    double c;
+    cplx sn, r;
+    givens(csub(T[lo + LDW * lo], sig), T[(lo + 1) + LDW * lo], c, sn, r);
     for (int kk = lo; kk < hi; ++kk) {
-      cplx a, b;
-      if (kk == lo) { a = csub(T[lo + LDW * lo], sig); b = T[(lo + 1) + LDW * lo]; }
-      else { a = T[kk + LDW * (kk - 1)]; b = T[(kk + 1) + LDW * (kk - 1)]; }
-      double c;
-      cplx sn, r;
-      givens(a, b, c, sn, r);
       // rows kk, kk+1 (columns kk..m-1) on lanes 0..31, U columns kk, kk+1 on lanes
       // 32..63, as one instruction stream: x1 <- c x1 + g x2, x2 <- c x2 - conj(g) x1
       // with g = s on T's rows and conj(s) on U's columns
@@ -2474,12 +2471,19 @@ ORF_DEV int win_schur(int m, cplx* T, cplx* U, int lane) {
       }
       wave_sync_lds();
       // columns kk, kk+1 for rows 0..min(kk+2, hi)
+      cplx n1 = cmk(0.0, 0.0);  // this lane's new T(q, kk)
       if (lane <= min(kk + 2, hi)) {
         const int q = lane;
         const cplx h1 = T[q + LDW * kk], h2 = T[q + LDW * (kk + 1)];
-        T[q + LDW * kk] = cadd(cscale(h1, c), cmulc(sn, h2));
+        n1 = cadd(cscale(h1, c), cmulc(sn, h2));
+        T[q + LDW * kk] = n1;
         T[q + LDW * (kk + 1)] = csub(cscale(h2, c), cmul(sn, h1));
       }
+      // the next rotation acts on T(kk+1, kk) and T(kk+2, kk): read out of the lanes
+      // that just formed them, so its Givens overlaps this phase's LDS round trip
+      if (kk + 1 < hi)
+        givens(cmk(lane_value(n1.x, kk + 1), lane_value(n1.y, kk + 1)),
+               cmk(lane_value(n1.x, kk + 2), lane_value(n1.y, kk + 2)), c, sn, r);
       wave_sync_lds();
     }
   }
