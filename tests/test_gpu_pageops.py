@@ -75,7 +75,8 @@ def test_pagemtimes_broadcast():
     np.testing.assert_allclose(np.transpose(Cf, (0, 2, 1)), A @ B, atol=1e-11)
 
 
-@pytest.mark.parametrize("n,nrhs,batch", [(5, 3, 4), (61, 122, 3), (98, 294, 2), (242, 242, 2)])
+@pytest.mark.parametrize("n,nrhs,batch", [(5, 3, 4), (61, 122, 3), (98, 294, 2), (242, 242, 2), (300, 7, 2),
+                                           (600, 3, 1)])
 def test_pagemldivide_and_mrdivide(n, nrhs, batch):
     rng = np.random.default_rng(3)
     A = _crand(rng, batch, n, n) + 2 * np.sqrt(n) * np.eye(n)
@@ -92,6 +93,21 @@ def test_pagemldivide_and_mrdivide(n, nrhs, batch):
     assert rc == 0, _lib.last_error()
     ref = np.swapaxes(np.linalg.solve(np.swapaxes(A, -1, -2), np.swapaxes(Br, -1, -2)), -1, -2)
     np.testing.assert_allclose(np.transpose(Xr, (0, 2, 1)), ref, atol=1e-10)
+
+
+@pytest.mark.parametrize("n,batch", [(40, 3), (242, 4), (300, 2)])
+def test_pagemldivide_general_matrices_pivot(n, batch):
+    """No diagonal dominance: nearly every column of the factorization takes an
+    interchange (panels of <= 256 rows factor in registers, taller ones in LDS)."""
+    rng = np.random.default_rng(11)
+    A = _crand(rng, batch, n, n)
+    B = _crand(rng, batch, n, 5)
+    X = np.zeros((batch, 5, n), complex)
+    lib = _lib.load_library()
+    rc = lib.orf_pagemldivide(_lib.context(), n, 5, _lib.dptr(_f(A)), n * n, _lib.dptr(_f(B)), _lib.dptr(X), batch)
+    assert rc == 0, _lib.last_error()
+    ref = np.linalg.solve(A, B)
+    np.testing.assert_allclose(np.transpose(X, (0, 2, 1)), ref, atol=1e-10 * np.abs(ref).max())
 
 
 def test_pagemldivide_needs_pivoting():
