@@ -2314,7 +2314,12 @@ constexpr int MBQ_NB = 8, MBQ_SEG = 16, MBQ_OCC = 2, MBQ_GRP = 8, MBQ_NW = 16;
 // Small matrices: 4 bulges and a 12-wide window.  The 8-bulge kernel's static
 // LDS (40 KB) next to an H held in LDS leaves one workgroup per CU where this
 // one (20 KB) keeps two: 8,192 pages of n = 61 take 42 ms instead of 94.
-constexpr int MBS_NB = 4, MBS_NW = 12, MBS_GRP = 16;
+// The same kernel compiled for three workgroups per CU (168 VGPRs; H and Z stay
+// in global memory, where the pages in flight are L2 / last-level-cache
+// resident) overlaps more pages' early-deflation windows, which at small n are
+// the critical path on one wave: 33 ms for those 8,192 pages (four per CU: 46 ms,
+// spills).  Tuning choice 3.
+constexpr int MBS_NB = 4, MBS_NW = 12, MBS_GRP = 16, MBS_OCC3 = 3;
 
 // Eigenvalues of a small m x m (m <= 8) complex matrix held in LDS (ld 8), by
 // single-shift QR without vectors; run by one thread.
@@ -3496,12 +3501,16 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
     // multishift passes of 8 bulges with a 16-wide window from n = 128 (H no
     // longer fits LDS there), else 4 bulges with a 12-wide window
     bool wide = n >= 128;
+    // 4 bulges at three workgroups per CU: batches that fill the chip
+    bool occ3 = !wide && !few;
     {
-      // 0 per-rotation (H in LDS), 1 multishift 4 bulges, 2 multishift 8 bulges
+      // 0 per-rotation (H in LDS), 1 multishift 4 bulges, 2 multishift 8 bulges,
+      // 3 multishift 4 bulges at three workgroups per CU
       const int tv = orf_tune_value(ORF_TUNE_QR, n, batch);
-      if (tv == 1 || tv == 2) {
+      if (tv >= 1 && tv <= 3) {
         perRotation = false;
         wide = tv == 2;
+        occ3 = tv == 3;
       } else if (tv == 0 && hFits) {
         perRotation = true;
       }
@@ -3517,6 +3526,9 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
       if (wide)
         schur_qr_aed_kernel<MBQ_NB, MBQ_SEG, MBQ_OCC, MBQ_GRP, MBQ_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
             n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn, mode);
+      else if (occ3)  // H and Z in global memory: 20 KB of LDS per workgroup
+        schur_qr_aed_kernel<MBS_NB, MBQ_SEG, MBS_OCC3, MBS_GRP, MBS_NW><<<batch, 256, 0, st>>>(
+            n, A, lda, sA, Zwork, sZ, w, info, 0, 0, mode);
       else
         schur_qr_aed_kernel<MBS_NB, MBQ_SEG, MBQ_OCC, MBS_GRP, MBS_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
             n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn, mode);
