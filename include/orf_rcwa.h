@@ -175,8 +175,8 @@ int orf_fields(orf_ctx* ctx, const orf_problem* p, int32_t nx, const double* x, 
  * (pages >= 512 or fewer), as measured by opticrcwafdtd_amd/autotune.py and
  * stored in opticrcwafdtd_amd/tuning_table.txt ("kind n large_batch value"
  * per line).  Kinds: 0 Schur QR (0 per-rotation with H in LDS, 1 multishift
- * with 4 bulges per pass, 2 multishift with 8 bulges per pass, 3 multishift
- * with 4 bulges at three workgroups per CU),
+ * with 4 bulges per pass, 2 multishift with 8 bulges per pass, 3 / 4 the
+ * same 4 / 8 bulges in two-wave workgroups, six / four per CU),
  * 1 Hessenberg reduction (0 rank-1 updates, 8 or 12 panel width), 2
  * eigenvectors (16 or 32 rows per lane, 0 one wave per vector).  A lookup
  * uses the entry of the largest tuned size <= n; choices a size cannot take

@@ -8,9 +8,9 @@ permittivity grid, the plan's own stage timers):
 
 * Hessenberg reduction: rank-1 updates (0) or 8 / 12-wide panels;
 * Schur QR: per-rotation kernel with H in LDS (0, n <= 101) or multishift
-  passes with early deflation, 4 bulges and a 12-wide window (1; 3: the same
-  at three workgroups per CU, H and Z in global memory) or 8 bulges and a
-  16-wide window (2);
+  passes with early deflation, 4 bulges and a 12-wide window (1) or 8 bulges
+  and a 16-wide window (2), each also in two-wave workgroups with H and Z in
+  global memory, six / four per CU (3 / 4);
 * eigenvectors: 16 / 32 rows per lane (n <= 256 / 512) or one wave per
   vector (0);
 
@@ -42,7 +42,7 @@ LDS_BYTES = 163840          # a workgroup's LDS (160 KiB); the per-rotation QR k
 
 def candidates(kind: int, n: int) -> list:
     if kind == 0:
-        return ([0] if n * n * 16 + 64 <= LDS_BYTES else []) + [1, 3, 2]
+        return ([0] if n * n * 16 + 64 <= LDS_BYTES else []) + [1, 3, 2, 4]
     if kind == 1:
         return [0] + ([12, 8] if n >= 32 else [])
     return ([16] if n <= 256 else []) + ([32] if n <= 512 else []) + [0]
@@ -101,7 +101,7 @@ def write_table(rows, path):
     with open(path, "w") as f:
         f.write("# kind n large_batch value -- written by opticrcwafdtd_amd/autotune.py\n")
         f.write("# kinds: 0 Schur QR (0 per-rotation, 1 multishift 4 bulges, 2 multishift 8 bulges,\n")
-        f.write("#        3 multishift 4 bulges at three workgroups per CU);\n")
+        f.write("#        3 / 4 multishift 4 / 8 bulges in two-wave workgroups);\n")
         f.write("#        1 Hessenberg (0 rank-1, 8 / 12 panel);\n")
         f.write("#        2 eigenvectors (16 / 32 rows per lane, 0 one wave per vector)\n")
         for kind, n, large, best, times in sorted(rows, key=lambda r: (r[0], r[2], r[1])):

@@ -2314,11 +2314,14 @@ constexpr int MBQ_NB = 8, MBQ_SEG = 16, MBQ_OCC = 2, MBQ_GRP = 8, MBQ_NW = 16;
 // Small matrices: 4 bulges and a 12-wide window.  The 8-bulge kernel's static
 // LDS (40 KB) next to an H held in LDS leaves one workgroup per CU where this
 // one (20 KB) keeps two: 8,192 pages of n = 61 take 42 ms instead of 94.
-// The same kernel compiled for three workgroups per CU (168 VGPRs; H and Z stay
-// in global memory, where the pages in flight are L2 / last-level-cache
-// resident) overlaps more pages' early-deflation windows, which at small n are
-// the critical path on one wave: 33 ms for those 8,192 pages (four per CU: 46 ms,
-// spills).  Tuning choice 3.
+// At small n the early-deflation window's serial rotation chain on one wave is
+// the critical path, so pages per CU count more than waves per page: the same
+// kernel compiled for 168 VGPRs and launched as two-wave workgroups (a chase
+// wave and one strip wave; H and Z in global memory, where the pages in flight
+// stay cache resident) runs six pages per CU: 20.5 ms for those 8,192 pages
+// (four-wave workgroups at three per CU: 33 ms; two-wave at 256 VGPRs, four per
+// CU: 24.6 ms; two-wave at 128 VGPRs: 41 ms, spills).  Tuning choice 3; choice 4
+// runs the 8-bulge kernel the same way at four per CU.
 constexpr int MBS_NB = 4, MBS_NW = 12, MBS_GRP = 16, MBS_OCC3 = 3;
 
 // Eigenvalues of a small m x m (m <= 8) complex matrix held in LDS (ld 8), by
@@ -3501,16 +3504,16 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
     // multishift passes of 8 bulges with a 16-wide window from n = 128 (H no
     // longer fits LDS there), else 4 bulges with a 12-wide window
     bool wide = n >= 128;
-    // 4 bulges at three workgroups per CU: batches that fill the chip
-    bool occ3 = !wide && !few;
+    // two-wave workgroups (H and Z in global memory): batches that fill the chip
+    bool twoWave = !wide && !few;
     {
       // 0 per-rotation (H in LDS), 1 multishift 4 bulges, 2 multishift 8 bulges,
-      // 3 multishift 4 bulges at three workgroups per CU
+      // 3 / 4 the same 4 / 8 bulges in two-wave workgroups, six / four per CU
       const int tv = orf_tune_value(ORF_TUNE_QR, n, batch);
-      if (tv >= 1 && tv <= 3) {
+      if (tv >= 1 && tv <= 4) {
         perRotation = false;
-        wide = tv == 2;
-        occ3 = tv == 3;
+        wide = tv == 2 || tv == 4;
+        twoWave = tv >= 3;
       } else if (tv == 0 && hFits) {
         perRotation = true;
       }
@@ -3523,11 +3526,14 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
       const size_t lim2 = lim + 20 * 1024 - stat;
       const int hIn = bytes <= lim2, zIn = 2 * bytes <= lim2;
       const int mode = g_qr_aed == 1 ? 0 : (g_qr_aed == 2 ? 1 : (g_qr_aed == 3 ? 2 : 3));
-      if (wide)
+      if (wide && twoWave)  // 40 KB of static LDS: four workgroups per CU
+        schur_qr_aed_kernel<MBQ_NB, MBQ_SEG, MBQ_OCC, MBQ_GRP, MBQ_NW><<<batch, 128, 0, st>>>(
+            n, A, lda, sA, Zwork, sZ, w, info, 0, 0, mode);
+      else if (wide)
         schur_qr_aed_kernel<MBQ_NB, MBQ_SEG, MBQ_OCC, MBQ_GRP, MBQ_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
             n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn, mode);
-      else if (occ3)  // H and Z in global memory: 20 KB of LDS per workgroup
-        schur_qr_aed_kernel<MBS_NB, MBQ_SEG, MBS_OCC3, MBS_GRP, MBS_NW><<<batch, 256, 0, st>>>(
+      else if (twoWave)  // 168 VGPRs and 21 KB of LDS: six workgroups per CU
+        schur_qr_aed_kernel<MBS_NB, MBQ_SEG, MBS_OCC3, MBS_GRP, MBS_NW><<<batch, 128, 0, st>>>(
             n, A, lda, sA, Zwork, sZ, w, info, 0, 0, mode);
       else
         schur_qr_aed_kernel<MBS_NB, MBQ_SEG, MBQ_OCC, MBS_GRP, MBS_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
