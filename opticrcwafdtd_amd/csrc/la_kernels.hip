@@ -2320,9 +2320,12 @@ constexpr int MBQ_NB = 8, MBQ_SEG = 16, MBQ_OCC = 2, MBQ_GRP = 8, MBQ_NW = 16;
 // wave and one strip wave; H and Z in global memory, where the pages in flight
 // stay cache resident) runs six pages per CU: 20.5 ms for those 8,192 pages
 // (four-wave workgroups at three per CU: 33 ms; two-wave at 256 VGPRs, four per
-// CU: 24.6 ms; two-wave at 128 VGPRs: 41 ms, spills).  Tuning choice 3; choice 4
-// runs the 8-bulge kernel the same way at four per CU.
-constexpr int MBS_NB = 4, MBS_NW = 12, MBS_GRP = 16, MBS_OCC3 = 3;
+// CU: 24.6 ms; two-wave at 128 VGPRs: 41 ms, spills).  With six pages per CU
+// a narrower early-deflation window shortens that chain more than the extra
+// windows cost: 8 wide, 17.5 ms (6: 16.9, 10: 19.3).  Tuning choice 3; choice 4
+// runs the 8-bulge kernel the same way at four per CU (its 16-wide window stays:
+// 12 / 14 wide, 153 / 148 vs 135-146 ms).
+constexpr int MBS_NB = 4, MBS_NW = 12, MBS_GRP = 16, MBS_OCC3 = 3, MBS_NW2 = 8;
 
 // Eigenvalues of a small m x m (m <= 8) complex matrix held in LDS (ld 8), by
 // single-shift QR without vectors; run by one thread.
@@ -3533,7 +3536,7 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
         schur_qr_aed_kernel<MBQ_NB, MBQ_SEG, MBQ_OCC, MBQ_GRP, MBQ_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
             n, A, lda, sA, Zwork, sZ, w, info, hIn, zIn, mode);
       else if (twoWave)  // 168 VGPRs and 21 KB of LDS: six workgroups per CU
-        schur_qr_aed_kernel<MBS_NB, MBQ_SEG, MBS_OCC3, MBS_GRP, MBS_NW><<<batch, 128, 0, st>>>(
+        schur_qr_aed_kernel<MBS_NB, MBQ_SEG, MBS_OCC3, MBS_GRP, MBS_NW2><<<batch, 128, 0, st>>>(
             n, A, lda, sA, Zwork, sZ, w, info, 0, 0, mode);
       else
         schur_qr_aed_kernel<MBS_NB, MBQ_SEG, MBQ_OCC, MBS_GRP, MBS_NW><<<batch, 256, (hIn + zIn) * bytes, st>>>(
