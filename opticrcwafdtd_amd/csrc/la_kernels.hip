@@ -1546,7 +1546,7 @@ __global__ __launch_bounds__(256, 3) void hessenberg_blocked_kernel(int n, cplx*
   __shared__ cplx sw[NB], sw2[NB];
   __shared__ double red[4];
   const int page = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
-  const int lane = t & 63, w = t >> 6;
+  const int lane = t & 63, w = t >> 6, nw = nt >> 6;
   cplx* A = Ag + (size_t)page * sA;
   cplx* V = Vg + (size_t)page * n * n;
   cplx* tau = taug + (size_t)page * n;
@@ -1565,7 +1565,7 @@ __global__ __launch_bounds__(256, 3) void hessenberg_blocked_kernel(int n, cplx*
       for (int r = k0 + 1 + t; r < n; r += nt) bv[r] = ydot_sub(Y, n, r, i, AA(r, c), sw);
       __syncthreads();
       if (i > 0) {
-        for (int j = w; j < i; j += 4) {  // w_j = V(:, j)^H b, a wave per reflector
+        for (int j = w; j < i; j += nw) {  // w_j = V(:, j)^H b, a wave per reflector
           double sr = 0.0, si = 0.0;
           for (int r = k0 + j + 1 + lane; r < n; r += 64) {
             const cplx p = cmulc(VS(r, j), bv[r]);
@@ -1617,7 +1617,7 @@ __global__ __launch_bounds__(256, 3) void hessenberg_blocked_kernel(int n, cplx*
       if (t == 0) tau[c] = tk;
       __syncthreads();
       // (c) Y(:, i) = tau (A v - Y (V^H v)) on rows k0+1..; T(:, i) = -tau T (V^H v), T(i, i) = tau
-      for (int j = w; j < i; j += 4) {
+      for (int j = w; j < i; j += nw) {
         double sr = 0.0, si = 0.0;
         for (int r = c + 1 + lane; r < n; r += 64) {
           const cplx p = cmulc(VS(r, j), bv[r]);
@@ -1735,7 +1735,7 @@ __global__ __launch_bounds__(256, 3) void hessenberg_blocked_kernel(int n, cplx*
     // column held in SL 64-row slots (SL sized to n: idle slots cost registers)
     {
       cplx x[SL];
-      for (int q = q0 + w; q < n; q += 4) {
+      for (int q = q0 + w; q < n; q += nw) {
 #pragma unroll
         for (int sl = 0; sl < SL; ++sl) {
           const int r = k0 + 1 + lane + 64 * sl;
@@ -3415,7 +3415,11 @@ void orf_zgeev_timed(hipStream_t st, int n, cplx* A, int lda, long long sA, cplx
     const int tv = orf_tune_value(ORF_TUNE_HESS, n, batch);
     if (tv == 0 || (tv == 12 && ok12) || (tv == 8 && ok8)) blockedNB = tv;
   }
-  if (blockedNB == 12 && n <= 257)
+  // batches that fill the chip at small n: two-wave workgroups, six pages per CU
+  // instead of three (8,192 pages of n = 61, reduction and Q: 5.2 vs 6.6 ms)
+  if (blockedNB == 12 && n <= 128 && batch >= 512)
+    hessenberg_blocked_kernel<12, 4><<<batch, 128, pan12, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
+  else if (blockedNB == 12 && n <= 257)
     hessenberg_blocked_kernel<12, 4><<<batch, 256, pan12, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
   else if (blockedNB == 12)
     hessenberg_blocked_kernel<12, 12><<<batch, 256, pan12, st>>>(n, A, lda, sA, Vh, tau, Zwork, sZ);
