@@ -2297,8 +2297,10 @@ __global__ __launch_bounds__(256) void schur_qr_kernel(int n, cplx* __restrict__
 
 
 // Multishift configuration: bulges per pass, bulge positions per segment, the
-// waves per SIMD the register budget is cut for (two QR workgroups per CU hide
-// the latency of the dependent rotation chain), rotations between compiler
+// waves per SIMD the register budget is cut for (two per SIMD: 256 VGPRs, so two
+// four-wave or four two-wave workgroups per CU hide the latency of the
+// dependent rotation chains; batches that fill the chip take the two-wave
+// launch, 139-141 vs 158-167 ms on the batch below), rotations between compiler
 // barriers in the strip updates, and the early-deflation window.  Measured on
 // the +-5 phase-library batch (n = 242, 2048 pages): 8 bulges (8 lanes per
 // bulge in the chase) with a 16-wide window: 187 ms, against 208 ms for 4
